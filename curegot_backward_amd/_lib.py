@@ -1,0 +1,122 @@
+"""ctypes binding of libcuregot_amd.so (the C ABI declared in include/curegot_amd.h).
+
+The library is the only compute path: if it is missing or fails to load,
+importing the solver raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuregot_amd.so")
+
+_dp = ctypes.c_void_p   # device pointers are passed as integers
+
+
+class SplrProblem(ctypes.Structure):
+    _fields_ = [
+        ("M", _dp), ("a", _dp), ("b", _dp),
+        ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+        ("reg", ctypes.c_double), ("tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int32),
+        ("density_max", ctypes.c_double), ("shift_max", ctypes.c_double),
+        ("sparsity_pattern_cycle", ctypes.c_int32), ("candidate_sinkhorn_iter", ctypes.c_int32),
+        ("x0", _dp), ("plan", _dp), ("dual", _dp),
+    ]
+
+
+class SplrInfo(ctypes.Structure):
+    _fields_ = [
+        ("niter", ctypes.c_int32), ("converged", ctypes.c_int32), ("valid", ctypes.c_int32),
+        ("low_rank", ctypes.c_int32), ("fallback", ctypes.c_int32),
+        ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+        ("ys", ctypes.c_double), ("f", ctypes.c_double), ("gnorm", ctypes.c_double),
+        ("K", ctypes.c_int64),
+        ("inv_diag", _dp), ("s", _dp), ("y", _dp), ("u", _dp), ("gamma", _dp),
+    ]
+
+
+class AdjointProblem(ctypes.Structure):
+    _fields_ = [
+        ("M", _dp), ("dual", _dp), ("G", _dp),
+        ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+        ("reg", ctypes.c_double),
+        ("inv_diag", _dp),
+        ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+        ("lam", _dp), ("grad", _dp),
+    ]
+
+
+class AdjointInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+class BcdProblem(ctypes.Structure):
+    _fields_ = [
+        ("M", _dp), ("a", _dp), ("b", _dp),
+        ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+        ("reg", ctypes.c_double), ("tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int32),
+        ("x0", _dp), ("plan", _dp), ("dual", _dp),
+    ]
+
+
+EXPORTS = (
+    "cgot_splr_workspace_size", "cgot_splr_solve",
+    "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
+    "cgot_bcd_workspace_size", "cgot_bcd_solve",
+    "cgot_lse_workspace_size", "cgot_lse_update",
+    "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
+    "cgot_last_error", "cgot_version",
+)
+
+_lib = None
+
+
+def load():
+    """Load libcuregot_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"curegot_backward_amd: {LIB_PATH} not found; build it with "
+            "`python -m curegot_backward_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    sz, i32, i64, dbl, vp = ctypes.c_size_t, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+    L.cgot_splr_workspace_size.argtypes = [i32, ctypes.POINTER(SplrProblem)]
+    L.cgot_splr_workspace_size.restype = sz
+    L.cgot_splr_solve.argtypes = [i32, ctypes.POINTER(SplrProblem), vp, sz, vp, ctypes.POINTER(SplrInfo)]
+    L.cgot_splr_solve.restype = ctypes.c_int
+    L.cgot_adjoint_workspace_size.argtypes = [i32, ctypes.POINTER(AdjointProblem)]
+    L.cgot_adjoint_workspace_size.restype = sz
+    L.cgot_adjoint_solve.argtypes = [i32, ctypes.POINTER(AdjointProblem), vp, sz, vp,
+                                     ctypes.POINTER(AdjointInfo)]
+    L.cgot_adjoint_solve.restype = ctypes.c_int
+    L.cgot_bcd_workspace_size.argtypes = [i32, ctypes.POINTER(BcdProblem)]
+    L.cgot_bcd_workspace_size.restype = sz
+    L.cgot_bcd_solve.argtypes = [i32, ctypes.POINTER(BcdProblem), vp, sz, vp, ctypes.POINTER(i32)]
+    L.cgot_bcd_solve.restype = ctypes.c_int
+    L.cgot_lse_workspace_size.argtypes = [i32, i32]
+    L.cgot_lse_workspace_size.restype = sz
+    L.cgot_lse_update.argtypes = [vp, vp, vp, i32, i32, dbl, i32, i32, vp, vp, sz, vp]
+    L.cgot_lse_update.restype = ctypes.c_int
+    L.cgot_T_sparsify_workspace_size.argtypes = [i32, i32, i64]
+    L.cgot_T_sparsify_workspace_size.restype = sz
+    L.cgot_T_sparsify.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, i64, vp, vp, vp, vp, vp, vp, vp,
+                                  vp, vp, vp, sz, vp]
+    L.cgot_T_sparsify.restype = ctypes.c_int
+    L.cgot_last_error.restype = ctypes.c_char_p
+    L.cgot_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().cgot_last_error().decode(errors="replace")
+        if rc == -2:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what}: {msg}")

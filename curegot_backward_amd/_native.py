@@ -1,0 +1,387 @@
+"""Native-layer mirror: the functions the reference exports from its pybind
+module ``curegot._internal_torch.torch`` (src/bindings.cpp:90-108,
+src/solvers_torch.cpp), implemented over the C ABI of libcuregot_amd.so.
+
+Tensors are plumbing: torch provides device memory (inputs, outputs and the
+solver workspace, taken from torch's caching allocator) and the current HIP
+stream; all compute runs in the library's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Sequence
+
+import torch
+
+from . import _lib
+
+__all__ = [
+    "sinkhorn_splr_bwd",
+    "sinkhorn_splr_bwd_batch",
+    "sinkhorn_splr",
+    "sinkhorn_bcd",
+    "SPLRBackwardContext",
+]
+
+
+def _stream(device) -> int:
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _to_dev(t: torch.Tensor, device) -> torch.Tensor:
+    return t.to(device=device, dtype=torch.float64).contiguous()
+
+
+def _target_device(M: torch.Tensor):
+    # reference: solvers_torch.cpp:51-65 (keep a CUDA input's device, else the default device)
+    if M.is_cuda:
+        return M.device
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _workspace(nbytes: int, device) -> torch.Tensor:
+    if nbytes == 0:
+        raise RuntimeError("curegot: " + _lib.load().cgot_last_error().decode())
+    return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+
+
+class SPLRBackwardContext:
+    """Backward context of one converged SPLR solve.
+
+    Mirrors the reference class (src/backward_context.h:83-376;
+    binding src/solvers_torch.cpp:650-704).  It keeps the converged duals and
+    a view of the solver workspace holding the matrix A_k of the last completed
+    iteration (its diagonal, the secant pair s, y and u = A_k^{-1} y); the plan
+    is not stored: the adjoint recomputes T from M and the duals.
+    """
+
+    def __init__(self, M, dual, reg, info: _lib.SplrInfo, workspace: torch.Tensor):
+        self._M = M
+        self._dual = dual
+        self._ws = workspace
+        self._reg = float(reg)
+        self._n = int(info.n)
+        self._m = int(info.m)
+        self._low_rank = bool(info.low_rank)
+        self._fallback = bool(info.fallback)
+        self._ys = float(info.ys)
+        N = self._n + self._m - 1
+        self._inv_diag = self._view(info.inv_diag, N)
+        self._s = self._view(info.s, N)
+        self._y = self._view(info.y, N)
+        self._u = self._view(info.u, N)
+
+    def _view(self, ptr, count) -> torch.Tensor:
+        off = int(ptr) - self._ws.data_ptr()
+        if off < 0 or off + 8 * count > self._ws.numel():
+            raise RuntimeError("SPLRBackwardContext: context pointer outside the workspace")
+        return self._ws[off:off + 8 * count].view(torch.float64)
+
+    # ---- metadata (reference solvers_torch.cpp:650-704) ----
+    @property
+    def reg(self) -> float:
+        return self._reg
+
+    @property
+    def dim(self) -> int:
+        return self._n + self._m - 1
+
+    @property
+    def low_rank_enabled(self) -> bool:
+        return self._low_rank
+
+    @property
+    def is_fallback(self) -> bool:
+        return self._fallback
+
+    @property
+    def adopted_forward_state(self) -> bool:
+        return not self._fallback
+
+    @property
+    def ys(self) -> float:
+        return self._ys
+
+    @property
+    def has_diagonal_preconditioner(self) -> bool:
+        return True
+
+    @property
+    def M(self) -> torch.Tensor:
+        return self._M
+
+    @property
+    def dual(self) -> torch.Tensor:
+        return self._dual
+
+    # ---- validation copies ----
+    def inv_diag(self) -> torch.Tensor:
+        return self._inv_diag.clone()
+
+    def secant_s(self) -> torch.Tensor:
+        return self._s.clone()
+
+    def secant_y(self) -> torch.Tensor:
+        return self._y.clone()
+
+    def ctx_point(self) -> torch.Tensor:
+        return self._dual[: self.dim].clone()
+
+    def _check_r(self, r: torch.Tensor):
+        if not isinstance(r, torch.Tensor) or not r.is_cuda:
+            raise RuntimeError("SPLRBackwardContext: r must be a CUDA tensor")
+        if r.dim() != 1 or r.shape[0] != self.dim:
+            raise RuntimeError(f"SPLRBackwardContext: r must have length {self.dim}")
+
+    # ---- production methods ----
+    def apply_diagonal_preconditioner(self, r: torch.Tensor) -> torch.Tensor:
+        """z = diag(A_k)^{-1} r (reference backward_context.cu:642-655)."""
+        self._check_r(r)
+        return r.to(torch.float64) * self._inv_diag
+
+    def inv_diag_ptr(self) -> int:
+        return self._inv_diag.data_ptr()
+
+
+def _kwargs_density(kwargs, n, m):
+    # reference defaults: solvers_torch.cpp:783-838
+    dmax = min(10.0 / min(n, m), 1.0)
+    if "density" in kwargs and kwargs["density"] is not None:
+        dmax = min(max(float(kwargs["density"]), 0.0), 1.0)
+    return dmax
+
+
+def _make_problem(M, a, b, reg, tol, max_iter, kwargs, plan, dual, x0):
+    n, m = M.shape
+    p = _lib.SplrProblem()
+    p.M, p.a, p.b = M.data_ptr(), a.data_ptr(), b.data_ptr()
+    p.n, p.m = n, m
+    p.reg, p.tol, p.max_iter = float(reg), float(tol), int(max_iter)
+    p.density_max = _kwargs_density(kwargs, n, m)
+    if p.density_max <= 0.0:
+        p.density_max = 1e-300     # explicit 0 keeps K = 1, as the reference's clamp does
+    p.shift_max = max(float(kwargs.get("shift", 1e-3)), 0.0)
+    p.sparsity_pattern_cycle = int(kwargs.get("sparsity_pattern_cycle", 30))
+    if p.sparsity_pattern_cycle <= 0:
+        raise RuntimeError("sparsity_pattern_cycle must be positive")
+    p.candidate_sinkhorn_iter = max(int(kwargs.get("candidate_sinkhorn_iter", 3)), 0)
+    p.x0 = x0.data_ptr() if x0 is not None else None
+    p.plan = plan.data_ptr()
+    p.dual = dual.data_ptr()
+    return p
+
+
+def _check_shapes(M, a, b):
+    if M.dim() != 2:
+        raise RuntimeError("M must be a 2D tensor")
+    if a.dim() != 1 or b.dim() != 1:
+        raise RuntimeError("a and b must be 1D tensors")
+    n, m = M.shape
+    if a.shape[0] != n or b.shape[0] != m:
+        raise RuntimeError("Shape mismatch: M is [n x m], a should be length n, b should be length m")
+
+
+def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tensor],
+                            bs: Sequence[torch.Tensor], reg, tol=1e-6, max_iter=1000, verbose=0,
+                            **kwargs):
+    """Ragged batch of backward-capable SPLR solves in one native call.
+
+    Ms/as_/bs are sequences of [n_i, m_i], [n_i], [m_i] tensors (shapes may
+    differ); reg may be a float or one value per instance.  Returns one result
+    dict per instance, each like sinkhorn_splr_bwd's.  Every instance's result
+    is bitwise the same as solving it alone.
+    """
+    B = len(Ms)
+    if B == 0:
+        return []
+    if len(as_) != B or len(bs) != B:
+        raise RuntimeError("Ms, as_ and bs must have the same length")
+    regs = [float(r) for r in reg] if isinstance(reg, (list, tuple)) else [float(reg)] * B
+    if len(regs) != B:
+        raise RuntimeError("reg must be a float or have one value per instance")
+    ctx_mode = kwargs.get("context_mode", "adopt")
+    if ctx_mode not in ("adopt", "snapshot"):
+        raise RuntimeError('context_mode must be "adopt" or "snapshot"')
+    device = _target_device(Ms[0])
+    x0s = kwargs.get("x0", None)
+    L = _lib.load()
+    probs = (_lib.SplrProblem * B)()
+    keep = []
+    outs = []
+    for i in range(B):
+        _check_shapes(Ms[i], as_[i], bs[i])
+        M = _to_dev(Ms[i], device)
+        a = _to_dev(as_[i], device)
+        b = _to_dev(bs[i], device)
+        n, m = M.shape
+        x0 = None
+        if x0s is not None:
+            x0 = x0s[i] if isinstance(x0s, (list, tuple)) else (x0s if B == 1 else x0s[i])
+            if x0.dim() != 1 or x0.shape[0] != n + m:
+                raise RuntimeError("x0 must be a 1D tensor of length n + m")
+            x0 = _to_dev(x0, device)
+        plan = torch.empty((n, m), dtype=torch.float64, device=device)
+        dual = torch.empty(n + m, dtype=torch.float64, device=device)
+        probs[i] = _make_problem(M, a, b, regs[i], tol, max_iter, kwargs, plan, dual, x0)
+        keep.append((M, a, b, x0))
+        outs.append((plan, dual))
+    with torch.cuda.device(device):
+        nbytes = L.cgot_splr_workspace_size(B, probs)
+        ws = _workspace(nbytes, device)
+        infos = (_lib.SplrInfo * B)()
+        rc = L.cgot_splr_solve(B, probs, ws.data_ptr(), nbytes, _stream(device), infos)
+        _lib.check(rc, "sinkhorn_splr_bwd")
+    results = []
+    for i in range(B):
+        info = infos[i]
+        plan, dual = outs[i]
+        if not info.valid:
+            raise RuntimeError(
+                "torch_sinkhorn_splr_bwd: forward pass failed (non-finite objective value); "
+                "no backward context available")
+        ctx = SPLRBackwardContext(keep[i][0], dual, regs[i], info, ws)
+        src = Ms[i]
+        res = {
+            "niter": int(info.niter),
+            "converged": bool(info.converged),
+            "plan": plan if (src.is_cuda and src.dtype == torch.float64) else plan.to(src.device, src.dtype),
+            "dual": dual if (src.is_cuda and src.dtype == torch.float64) else dual.to(src.device, src.dtype),
+            "context": ctx,
+        }
+        if verbose > 0:
+            print(f"sinkhorn_splr_bwd (HIP) [{info.n} x {info.m}] niter={info.niter} "
+                  f"converged={bool(info.converged)} gnorm={info.gnorm:.3e}")
+        results.append(res)
+    return results
+
+
+def sinkhorn_splr_bwd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Backward-capable SPLR forward.  Reference: torch_sinkhorn_splr_bwd,
+    src/solvers_torch.cpp:707-912.  Returns {niter, converged, plan, dual, context}."""
+    _check_shapes(M, a, b)
+    if "x0" in kwargs and kwargs["x0"] is not None:
+        kwargs = dict(kwargs)
+        kwargs["x0"] = [kwargs["x0"]]
+    return sinkhorn_splr_bwd_batch([M], [a], [b], reg, tol, max_iter, verbose, **kwargs)[0]
+
+
+def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Forward-only SPLR (reference torch_sinkhorn_splr, src/solvers_torch.cpp:130-288):
+    {niter, plan, dual}."""
+    out = sinkhorn_splr_bwd(M, a, b, reg, tol, max_iter, verbose, **kwargs)
+    return {"niter": out["niter"], "plan": out["plan"], "dual": out["dual"]}
+
+
+def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Sinkhorn/BCD solve (reference torch_sinkhorn_bcd, src/solvers_torch.cpp:20-127):
+    {niter, plan, dual}; the dual is not gauge-fixed."""
+    _check_shapes(M, a, b)
+    device = _target_device(M)
+    Md, ad, bd = _to_dev(M, device), _to_dev(a, device), _to_dev(b, device)
+    n, m = Md.shape
+    x0 = kwargs.get("x0", None)
+    if x0 is not None:
+        if x0.dim() != 1 or x0.shape[0] != n + m:
+            raise RuntimeError("x0 must be a 1D tensor of length n + m")
+        x0 = _to_dev(x0, device)
+    plan = torch.empty((n, m), dtype=torch.float64, device=device)
+    dual = torch.empty(n + m, dtype=torch.float64, device=device)
+    p = _lib.BcdProblem()
+    p.M, p.a, p.b = Md.data_ptr(), ad.data_ptr(), bd.data_ptr()
+    p.n, p.m, p.reg, p.tol, p.max_iter = n, m, float(reg), float(tol), int(max_iter)
+    p.x0 = x0.data_ptr() if x0 is not None else None
+    p.plan, p.dual = plan.data_ptr(), dual.data_ptr()
+    L = _lib.load()
+    niter = ctypes.c_int32(0)
+    with torch.cuda.device(device):
+        nbytes = L.cgot_bcd_workspace_size(1, ctypes.byref(p))
+        ws = _workspace(nbytes, device)
+        rc = L.cgot_bcd_solve(1, ctypes.byref(p), ws.data_ptr(), nbytes, _stream(device), ctypes.byref(niter))
+        _lib.check(rc, "sinkhorn_bcd")
+    if not (M.is_cuda and M.dtype == torch.float64):
+        plan, dual = plan.to(M.device, M.dtype), dual.to(M.device, M.dtype)
+    if verbose > 0:
+        print(f"sinkhorn_bcd (HIP) [{n} x {m}] niter={niter.value}")
+    return {"niter": int(niter.value), "plan": plan, "dual": dual}
+
+
+def adjoint_batch(items, tol, max_iter):
+    """Batched implicit backward.  items: list of (M, dual, G, reg, inv_diag or None).
+    Returns (lambdas, grads, infos)."""
+    B = len(items)
+    device = items[0][0].device
+    probs = (_lib.AdjointProblem * B)()
+    lams, grads, keep = [], [], []
+    for i, (M, dual, G, reg, inv) in enumerate(items):
+        n, m = M.shape
+        Gd = G.to(device=device, dtype=torch.float64).contiguous()
+        lam = torch.empty(n + m - 1, dtype=torch.float64, device=device)
+        grad = torch.empty((n, m), dtype=torch.float64, device=device)
+        p = probs[i]
+        p.M, p.dual, p.G = M.data_ptr(), dual.data_ptr(), Gd.data_ptr()
+        p.n, p.m, p.reg = n, m, float(reg)
+        p.inv_diag = inv.data_ptr() if inv is not None else None
+        p.tol, p.max_iter = float(tol), int(max_iter)
+        p.lam, p.grad = lam.data_ptr(), grad.data_ptr()
+        keep.append(Gd)
+        lams.append(lam)
+        grads.append(grad)
+    L = _lib.load()
+    infos = (_lib.AdjointInfo * B)()
+    with torch.cuda.device(device):
+        nbytes = L.cgot_adjoint_workspace_size(B, probs)
+        ws = _workspace(nbytes, device)
+        rc = L.cgot_adjoint_solve(B, probs, ws.data_ptr(), nbytes, _stream(device), infos)
+        _lib.check(rc, "solve_backward")
+    return lams, grads, [infos[i] for i in range(B)]
+
+
+def lse_update(M, a, b, dual, reg, which, repeats=1):
+    """One (or `repeats`) BCD half-sweep(s) in place on dual (device tensors)."""
+    n, m = M.shape
+    L = _lib.load()
+    with torch.cuda.device(M.device):
+        nbytes = L.cgot_lse_workspace_size(n, m)
+        ws = _workspace(nbytes, M.device)
+        rc = L.cgot_lse_update(M.data_ptr(), a.data_ptr(), b.data_ptr(), n, m, float(reg), int(which),
+                               int(repeats), dual.data_ptr(), ws.data_ptr(), nbytes, _stream(M.device))
+        _lib.check(rc, "lse_update")
+    return dual
+
+
+def T_sparsify(alpha, beta, M, a, b, reg, shift, K, rhs=None):
+    """Device T pass + top-K sparsified Hessian (test hook; reference
+    curegot.tests.test_T_computation_sparsify, src/tests.cpp:36-150)."""
+    device = _target_device(M)
+    Md = _to_dev(M, device)
+    n, m = Md.shape
+    gamma = _to_dev(torch.cat([torch.as_tensor(alpha).flatten(), torch.as_tensor(beta).flatten()]), device)
+    ad, bd = _to_dev(torch.as_tensor(a), device), _to_dev(torch.as_tensor(b), device)
+    N = n + m - 1
+    Te = n * (m - 1)
+    Kp = min(max(int(K), 1), Te) if Te > 0 else 0
+    f64 = dict(dtype=torch.float64, device=device)
+    rows, cols, grad = torch.empty(n, **f64), torch.empty(m, **f64), torch.empty(N, **f64)
+    scal = torch.empty(2, **f64)
+    vals = torch.empty(Kp + N, **f64)
+    colind = torch.empty(Kp + N, dtype=torch.int32, device=device)
+    rowptr = torch.empty(N + 1, dtype=torch.int32, device=device)
+    rhs_d = _to_dev(torch.as_tensor(rhs), device) if rhs is not None else None
+    sol = torch.empty(N, **f64) if rhs is not None else None
+    L = _lib.load()
+    with torch.cuda.device(device):
+        nbytes = L.cgot_T_sparsify_workspace_size(n, m, int(K))
+        ws = _workspace(nbytes, device)
+        rc = L.cgot_T_sparsify(gamma.data_ptr(), Md.data_ptr(), ad.data_ptr(), bd.data_ptr(), n, m,
+                               float(reg), float(shift), int(K), rows.data_ptr(), cols.data_ptr(),
+                               grad.data_ptr(), scal.data_ptr(), vals.data_ptr(), colind.data_ptr(),
+                               rowptr.data_ptr(), rhs_d.data_ptr() if rhs_d is not None else None,
+                               sol.data_ptr() if sol is not None else None, ws.data_ptr(), nbytes,
+                               _stream(device))
+        _lib.check(rc, "T_sparsify")
+    out = {"Trowsums": rows, "Tcolsums": cols, "grad": grad, "objfn": float(scal[0]),
+           "gnorm": float(scal[1]), "csr_val": vals, "csr_colind": colind, "csr_rowptr": rowptr}
+    if sol is not None:
+        out["sol"] = sol
+    return out

@@ -1,0 +1,94 @@
+"""Production backward (implicit gradient) of the SPLR solver.
+
+Mirrors curegot.backward (reference curegot/backward.py:46-144): given the
+upstream gradient G = dL/dT and the backward context of a converged forward,
+solve the reduced-Hessian system
+
+    Q(T*) lambda = mu,      mu = [(G.T*) 1_m, ((G.T*)^T 1_n)[:m-1]]
+
+by preconditioned CG and assemble dL/dM = (1/eta) T*.(lambda_a 1^T + 1 lambda_b^T - G).
+
+The whole PCG runs in the library's HIP kernels (one pass over M per
+iteration, T* recomputed from the context's duals; no stored plan) and the
+batch variant solves many instances in lockstep.
+
+Preconditioners: "diag" (default; 1/diag(A_k) of the context matrix, the
+reference's native diagonal) and "none" (plain CG).  "splr" (A_k^{-1} with the
+rank-2 Woodbury term) is accepted by the reference; here it raises
+NotImplementedError.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _native
+
+__all__ = ["solve_backward", "solve_backward_batch", "select_preconditioner"]
+
+_VALID_PRECONDITIONERS = ("diag", "splr", "none")
+
+
+def select_preconditioner(ctx, preconditioner):
+    """Return the inverse-diagonal tensor for "diag", None for "none".
+    Reference: select_preconditioner, curegot/backward.py:46-69."""
+    if preconditioner == "diag":
+        if not bool(ctx.has_diagonal_preconditioner):
+            raise RuntimeError("SPLRBackwardContext: diagonal preconditioner not set up")
+        return ctx._inv_diag
+    if preconditioner == "none":
+        return None
+    if preconditioner == "splr":
+        raise NotImplementedError(
+            "preconditioner='splr' (factorised A_k + rank-2 Woodbury) is not available in the "
+            "MI355X build; use 'diag' (default) or 'none'")
+    raise ValueError(
+        f"invalid preconditioner '{preconditioner}'; "
+        f"expected one of: {' | '.join(_VALID_PRECONDITIONERS)}"
+    )
+
+
+def _check(T, G):
+    if T is not None:
+        if T.dim() != 2:
+            raise ValueError("T must be a 2D tensor")
+        if G.shape != T.shape:
+            raise ValueError("G must have the same shape as T")
+    elif G.dim() != 2:
+        raise ValueError("G must be a 2D tensor")
+
+
+def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=2000):
+    """items: list of (G, ctx, eta).  Returns one result dict per item."""
+    jobs = []
+    for G, ctx, eta in items:
+        if G.shape != ctx.M.shape:
+            raise ValueError("G must have the same shape as T")
+        inv = select_preconditioner(ctx, preconditioner)
+        jobs.append((ctx.M, ctx.dual, G, float(eta), inv))
+    lams, grads, infos = _native.adjoint_batch(jobs, tol, max_iter)
+    out = []
+    for lam, grad, info, (G, ctx, eta) in zip(lams, grads, infos, items):
+        if info.status == 2:
+            raise RuntimeError("PCG breakdown: p^T A p <= 0 (matrix not SPD?)")
+        out.append({
+            "lambda": lam,
+            "grad": grad.to(G.dtype) if G.dtype != torch.float64 else grad,
+            "iterations": int(info.iterations),
+            "converged": bool(info.converged),
+            "preconditioner": preconditioner,
+        })
+    return out
+
+
+def solve_backward(T, G, ctx, eta, *, preconditioner="diag", tol=1e-12, max_iter=2000):
+    """Solve the backward pass and assemble the implicit gradient.
+
+    Args mirror the reference (curegot/backward.py:72-81).  T is checked for
+    shape only: the plan is recomputed inside the kernels from the context's
+    cost matrix and converged duals (it equals the forward's plan).
+
+    Returns dict(lambda [n+m-1], grad [n, m], iterations, converged, preconditioner).
+    """
+    _check(T, G)
+    return solve_backward_batch([(G, ctx, eta)], preconditioner=preconditioner, tol=tol,
+                                max_iter=max_iter)[0]

@@ -1,0 +1,174 @@
+// Shared definitions for the MI355X (gfx950) entropic-OT kernels.
+//
+// Every kernel works on a ragged batch of OT instances.  An instance b owns a
+// row-major cost matrix M_b [n_b x m_b] (FP64, caller memory) and a slice of
+// one workspace arena (caller memory).  Kernels receive the descriptor table
+// and either a tile->instance map (M-sized passes) or iterate one workgroup
+// range per instance (vector work).  All reductions are fixed-order, so every
+// result depends only on the instance's own data: an instance gives bitwise
+// the same answer alone and inside any batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CG_WG 256          // threads per workgroup (4 wave64)
+#define CG_WAVES 4
+#define CG_CMAX 8          // columns per lane in an M-pass strip
+#define CG_RPW 8           // rows per wave in an M-pass tile
+#define CG_RB (CG_WAVES * CG_RPW)   // rows per M-pass tile
+#define CG_NB 64           // dense Cholesky block size
+
+#define CG_DEV __device__ __forceinline__
+
+// Per-instance descriptor (device resident, written once per solve).
+struct CgInst {
+    const double* M;       // [n*m]
+    int n, m, N;           // N = n+m-1
+    int c;                 // columns per lane (1..CG_CMAX)
+    int sw;                // strip width 64*c
+    int nstrips;           // ceil(m/sw)
+    int nrb;               // ceil(n/CG_RB)
+    int tile0;             // first global tile of this instance
+    int ntiles;            // nrb*nstrips
+    int vec0;              // first vector workgroup (ceil((n+m)/CG_WG) of them)
+    int nvec;
+    int row0;              // first row-wave workgroup (ceil(n/CG_WAVES) of them)
+    int nrow;
+    double reg, inv_reg;
+    // dual / gradient state, all FP64
+    double* ab;            // [n+m]   (a, b)
+    double* logab;         // [n+m]
+    double* gamma;         // [n+m]
+    double* gamma_prev;    // [n+m]
+    double* gamma_bcd;     // [n+m]
+    double* grad;          // [N]
+    double* grad_prev;     // [N]
+    double* grad_bcd;      // [N]
+    double* dir;           // [N]  Newton / quasi-Newton direction
+    double* yv;            // [N]  secant y
+    double* sv;            // [N]  secant s
+    double* uv;            // [N]  A^{-1} y
+    double* rowsum;        // [n]
+    double* colsum;        // [m]
+    double* rp0;           // [n*nstrips] row partials (sum, or LSE max)
+    double* rp1;           // [n*nstrips] row partials (LSE sum)
+    double* cp0;           // [nrb*m]     column partials
+    double* cp1;           // [nrb*m]
+    double* wgpart;        // [4*max(nfin,nvec)] per-workgroup scalar partials
+    unsigned int* counter; // [4] last-workgroup counters
+    double* scal;          // [CG_NSCAL] per-instance scalars
+    // sparsified Hessian (pattern + values), W_ij = T_ij on the pattern
+    int64_t Kmax;
+    int* w_rowptr;         // [n+1]   CSR of W by row i: column j
+    int* w_colj;           // [Kmax]
+    double* w_rval;        // [Kmax]
+    int* w_colptr;         // [m]     CSC of W by column j (m-1 used): row i
+    int* w_rowi;           // [Kmax]
+    double* w_cval;        // [Kmax]
+    double* Da;            // [n]  diag(T1) + shift*reg
+    double* Db;            // [m-1]
+    // top-K selection scratch
+    unsigned int* hist;    // [CG_HBINS]
+    int* rcnt;             // [n]   selected per row
+    int* rtie;             // [n]   ties per row
+    int* rcut;             // [n]   tie cut column per row
+    int* ccnt;             // [nrb*m] selected per (row block, column)
+    // dense Schur complement (row-major P x P, lower triangle used)
+    int P;                 // min(n, m-1) Schur size
+    int beta_side;         // 1: Schur onto beta (P = m-1), 0: onto alpha (P = n)
+    double* S;             // [P*P]
+    double* rhs;           // [2*P] Schur right-hand sides / solutions
+    double* ltmp;          // [2*max(n,m)] Schur solutions
+    double* inv_diag;      // [N] 1/diag(A) of the context matrix
+    // outputs
+    double* plan;          // [n*m] or null
+    double* dual;          // [n+m] or null
+};
+
+// Scalar slots in CgInst::scal
+enum {
+    CS_F = 0,      // objective at the last T pass
+    CS_G2,         // ||g||^2 at the last T pass
+    CS_DG,         // <g, dir> at the last T pass
+    CS_DG0,        // <g_prev, dir>
+    CS_YS, CS_YY,  // secant products
+    CS_LOWRANK,    // 1.0 if the low-rank term is active
+    CS_FBCD, CS_G2BCD,
+    CS_STEP,       // trial step (host written)
+    CS_SHIFT,      // shift*reg for A (host written)
+    CS_KTH_HI,     // top-K threshold: key bits (as two 32-bit halves stored in doubles)
+    CS_KTH_LO,
+    CS_NEED,       // ties still to take
+    CS_KSEL,       // K
+    CS_PIVOT,      // Cholesky failure flag (first failing pivot + 1)
+    CS_SG, CS_YAY, CS_YAG,
+    CS_G2A,        // sum of (T1 - a)^2 at the last T pass
+    // BCD stopping sums (share slots with the PCG block)
+    BCD_SUMS = 24,
+    // backward PCG
+    PCG_STATUS = 24, PCG_BNORM, PCG_RZ, PCG_BETA, PCG_ALPHA, PCG_ITER,
+    CG_NSCAL = 32
+};
+
+// PCG status values (PCG_STATUS)
+#define PCG_RUNNING 0.0
+#define PCG_CONVERGED 1.0
+#define PCG_BREAKDOWN 2.0
+#define PCG_MAXIT 3.0
+#define PCG_ZERO_RHS 4.0
+
+#define CG_HBITS0 12
+#define CG_HBITS 13
+#define CG_HBINS 8192
+
+// ---------------------------------------------------------------------------
+// wave64 / workgroup reductions (fixed order => deterministic)
+// ---------------------------------------------------------------------------
+CG_DEV double cg_wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+CG_DEV double cg_wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum over a 256-thread workgroup; result valid in every thread.
+CG_DEV double cg_block_sum(double v, double* lds4)
+{
+    v = cg_wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double r = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    return r;
+}
+
+CG_DEV double cg_block_max(double v, double* lds4)
+{
+    v = cg_wave_max(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    return fmax(fmax(lds4[0], lds4[1]), fmax(lds4[2], lds4[3]));
+}
+
+// Merge of two (max, sum-of-exp) log-sum-exp partials.
+CG_DEV void cg_lse_merge(double& mx, double& s, double mx2, double s2)
+{
+    if (mx2 == -INFINITY) return;
+    if (mx == -INFINITY) { mx = mx2; s = s2; return; }
+    if (mx2 > mx) { s = s * exp(mx - mx2) + s2; mx = mx2; }
+    else { s = s + s2 * exp(mx2 - mx); }
+}
+
+// Order-preserving key of a non-negative double (T >= 0).
+CG_DEV unsigned long long cg_key(double v) { return (unsigned long long)__double_as_longlong(v); }

@@ -1,0 +1,1336 @@
+// Host driver of the ragged-batch SPLR solver (C ABI in include/curegot_amd.h).
+//
+// One driver serves the plain single solve (B = 1) and the batched solve: all
+// instances advance in lockstep, every launch covers every instance that takes
+// part in that phase, and the host reads the per-instance scalars (objective,
+// gradient norm, directional derivative) once per line-search round for the
+// whole batch.  The per-instance control logic is the reference's, restated:
+//   cuda_sinkhorn_splr_bwd (src/sinkhorn_splr.cu:1305-1611): density/shift
+//   schedules, Sinkhorn candidate every sparsity_pattern_cycle iterations,
+//   low-rank rule, stop-after-step semantics, context export;
+//   SPLRSolver::line_search_wolfe (src/sinkhorn_splr.cu:667-920) as a resumable
+//   state machine (one trial per round).
+// Kernels are deterministic and per-instance, so a result does not depend on
+// the batch it was solved in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cgot_kernels.h"
+#include "../../include/curegot_amd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define CG_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                                     " at " __FILE__ ":" + std::to_string(__LINE__));   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Pinned host staging (per thread, grown on demand, kept for the process).
+// ---------------------------------------------------------------------------
+struct Pinned {
+    double* p = nullptr;
+    size_t cap = 0;
+    double* get(size_t n)
+    {
+        if (n > cap) {
+            // the old buffer may still be read by an in-flight copy: keep it
+            double* q = nullptr;
+            CG_HIP(hipHostMalloc((void**)&q, std::max<size_t>(n, 1024) * sizeof(double), hipHostMallocDefault));
+            p = q;
+            cap = std::max<size_t>(n, 1024);
+        }
+        return p;
+    }
+};
+thread_local Pinned g_ctl_host;
+thread_local Pinned g_scal_host;
+
+// ---------------------------------------------------------------------------
+// Workspace layout: a bump allocator over the caller's bytes.
+// ---------------------------------------------------------------------------
+struct Bump {
+    char* base;
+    size_t off = 0;
+    explicit Bump(void* b) : base((char*)b) {}
+    template <class T>
+    T* take(size_t count)
+    {
+        off = (off + 255) & ~(size_t)255;
+        T* r = base ? (T*)(base + off) : nullptr;
+        off += std::max<size_t>(count, 1) * sizeof(T);
+        return r;
+    }
+};
+
+constexpr int kCtlSlots = 16;
+
+int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Geometry common to every instance kind.
+void set_geometry(CgInst& I, int n, int m, double reg)
+{
+    I.n = n;
+    I.m = m;
+    I.N = n + m - 1;
+    I.c = std::min(CG_CMAX, std::max(1, cdiv(m, 64)));
+    I.sw = 64 * I.c;
+    I.nstrips = cdiv(m, I.sw);
+    I.nrb = cdiv(n, CG_RB);
+    I.ntiles = I.nrb * I.nstrips;
+    I.nvec = cdiv(n + m, CG_WG);
+    I.nrow = cdiv(n, CG_WAVES);
+    I.reg = reg;
+    I.inv_reg = 1.0 / reg;
+}
+
+// Maps and per-launch tables shared by all solvers.
+struct Batch {
+    int B = 0;
+    std::vector<CgInst> h;     // host copy of descriptors
+    CgInst* d_insts = nullptr;
+    int* d_tile = nullptr;
+    int* d_vec = nullptr;
+    int* d_row = nullptr;
+    int ntiles = 0, nvec = 0, nrow = 0;
+    double* d_ctl = nullptr;   // kCtlSlots * B * CG_NCTL
+    double* d_scal = nullptr;  // B * CG_NSCAL
+    int* d_pref = nullptr;     // dense prefix tables
+    int pref_cap = 0;
+    const double** d_ptrs = nullptr;   // 3*B pointers
+    unsigned int* zero_base = nullptr; // counters + histograms of all instances
+    size_t zero_bytes = 0;
+    hipStream_t stream = nullptr;
+    int slot = 0;              // next ctl slot
+    double* hctl = nullptr;    // pinned, kCtlSlots * B * CG_NCTL
+    double* hscal = nullptr;   // pinned, B * CG_NSCAL
+
+    void layout_common(Bump& bp, int nblk_max)
+    {
+        d_insts = bp.take<CgInst>(B);
+        d_tile = bp.take<int>(ntiles);
+        d_vec = bp.take<int>(nvec);
+        d_row = bp.take<int>(nrow);
+        d_ctl = bp.take<double>((size_t)kCtlSlots * B * CG_NCTL);
+        d_scal = bp.take<double>((size_t)B * CG_NSCAL);
+        pref_cap = (3 * nblk_max + 2) * (B + 1);
+        d_pref = bp.take<int>(pref_cap);
+        d_ptrs = bp.take<const double*>(3 * (size_t)B);
+    }
+
+    void assign_maps()
+    {
+        ntiles = nvec = nrow = 0;
+        for (auto& I : h) {
+            I.tile0 = ntiles; ntiles += I.ntiles;
+            I.vec0 = nvec; nvec += I.nvec;
+            I.row0 = nrow; nrow += I.nrow;
+        }
+    }
+
+    void upload_maps(const std::vector<const double*>& ptrs)
+    {
+        std::vector<int> tile(ntiles), vec(nvec), row(nrow);
+        for (int b = 0; b < B; b++) {
+            const CgInst& I = h[b];
+            for (int t = 0; t < I.ntiles; t++) tile[I.tile0 + t] = b;
+            for (int t = 0; t < I.nvec; t++) vec[I.vec0 + t] = b;
+            for (int t = 0; t < I.nrow; t++) row[I.row0 + t] = b;
+            h[b].scal = d_scal + (size_t)b * CG_NSCAL;
+        }
+        CG_HIP(hipMemcpyAsync(d_insts, h.data(), sizeof(CgInst) * B, hipMemcpyHostToDevice, stream));
+        CG_HIP(hipMemcpyAsync(d_tile, tile.data(), sizeof(int) * std::max(ntiles, 1), hipMemcpyHostToDevice, stream));
+        CG_HIP(hipMemcpyAsync(d_vec, vec.data(), sizeof(int) * std::max(nvec, 1), hipMemcpyHostToDevice, stream));
+        CG_HIP(hipMemcpyAsync(d_row, row.data(), sizeof(int) * std::max(nrow, 1), hipMemcpyHostToDevice, stream));
+        if (!ptrs.empty())
+            CG_HIP(hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(void*) * ptrs.size(), hipMemcpyHostToDevice, stream));
+        // pageable sources: make sure the copies are done before the vectors die
+        CG_HIP(hipStreamSynchronize(stream));
+        hctl = g_ctl_host.get((size_t)kCtlSlots * B * CG_NCTL);
+        hscal = g_scal_host.get((size_t)B * CG_NSCAL);
+    }
+
+    // Upload a control block (ctl[b*CG_NCTL + field]) and return its device copy.
+    const double* ctl(const std::vector<double>& c)
+    {
+        if (slot == kCtlSlots) sync();
+        double* hs = hctl + (size_t)slot * B * CG_NCTL;
+        double* ds = d_ctl + (size_t)slot * B * CG_NCTL;
+        std::memcpy(hs, c.data(), sizeof(double) * B * CG_NCTL);
+        CG_HIP(hipMemcpyAsync(ds, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
+        slot++;
+        return ds;
+    }
+
+    void sync()
+    {
+        CG_HIP(hipStreamSynchronize(stream));
+        slot = 0;
+    }
+
+    // Read every instance's scalars (one copy + one sync for the batch).
+    void read_scal()
+    {
+        CG_HIP(hipMemcpyAsync(hscal, d_scal, sizeof(double) * B * CG_NSCAL, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+    double sc(int b, int k) const { return hscal[(size_t)b * CG_NSCAL + k]; }
+
+    // ---- launches ----
+    void mpass(void (*k)(const CgInst*, const int*, const double*, int), const double* c, int mode)
+    {
+        if (ntiles) hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c, mode);
+    }
+    void vec(void (*k)(const CgInst*, const int*, const double*, int), const double* c, int mode)
+    {
+        if (nvec) hipLaunchKernelGGL(k, dim3(nvec), dim3(CG_WG), 0, stream, d_insts, d_vec, c, mode);
+    }
+    void vec0(void (*k)(const CgInst*, const int*, const double*), const double* c)
+    {
+        if (nvec) hipLaunchKernelGGL(k, dim3(nvec), dim3(CG_WG), 0, stream, d_insts, d_vec, c);
+    }
+    void tiles0(void (*k)(const CgInst*, const int*, const double*), const double* c)
+    {
+        if (ntiles) hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c);
+    }
+    void rows0(void (*k)(const CgInst*, const int*, const double*), const double* c)
+    {
+        if (nrow) hipLaunchKernelGGL(k, dim3(nrow), dim3(CG_WG), 0, stream, d_insts, d_row, c);
+    }
+    void per_inst(void (*k)(const CgInst*, const double*, int), const double* c, int arg)
+    {
+        hipLaunchKernelGGL(k, dim3(B), dim3(CG_WG), 0, stream, d_insts, c, arg);
+    }
+    void per_inst0(void (*k)(const CgInst*, const double*), const double* c)
+    {
+        hipLaunchKernelGGL(k, dim3(B), dim3(CG_WG), 0, stream, d_insts, c);
+    }
+    void check_launch() { CG_HIP(hipGetLastError()); }
+};
+
+std::vector<double> ctl_vec(int B) { return std::vector<double>((size_t)B * CG_NCTL, 0.0); }
+
+// T pass (M pass + finalize) for the instances marked active in c.
+void tpass(Batch& bt, const double* c, int mode)
+{
+    bt.mpass(k_tpass, c, mode);
+    bt.vec(k_tpass_fin, c, mode);
+}
+
+void row_lse(Batch& bt, const double* c, int mode)
+{
+    bt.mpass(k_row_lse, c, mode);
+    bt.vec(k_rowlse_fin, c, mode);
+}
+
+void col_lse(Batch& bt, const double* c, int mode)
+{
+    bt.mpass(k_col_lse, c, mode);
+    bt.vec(k_collse_fin, c, mode);
+}
+
+// ---------------------------------------------------------------------------
+// Strong-Wolfe line search as a resumable state machine.
+// Reference: SPLRSolver::line_search_wolfe, src/sinkhorn_splr.cu:667-920, with
+// the step selection of src/utils.h:76-230 (Moré & Thuente 1994).
+// ---------------------------------------------------------------------------
+double quad_fg(double a, double b, double fa, double ga, double fb)
+{
+    const double h = b - a;
+    return a + 0.5 * h * h * ga / (fa - fb + h * ga);
+}
+double quad_gg(double a, double b, double ga, double gb) { return a + ga / (ga - gb) * (b - a); }
+double cubic_min(double a, double b, double fa, double fb, double ga, double gb, bool& exists)
+{
+    const double h = b - a, h2 = h * h, s = a + b;
+    const double df = fb - fa, dg = gb - ga;
+    const double c3 = (ga + gb) * h - 2.0 * df;
+    const double c2 = 0.5 * (dg * h2 - 3.0 * s * c3);
+    const double c1 = df * h2 - s * c2 - (a * s + b * b) * c3;
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (std::fabs(c3) < eps * std::fabs(c2) || std::fabs(c3) < eps * std::fabs(c1)) {
+        exists = (c2 * h > 0.0);
+        return exists ? (-0.5 * c1 / c2) : b;
+    }
+    const double u = c2 / (3.0 * c3), v = c1 / c2, ratio = v / u;
+    exists = (ratio <= 1.0);
+    if (!exists) return b;
+    double r1, r2;
+    if (std::fabs(u) >= std::fabs(v)) {
+        const double w = 1.0 + std::sqrt(1.0 - ratio);
+        r1 = -u * w;
+        r2 = -v / w;
+    } else {
+        const double sd = std::sqrt(std::fabs(u)) * std::sqrt(std::fabs(v)) * std::sqrt(1.0 - u / v);
+        r1 = -u - sd;
+        r2 = -u + sd;
+    }
+    return (c3 * h > 0.0) ? std::max(r1, r2) : std::min(r1, r2);
+}
+double step_sel(double al, double au, double at, double fl, double fu, double ft, double gl, double gu, double gt)
+{
+    if (al == au) return al;
+    if (!std::isfinite(ft) || !std::isfinite(gt)) return 0.5 * (al + at);
+    bool c_ok;
+    const double ac = cubic_min(al, at, fl, ft, gl, gt, c_ok);
+    const double aq = quad_fg(al, at, fl, gl, ft);
+    if (ft > fl) {
+        if (!c_ok) return aq;
+        return (std::fabs(ac - al) < std::fabs(aq - al)) ? ac : 0.5 * (aq + ac);
+    }
+    const double as = quad_gg(al, at, gl, gt);
+    if (gt * gl < 0.0) return (std::fabs(ac - at) >= std::fabs(as - at)) ? ac : as;
+    if (std::fabs(gt) < std::fabs(gl)) {
+        const bool take_c = c_ok && ((ac - at) * (at - al) > 0.0) && (std::fabs(ac - at) < std::fabs(as - at));
+        const double r = take_c ? ac : as;
+        const double lim = at + 0.66 * (au - at);
+        return (at > al) ? std::min(lim, r) : std::max(lim, r);
+    }
+    if (!std::isfinite(au) || !std::isfinite(fu) || !std::isfinite(gu)) return at + 1.1 * (at - al);
+    bool e_ok;
+    const double ae = cubic_min(at, au, ft, fu, gt, gu, e_ok);
+    const double lim = at + 0.66 * (au - at);
+    return (at > al) ? std::min(lim, ae) : std::max(lim, ae);
+}
+
+struct Wolfe {
+    static constexpr double c1 = 1e-4, c2 = 0.9, step_min = 1e-10, step_max = 2.0;
+    static constexpr int max_trials = 20;
+    double init_step, step, fx_init, dg0 = 0, dec = 0, curv = 0;
+    double lo = 0, hi = INFINITY, f_lo = 0, f_hi = INFINITY, g_lo = 0, g_hi = INFINITY, psi_lo = 0;
+    bool bracketed = false, min_guard = true;
+    int shrink_fail = 0, trials = 0;
+    double width = INFINITY;
+    double fx = 0, last_eval = 0;
+    bool finished = false, recompute = false;
+    double result = 0;
+
+    void begin(double init, double f0)
+    {
+        init_step = std::min(std::max(init, step_min), step_max);
+        step = init_step;
+        fx_init = f0;
+        fx = f0;
+        trials = 0;
+        finished = false;
+        recompute = false;
+    }
+
+    // Feed the trial at `step` (f, <g, d>; dg0_ = <g_prev, d> on the first trial).
+    // Afterwards either finished (result, recompute) or `step` is the next trial.
+    void feed(double f, double dg, double dg0_)
+    {
+        if (trials == 0) {
+            dg0 = dg0_;
+            if (dg0 > 0.0) {   // not a descent direction (reference returns init_step, recompute)
+                finished = true;
+                recompute = true;
+                result = init_step;
+                last_eval = step;
+                fx = f;
+                return;
+            }
+            dec = c1 * dg0;
+            curv = -c2 * dg0;
+            g_lo = (1.0 - c1) * dg0;
+        }
+        trials++;
+        fx = f;
+        last_eval = step;
+        const double psi = f - fx_init - step * dec;
+        const double dpsi = dg - dec;
+        if ((psi <= 0.0 && std::fabs(dg) <= curv) ||
+            (step <= step_min && (psi > 0.0 || dpsi >= 0.0)) ||
+            (step >= step_max && (psi <= 0.0 && dpsi < 0.0))) {
+            finished = true;
+            result = step;
+            return;
+        }
+        const double ft = psi, gt = dpsi;
+        if (min_guard && (psi <= 0.0 && dpsi < 0.0)) min_guard = false;
+        const bool case2 = (psi <= psi_lo) && (dpsi * (lo - step) > 0.0);
+        double nxt;
+        if (case2) {
+            nxt = std::min(step_max, step + 1.1 * (step - lo));
+        } else {
+            nxt = step_sel(lo, hi, step, f_lo, f_hi, ft, g_lo, g_hi, gt);
+            nxt = std::min(std::max(nxt, step_min), step_max);
+            if (min_guard) {
+                const double up = std::max(step_min, (7.0 / 12.0) * step);
+                nxt = std::min(std::max(nxt, step_min), up);
+            }
+        }
+        if (psi > psi_lo) {
+            hi = step; f_hi = ft; g_hi = gt;
+        } else if (case2) {
+            lo = step; f_lo = ft; g_lo = gt; psi_lo = psi;
+        } else {
+            hi = lo; f_hi = f_lo; g_hi = g_lo;
+            lo = step; f_lo = ft; g_lo = gt; psi_lo = psi;
+        }
+        if (!bracketed && !case2) {
+            const double l = std::min(lo, hi), r = std::max(lo, hi);
+            bracketed = (l >= step_min && r <= step_max);
+        }
+        if (bracketed) {
+            const double wprev = width;
+            width = std::fabs(hi - lo);
+            if (wprev < INFINITY && width > 0.66 * wprev) shrink_fail++;
+            else shrink_fail = 0;
+            if (shrink_fail >= 2) { nxt = 0.5 * (lo + hi); shrink_fail = 0; }
+        }
+        step = nxt;
+        if (trials >= max_trials) {
+            finished = true;
+            const double psi2 = fx - fx_init - step * dec;
+            if (psi2 <= psi_lo) { result = step; recompute = false; return; }
+            recompute = true;
+            result = (lo <= 0.0) ? init_step : lo;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// SPLR batch solver
+// ---------------------------------------------------------------------------
+struct HostInst {
+    int n, m, N;
+    double reg, tol;
+    int max_iter;
+    double dmax, dmin, density, shift_max;
+    int cycle, cand;
+    long long Te, Kmax, K;
+    int P, nblk;
+    bool done = false, converged = false, invalid = false;
+    int niter = 0;
+    double f = 0, gnorm = 0, gnorm_init = 0, step_prev = 1.0;
+    bool analyze = false, update = false;
+    Wolfe ls;
+    bool in_ls = false;
+};
+
+void resolve_defaults(const cgot_splr_problem& p, double& dmax, double& shift_max, int& cycle, int& cand)
+{
+    dmax = (p.density_max > 0.0) ? std::min(p.density_max, 1.0) : std::min(10.0 / std::min(p.n, p.m), 1.0);
+    dmax = std::min(std::max(dmax, 0.0), 1.0);
+    shift_max = (p.shift_max >= 0.0) ? p.shift_max : 1e-3;
+    cycle = (p.sparsity_pattern_cycle > 0) ? p.sparsity_pattern_cycle : 30;
+    cand = (p.candidate_sinkhorn_iter >= 0) ? p.candidate_sinkhorn_iter : 3;
+}
+
+long long kmax_of(double dmax, int n, int m)
+{
+    long long K = (long long)(dmax * (double)n * (double)(m - 1));
+    return std::max(K, 1LL);
+}
+
+// Lay out the workspace of an SPLR batch; with base == nullptr only sizes.
+size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
+{
+    const int B = bt.B;
+    bt.h.assign(B, CgInst{});
+    int nblk_max = 0;
+    for (int b = 0; b < B; b++) {
+        const auto& p = probs[b];
+        CgInst& I = bt.h[b];
+        set_geometry(I, p.n, p.m, p.reg);
+        double dmax, shmax;
+        int cyc, cand;
+        resolve_defaults(p, dmax, shmax, cyc, cand);
+        I.Kmax = kmax_of(dmax, p.n, p.m);
+        I.beta_side = (p.m - 1 <= p.n) ? 1 : 0;
+        I.P = I.beta_side ? (p.m - 1) : p.n;
+        nblk_max = std::max(nblk_max, cdiv(I.P, CG_NB));
+        I.M = p.M;
+        I.plan = p.plan;
+        I.dual = p.dual;
+    }
+    bt.assign_maps();
+    Bump bp(base);
+    bt.layout_common(bp, nblk_max);
+    // scratch that must start at zero, contiguous for one clear
+    bt.zero_base = bp.take<unsigned int>((size_t)B * (4 + CG_HBINS));
+    bt.zero_bytes = (size_t)B * (4 + CG_HBINS) * sizeof(unsigned int);
+    for (int b = 0; b < B; b++) {
+        CgInst& I = bt.h[b];
+        const int n = I.n, m = I.m, N = I.N;
+        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * (4 + CG_HBINS) : nullptr;
+        I.hist = bt.zero_base ? I.counter + 4 : nullptr;
+        I.ab = bp.take<double>(n + m);
+        I.logab = bp.take<double>(n + m);
+        I.gamma = bp.take<double>(n + m);
+        I.gamma_prev = bp.take<double>(n + m);
+        I.gamma_bcd = bp.take<double>(n + m);
+        I.grad = bp.take<double>(N);
+        I.grad_prev = bp.take<double>(N);
+        I.grad_bcd = bp.take<double>(N);
+        I.dir = bp.take<double>(N);
+        I.yv = bp.take<double>(N);
+        I.sv = bp.take<double>(N);
+        I.uv = bp.take<double>(N);
+        I.rowsum = bp.take<double>(n);
+        I.colsum = bp.take<double>(m);
+        I.rp0 = bp.take<double>((size_t)n * I.nstrips);
+        I.rp1 = bp.take<double>((size_t)n * I.nstrips);
+        I.cp0 = bp.take<double>((size_t)I.nrb * m);
+        I.cp1 = bp.take<double>((size_t)I.nrb * m);
+        I.wgpart = bp.take<double>((size_t)8 * I.nvec);
+        I.w_rowptr = bp.take<int>(n + 1);
+        I.w_colj = bp.take<int>(I.Kmax);
+        I.w_rval = bp.take<double>(I.Kmax);
+        I.w_colptr = bp.take<int>(m);
+        I.w_rowi = bp.take<int>(I.Kmax);
+        I.w_cval = bp.take<double>(I.Kmax);
+        I.Da = bp.take<double>(n);
+        I.Db = bp.take<double>(m);
+        I.rcnt = bp.take<int>(n);
+        I.rtie = bp.take<int>(n);
+        I.rcut = bp.take<int>(n);
+        I.ccnt = bp.take<int>((size_t)I.nrb * m);
+        I.S = bp.take<double>((size_t)I.P * I.P);
+        I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
+        I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
+        I.inv_diag = bp.take<double>(N);
+    }
+    return bp.off + 256;
+}
+
+struct SplrSolver {
+    Batch& bt;
+    std::vector<HostInst> hi;
+
+    SplrSolver(Batch& b) : bt(b) {}
+
+    std::vector<double> mask(const std::vector<char>& on)
+    {
+        auto c = ctl_vec(bt.B);
+        for (int b = 0; b < bt.B; b++) c[(size_t)b * CG_NCTL + CTL_ACTIVE] = on[b] ? 1.0 : 0.0;
+        return c;
+    }
+
+    // Zero-initialised device scratch that relies on it (counters, histograms, pivots).
+    void zero_scratch()
+    {
+        CG_HIP(hipMemsetAsync(bt.zero_base, 0, bt.zero_bytes, bt.stream));
+        CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * bt.B * CG_NSCAL, bt.stream));
+    }
+
+    // New top-K pattern (on = instances whose pattern is rebuilt) and value
+    // refresh (on2 = instances whose A is rebuilt), at the current gamma.
+    void sphess(const std::vector<char>& newpat, const std::vector<char>& refresh)
+    {
+        const int B = bt.B;
+        bool anynew = false;
+        auto c = ctl_vec(B);
+        for (int b = 0; b < B; b++) {
+            const HostInst& H = hi[b];
+            const bool on = newpat[b] && H.Te > 0;
+            c[(size_t)b * CG_NCTL + CTL_ACTIVE] = on ? 1.0 : 0.0;
+            if (on) {
+                double density = std::min(std::max(H.density, 0.0), 1.0);
+                long long K = (long long)(density * (double)H.Te);
+                K = std::min(K, H.Kmax);
+                K = std::max(K, 1LL);
+                K = std::min(K, H.Te);
+                hi[b].K = K;
+                c[(size_t)b * CG_NCTL + CTL_KSEL] = (double)K;
+                anynew = true;
+            }
+        }
+        if (anynew) {
+            const double* dc = bt.ctl(c);
+            for (int pass = 0; pass < 5; pass++) {
+                bt.mpass(k_hist, dc, pass);
+                bt.per_inst(k_hist_scan, dc, pass);
+            }
+            bt.rows0(k_sel_rows_count, dc);
+            bt.per_inst0(k_sel_rows_scan, dc);
+            bt.rows0(k_sel_rows_write, dc);
+            bt.tiles0(k_sel_cols_count, dc);
+            bt.per_inst0(k_sel_cols_scan, dc);
+            bt.tiles0(k_sel_cols_write, dc);
+        }
+        // instances without columns (m == 1): empty pattern
+        for (int b = 0; b < B; b++) {
+            if (newpat[b] && hi[b].Te == 0) {
+                hi[b].K = 0;
+                CG_HIP(hipMemsetAsync(bt.h[b].w_rowptr, 0, sizeof(int) * (bt.h[b].n + 1), bt.stream));
+                CG_HIP(hipMemsetAsync(bt.h[b].w_colptr, 0, sizeof(int) * bt.h[b].m, bt.stream));
+            }
+        }
+        auto r = ctl_vec(B);
+        bool any = false;
+        for (int b = 0; b < B; b++) {
+            if (!refresh[b]) continue;
+            any = true;
+            r[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+            const double shift = std::min(hi[b].gnorm, hi[b].shift_max);
+            r[(size_t)b * CG_NCTL + CTL_SHIFT] = shift * hi[b].reg;
+        }
+        if (any) bt.vec0(k_refresh_values, bt.ctl(r));
+    }
+
+    // Search direction for the instances in `on`.
+    void search_direction(const std::vector<char>& on, const double* dc)
+    {
+        const int B = bt.B;
+        int nblk_max = 0;
+        for (int b = 0; b < B; b++)
+            if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
+        // dense prefix tables: fill | panel[kb] | update[kb] | trsv[kb]
+        std::vector<int> pref((size_t)(3 * nblk_max + 1) * (B + 1), 0);
+        auto tab = [&](int t) { return pref.data() + (size_t)t * (B + 1); };
+        for (int b = 0; b < B; b++) {
+            const int nb = on[b] ? hi[b].nblk : 0;
+            tab(0)[b + 1] = nb * (nb + 1) / 2;
+            for (int kb = 0; kb < nblk_max; kb++) {
+                const int T = nb - kb - 1;
+                tab(1 + kb)[b + 1] = (kb < nb) ? (nb - kb) : 0;
+                tab(1 + nblk_max + kb)[b + 1] = (T > 0) ? T * (T + 1) / 2 : 0;
+                tab(1 + 2 * nblk_max + kb)[b + 1] = (kb < nb) ? (kb + 1) : 0;
+            }
+        }
+        for (int t = 0; t < 3 * nblk_max + 1; t++)
+            for (int b = 0; b < B; b++) tab(t)[b + 1] += tab(t)[b];
+        // The table is staged through a pinned buffer.  Every iteration reads the
+        // scalars back (a stream sync) before the next search direction, so the
+        // previous copy out of this buffer has completed when it is rewritten.
+        static thread_local Pinned pinned_pref;
+        if (nblk_max > 0) {
+            if ((int)pref.size() > bt.pref_cap) throw std::runtime_error("internal: prefix table overflow");
+            int* hp = (int*)pinned_pref.get((pref.size() + 1) / 2 + 1);
+            std::memcpy(hp, pref.data(), pref.size() * sizeof(int));
+            CG_HIP(hipMemcpyAsync(bt.d_pref, hp, pref.size() * sizeof(int), hipMemcpyHostToDevice, bt.stream));
+        }
+        const int* dtab0 = bt.d_pref;
+        auto dtab = [&](int t) { return dtab0 + (size_t)t * (B + 1); };
+        auto total = [&](int t) { return tab(t)[B]; };
+        if (nblk_max > 0 && total(0) > 0)
+            hipLaunchKernelGGL(k_chol_fill, dim3(total(0)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(0));
+        bt.vec0(k_schur_build, dc);
+        bt.vec0(k_schur_rhs, dc);
+        for (int kb = 0; kb < nblk_max; kb++) {
+            const int tp = total(1 + kb);
+            if (tp > 0)
+                hipLaunchKernelGGL(k_chol_panel, dim3(tp), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(1 + kb), kb);
+            const int tu = total(1 + nblk_max + kb);
+            if (tu > 0)
+                hipLaunchKernelGGL(k_chol_update, dim3(tu), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
+                                   dtab(1 + nblk_max + kb), kb);
+        }
+        for (int kb = nblk_max - 1; kb >= 0; kb--) {
+            const int tt = total(1 + 2 * nblk_max + kb);
+            if (tt > 0)
+                hipLaunchKernelGGL(k_trsv_back, dim3(tt), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
+                                   dtab(1 + 2 * nblk_max + kb), kb);
+        }
+        bt.vec0(k_schur_back, dc);
+    }
+
+    void woodbury(const double* dc)
+    {
+        bt.vec0(k_woodbury_dots, dc);
+        bt.vec0(k_woodbury_apply, dc);
+    }
+
+    void init_host(const cgot_splr_problem* probs)
+    {
+        const int B = bt.B;
+        hi.assign(B, HostInst{});
+        for (int b = 0; b < B; b++) {
+            const auto& p = probs[b];
+            HostInst& H = hi[b];
+            H.n = p.n; H.m = p.m; H.N = p.n + p.m - 1;
+            H.reg = p.reg; H.tol = p.tol; H.max_iter = p.max_iter;
+            resolve_defaults(p, H.dmax, H.shift_max, H.cycle, H.cand);
+            H.dmin = 0.01 * H.dmax;
+            H.density = 0.1 * H.dmax;
+            H.Te = (long long)p.n * (p.m - 1);
+            H.Kmax = kmax_of(H.dmax, p.n, p.m);
+            H.K = 0;
+            H.P = bt.h[b].P;
+            H.nblk = cdiv(H.P, CG_NB);
+        }
+    }
+
+    void run(const cgot_splr_problem* probs, cgot_splr_info* info)
+    {
+        const int B = bt.B;
+        init_host(probs);
+        zero_scratch();
+        std::vector<char> all(B, 1);
+        const double* dall = bt.ctl(mask(all));
+        // init: ab, log ab, gamma = x0 (gauge shifted) or beta = 0 and alpha = row LSE
+        hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dall,
+                           (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
+        {
+            std::vector<char> wx(B, 0), nox(B, 0);
+            for (int b = 0; b < B; b++) (probs[b].x0 ? wx : nox)[b] = 1;
+            if (std::count(wx.begin(), wx.end(), 1)) bt.per_inst(k_gauge_shift, bt.ctl(mask(wx)), CG_PT_GAMMA);
+            if (std::count(nox.begin(), nox.end(), 1)) row_lse(bt, bt.ctl(mask(nox)), CG_PT_GAMMA);
+        }
+        tpass(bt, dall, CG_PT_GAMMA);
+        bt.check_launch();
+        bt.read_scal();
+        for (int b = 0; b < B; b++) {
+            HostInst& H = hi[b];
+            H.f = bt.sc(b, CS_F);
+            H.gnorm = std::sqrt(bt.sc(b, CS_G2));
+            H.gnorm_init = H.gnorm;
+        }
+        sphess(all, all);
+
+        for (int iter = 0;; iter++) {
+            // loop-entry tests
+            std::vector<char> act(B, 0);
+            int nact = 0;
+            for (int b = 0; b < B; b++) {
+                HostInst& H = hi[b];
+                if (H.done) continue;
+                if (iter >= H.max_iter) { H.done = true; continue; }
+                if (H.gnorm < H.tol) { H.converged = true; H.done = true; continue; }
+                if (!std::isfinite(H.f)) { H.done = true; continue; }
+                H.analyze = (iter % H.cycle == 0);
+                H.update = (iter % H.cycle == H.cycle - 1);
+                act[b] = 1;
+                nact++;
+            }
+            if (nact == 0) break;
+            auto c = mask(act);
+            for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ALLOW_LR] = iter > 0 ? 1.0 : 0.0;
+            const double* dc = bt.ctl(c);
+            bt.vec0(k_low_rank, dc);
+            // Sinkhorn candidate (3 BCD sweeps from the current beta)
+            std::vector<char> cand(B, 0);
+            int ncand = 0;
+            for (int b = 0; b < B; b++)
+                if (act[b] && hi[b].analyze && hi[b].cand != 0) { cand[b] = 1; ncand++; }
+            if (ncand) {
+                const double* cc = bt.ctl(mask(cand));
+                bt.vec(k_vcopy, cc, 1);
+                row_lse(bt, cc, CG_PT_BCD);
+                int maxc = 0;
+                for (int b = 0; b < B; b++) if (cand[b]) maxc = std::max(maxc, hi[b].cand);
+                for (int s = 0; s < maxc; s++) {
+                    std::vector<char> cs(B, 0);
+                    for (int b = 0; b < B; b++) cs[b] = cand[b] && s < hi[b].cand;
+                    const double* c2 = bt.ctl(mask(cs));
+                    col_lse(bt, c2, CG_PT_BCD);
+                    row_lse(bt, c2, CG_PT_BCD);
+                }
+                bt.per_inst(k_gauge_shift, cc, CG_PT_BCD);
+                tpass(bt, cc, CG_PT_BCD);
+            }
+            search_direction(act, dc);
+            woodbury(dc);
+            bt.vec0(k_save_history, dc);
+            // line search rounds
+            for (int b = 0; b < B; b++)
+                if (act[b]) {
+                    hi[b].ls = Wolfe{};
+                    hi[b].ls.begin(std::min(1.0, 1.5 * hi[b].step_prev), hi[b].f);
+                    hi[b].in_ls = true;
+                }
+            bool first = true;
+            std::vector<double> g2(B, 0.0), fbcd(B, 0.0), g2bcd(B, 0.0);
+            while (true) {
+                auto t = ctl_vec(B);
+                int ntr = 0;
+                for (int b = 0; b < B; b++)
+                    if (act[b] && hi[b].in_ls) {
+                        t[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+                        t[(size_t)b * CG_NCTL + CTL_STEP] = hi[b].ls.step;
+                        ntr++;
+                    }
+                if (ntr == 0) break;
+                tpass(bt, bt.ctl(t), CG_PT_TRIAL);
+                bt.check_launch();
+                bt.read_scal();
+                for (int b = 0; b < B; b++) {
+                    if (!(act[b] && hi[b].in_ls)) continue;
+                    if (first && cand[b]) { fbcd[b] = bt.sc(b, CS_FBCD); g2bcd[b] = bt.sc(b, CS_G2BCD); }
+                    g2[b] = bt.sc(b, CS_G2);
+                    hi[b].ls.feed(bt.sc(b, CS_F), bt.sc(b, CS_DG), bt.sc(b, CS_DG0));
+                    if (hi[b].ls.finished) hi[b].in_ls = false;
+                }
+                first = false;
+            }
+            // recompute where the returned step is not the last evaluated point
+            {
+                auto t = ctl_vec(B);
+                int nre = 0;
+                for (int b = 0; b < B; b++) {
+                    if (!act[b]) continue;
+                    const Wolfe& L = hi[b].ls;
+                    if (L.recompute && L.result != L.last_eval) {
+                        t[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+                        t[(size_t)b * CG_NCTL + CTL_STEP] = L.result;
+                        nre++;
+                    }
+                }
+                if (nre) {
+                    tpass(bt, bt.ctl(t), CG_PT_TRIAL);
+                    bt.read_scal();
+                    for (int b = 0; b < B; b++)
+                        if (t[(size_t)b * CG_NCTL + CTL_ACTIVE] != 0.0) {
+                            hi[b].ls.fx = bt.sc(b, CS_F);
+                            g2[b] = bt.sc(b, CS_G2);
+                        }
+                }
+            }
+            std::vector<char> newpat(B, 0), refresh(B, 0), use_bcd(B, 0);
+            int nbcd = 0;
+            for (int b = 0; b < B; b++) {
+                if (!act[b]) continue;
+                HostInst& H = hi[b];
+                H.step_prev = H.ls.result;
+                const double gnorm_pre = H.gnorm;
+                H.f = H.ls.fx;
+                H.gnorm = std::sqrt(g2[b]);
+                if (cand[b]) {
+                    const double gb = std::sqrt(g2bcd[b]);
+                    if (fbcd[b] < H.f && gb < 10.0 * H.gnorm) {
+                        use_bcd[b] = 1;
+                        nbcd++;
+                        H.f = fbcd[b];
+                        H.gnorm = gb;
+                    }
+                }
+                if (H.update) {
+                    const bool bad = (gnorm_pre < H.gnorm_init) && (H.gnorm > gnorm_pre);
+                    H.density *= bad ? 1.1 : 0.99;
+                    H.density = std::min(H.dmax, std::max(H.dmin, H.density));
+                }
+                const bool conv_after = std::isfinite(H.gnorm) && (H.gnorm < H.tol);
+                const bool max_reached = (iter + 1 >= H.max_iter);
+                const bool inv_after = !std::isfinite(H.f) || !std::isfinite(H.gnorm);
+                H.niter = iter + 1;
+                if (conv_after || max_reached || inv_after) {
+                    H.done = true;
+                    if (inv_after) H.invalid = true;
+                    if (!max_reached && conv_after) H.converged = true;
+                } else {
+                    refresh[b] = 1;
+                    newpat[b] = H.update ? 1 : 0;
+                }
+            }
+            if (nbcd) {
+                const double* cb = bt.ctl(mask(use_bcd));
+                bt.vec0(k_copy_bcd, cb);
+                tpass(bt, cb, CG_PT_GAMMA);   // row/col sums and grad at the accepted point
+            }
+            sphess(newpat, refresh);
+            bt.check_launch();
+        }
+
+        // outputs and context
+        const double* dall2 = bt.ctl(mask(all));
+        bt.vec0(k_output_dual, dall2);
+        bt.tiles0(k_plan, dall2);
+        bt.vec(k_vcopy, dall2, 2);
+        bt.check_launch();
+        bt.read_scal();
+        for (int b = 0; b < B; b++) {
+            const HostInst& H = hi[b];
+            cgot_splr_info& o = info[b];
+            const CgInst& I = bt.h[b];
+            o.niter = H.niter;
+            o.converged = H.converged ? 1 : 0;
+            o.valid = (H.invalid || !std::isfinite(H.f)) ? 0 : 1;
+            o.fallback = (H.niter == 0) ? 1 : 0;
+            o.low_rank = (H.niter > 0 && bt.sc(b, CS_LOWRANK) != 0.0) ? 1 : 0;
+            o.ys = (H.niter > 0) ? bt.sc(b, CS_YS) : 0.0;
+            o.n = H.n;
+            o.m = H.m;
+            o.f = H.f;
+            o.gnorm = H.gnorm;
+            o.K = H.K;
+            o.inv_diag = I.inv_diag;
+            o.s = I.sv;
+            o.y = I.yv;
+            o.u = I.uv;
+            o.gamma = I.gamma;
+        }
+    }
+};
+
+void check_problem(const cgot_splr_problem& p)
+{
+    if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
+    if (!p.M || !p.a || !p.b) throw std::invalid_argument("M, a and b must be non-null device pointers");
+    if (!(p.reg > 0.0) || !std::isfinite(p.reg)) throw std::invalid_argument("reg must be positive and finite");
+    if ((long long)p.n * (long long)p.m >= (1LL << 31) || (long long)p.n + p.m >= (1LL << 28))
+        throw std::invalid_argument("instance too large for 32-bit indices");
+}
+
+
+// ---------------------------------------------------------------------------
+// Adjoint (backward) batch: PCG on Q(T*) lambda = mu, T* recomputed from duals.
+// ---------------------------------------------------------------------------
+size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
+{
+    const int B = bt.B;
+    bt.h.assign(B, CgInst{});
+    for (int b = 0; b < B; b++) {
+        const auto& p = probs[b];
+        CgInst& I = bt.h[b];
+        set_geometry(I, p.n, p.m, p.reg);
+        I.M = p.M;
+        I.gamma = const_cast<double*>(p.dual);
+        I.dir = p.lambda;
+        I.Da = const_cast<double*>(p.inv_diag);
+    }
+    bt.assign_maps();
+    Bump bp(base);
+    bt.layout_common(bp, 0);
+    bt.zero_base = bp.take<unsigned int>((size_t)B * 4);
+    bt.zero_bytes = (size_t)B * 4 * sizeof(unsigned int);
+    for (int b = 0; b < B; b++) {
+        CgInst& I = bt.h[b];
+        const int n = I.n, m = I.m, N = I.N;
+        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * 4 : nullptr;
+        I.yv = bp.take<double>(N);
+        I.sv = bp.take<double>(N);
+        I.uv = bp.take<double>(N);
+        I.grad = bp.take<double>(N);
+        I.rowsum = bp.take<double>(n);
+        I.colsum = bp.take<double>(m);
+        I.rp0 = bp.take<double>((size_t)n * I.nstrips);
+        I.rp1 = bp.take<double>((size_t)n * I.nstrips);
+        I.cp0 = bp.take<double>((size_t)I.nrb * m);
+        I.cp1 = bp.take<double>((size_t)I.nrb * m);
+        I.wgpart = bp.take<double>((size_t)8 * I.nvec);
+    }
+    return bp.off + 256;
+}
+
+void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info* info)
+{
+    const int B = bt.B;
+    CG_HIP(hipMemsetAsync(bt.zero_base, 0, bt.zero_bytes, bt.stream));
+    CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * B * CG_NSCAL, bt.stream));
+    auto c = ctl_vec(B);
+    for (int b = 0; b < B; b++) {
+        c[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+        c[(size_t)b * CG_NCTL + CTL_STEP] = probs[b].tol;
+        c[(size_t)b * CG_NCTL + CTL_AUX] = (double)probs[b].max_iter;
+    }
+    const double* dc = bt.ctl(c);
+    const double* const* Gs = (const double* const*)bt.d_ptrs;
+    const double* const* lams = (const double* const*)(bt.d_ptrs + B);
+    double* const* outs = (double* const*)(bt.d_ptrs + 2 * B);
+    hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc,
+                       (const double* const*)nullptr, Gs, (int)CG_QV_RHS);
+    bt.vec0(k_pcg_init, dc);
+    bt.check_launch();
+    bt.read_scal();
+    std::vector<char> running(B, 0);
+    int nrun = 0;
+    for (int b = 0; b < B; b++) {
+        running[b] = (bt.sc(b, PCG_STATUS) == PCG_RUNNING) && probs[b].max_iter > 0;
+        nrun += running[b];
+    }
+    int chunk = 4;
+    while (nrun > 0) {
+        auto c2 = c;
+        for (int b = 0; b < B; b++) c2[(size_t)b * CG_NCTL + CTL_ACTIVE] = running[b] ? 1.0 : 0.0;
+        const double* d2 = bt.ctl(c2);
+        for (int k = 0; k < chunk; k++) {
+            hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, d2,
+                               (const double* const*)nullptr, (const double* const*)nullptr, (int)CG_QV_PCG);
+            bt.vec0(k_pcg_q_fin, d2);
+            bt.vec0(k_pcg_update, d2);
+        }
+        bt.check_launch();
+        bt.read_scal();
+        nrun = 0;
+        for (int b = 0; b < B; b++) {
+            if (running[b] && bt.sc(b, PCG_STATUS) != PCG_RUNNING) running[b] = 0;
+            nrun += running[b];
+        }
+        chunk = std::min(chunk * 2, 32);
+    }
+    hipLaunchKernelGGL(k_grad_assemble, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc,
+                       lams, Gs, outs);
+    bt.check_launch();
+    bt.sync();
+    for (int b = 0; b < B; b++) {
+        const double st = bt.sc(b, PCG_STATUS);
+        info[b].status = (int)st;
+        info[b].iterations = (st == PCG_ZERO_RHS) ? 0 : (int)bt.sc(b, PCG_ITER);
+        info[b].converged = (st == PCG_CONVERGED || st == PCG_ZERO_RHS) ? 1 : 0;
+        if (probs[b].max_iter <= 0 && st == PCG_RUNNING) { info[b].status = 3; info[b].converged = 0; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BCD (Sinkhorn) batch.  Reference: cuda_sinkhorn_bcd, src/sinkhorn_bcd_kernel.cu:429-634.
+// ---------------------------------------------------------------------------
+size_t bcd_layout(Batch& bt, const cgot_bcd_problem* probs, void* base)
+{
+    const int B = bt.B;
+    bt.h.assign(B, CgInst{});
+    for (int b = 0; b < B; b++) {
+        const auto& p = probs[b];
+        CgInst& I = bt.h[b];
+        set_geometry(I, p.n, p.m, p.reg);
+        I.M = p.M;
+        I.plan = p.plan;
+        I.dual = p.dual;
+    }
+    bt.assign_maps();
+    Bump bp(base);
+    bt.layout_common(bp, 0);
+    bt.zero_base = bp.take<unsigned int>((size_t)B * 4);
+    bt.zero_bytes = (size_t)B * 4 * sizeof(unsigned int);
+    for (int b = 0; b < B; b++) {
+        CgInst& I = bt.h[b];
+        const int n = I.n, m = I.m, N = I.N;
+        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * 4 : nullptr;
+        I.ab = bp.take<double>(n + m);
+        I.logab = bp.take<double>(n + m);
+        I.gamma = bp.take<double>(n + m);
+        I.gamma_prev = bp.take<double>(n + m);
+        I.grad = bp.take<double>(N);
+        I.grad_prev = bp.take<double>(N);
+        I.rowsum = bp.take<double>(n);
+        I.colsum = bp.take<double>(m);
+        I.rp0 = bp.take<double>((size_t)n * I.nstrips);
+        I.rp1 = bp.take<double>((size_t)n * I.nstrips);
+        I.cp0 = bp.take<double>((size_t)I.nrb * m);
+        I.cp1 = bp.take<double>((size_t)I.nrb * m);
+        I.wgpart = bp.take<double>((size_t)8 * I.nvec);
+    }
+    return bp.off + 256;
+}
+
+void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
+{
+    const int B = bt.B;
+    CG_HIP(hipMemsetAsync(bt.zero_base, 0, bt.zero_bytes, bt.stream));
+    CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * B * CG_NSCAL, bt.stream));
+    auto mk = [&](const std::vector<char>& on) {
+        auto c = ctl_vec(B);
+        for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ACTIVE] = on[b] ? 1.0 : 0.0;
+        return bt.ctl(c);
+    };
+    std::vector<char> all(B, 1), act(B, 1), marg(B, 0);
+    hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, mk(all),
+                       (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
+    for (int b = 0; b < B; b++) {
+        niter[b] = 0;
+        if (probs[b].max_iter <= 0) act[b] = 0;
+    }
+    for (int it = 0;; it++) {
+        int na = 0;
+        for (int b = 0; b < B; b++) {
+            if (act[b] && it >= probs[b].max_iter) act[b] = 0;
+            na += act[b];
+        }
+        if (na == 0) break;
+        const double* dc = mk(act);
+        bt.vec(k_vcopy, dc, 0);
+        row_lse(bt, dc, CG_PT_GAMMA);
+        col_lse(bt, dc, CG_PT_GAMMA);
+        std::vector<char> p1(B, 0), p2(B, 0);
+        for (int b = 0; b < B; b++) { p1[b] = act[b] && !marg[b]; p2[b] = act[b] && marg[b]; }
+        if (std::count(p1.begin(), p1.end(), 1)) bt.vec0(k_bcd_diff, mk(p1));
+        if (std::count(p2.begin(), p2.end(), 1)) tpass(bt, mk(p2), CG_PT_GAMMA);
+        bt.check_launch();
+        bt.read_scal();
+        for (int b = 0; b < B; b++) {
+            if (!act[b]) continue;
+            niter[b] = it + 1;
+            const double tol = probs[b].tol;
+            if (p1[b]) {
+                const double* S = &bt.hscal[(size_t)b * CG_NSCAL + BCD_SUMS];
+                const int n = probs[b].n, m = probs[b].m;
+                const double cc = (S[1] - S[0]) / (n + m), dd = (S[3] - S[2]) / (n + m), K = cc - dd;
+                const double d2 = S[5] + 2.0 * K * S[4] + n * K * K + S[7] - 2.0 * K * S[6] + m * K * K;
+                if (std::sqrt(std::max(d2, 0.0)) < tol * 10) marg[b] = 1;
+            } else {
+                if (std::sqrt(bt.sc(b, CS_G2A)) < tol) act[b] = 0;
+            }
+        }
+    }
+    const double* da = mk(all);
+    bt.vec0(k_output_dual, da);
+    bt.tiles0(k_plan, da);
+    bt.check_launch();
+    bt.sync();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define CG_API_BEGIN try {
+#define CG_API_END                         \
+    }                                      \
+    catch (const std::invalid_argument& e) \
+    {                                      \
+        g_last_error = e.what();           \
+        return -2;                         \
+    }                                      \
+    catch (const std::exception& e)        \
+    {                                      \
+        g_last_error = e.what();           \
+        return -1;                         \
+    }
+
+extern "C" {
+
+const char* cgot_last_error(void) { return g_last_error.c_str(); }
+const char* cgot_version(void) { return "curegot_amd 0.1.0 gfx950"; }
+
+size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs)
+{
+    try {
+        Batch bt;
+        bt.B = B;
+        for (int b = 0; b < B; b++) check_problem(probs[b]);
+        return splr_layout(bt, probs, nullptr);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
+                    void* stream, cgot_splr_info* info)
+{
+    CG_API_BEGIN
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    for (int b = 0; b < B; b++) check_problem(probs[b]);
+    Batch bt;
+    bt.B = B;
+    bt.stream = (hipStream_t)stream;
+    const size_t need = splr_layout(bt, probs, nullptr);
+    if (workspace == nullptr || workspace_bytes < need)
+        throw std::invalid_argument("workspace too small: need " + std::to_string(need) + " bytes");
+    splr_layout(bt, probs, workspace);
+    std::vector<const double*> ptrs(3 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        ptrs[3 * b] = probs[b].a;
+        ptrs[3 * b + 1] = probs[b].b;
+        ptrs[3 * b + 2] = probs[b].x0;
+    }
+    bt.upload_maps(ptrs);
+    SplrSolver s(bt);
+    s.run(probs, info);
+    return 0;
+    CG_API_END
+}
+
+
+size_t cgot_adjoint_workspace_size(int32_t B, const cgot_adjoint_problem* probs)
+{
+    try {
+        Batch bt;
+        bt.B = B;
+        return adjoint_layout(bt, probs, nullptr);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* workspace, size_t workspace_bytes,
+                       void* stream, cgot_adjoint_info* info)
+{
+    CG_API_BEGIN
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    for (int b = 0; b < B; b++) {
+        const auto& p = probs[b];
+        if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
+        if (!p.M || !p.dual || !p.G || !p.lambda || !p.grad) throw std::invalid_argument("null pointer");
+        if (!(p.reg > 0.0)) throw std::invalid_argument("reg must be positive");
+    }
+    Batch bt;
+    bt.B = B;
+    bt.stream = (hipStream_t)stream;
+    const size_t need = adjoint_layout(bt, probs, nullptr);
+    if (workspace == nullptr || workspace_bytes < need)
+        throw std::invalid_argument("workspace too small: need " + std::to_string(need) + " bytes");
+    adjoint_layout(bt, probs, workspace);
+    std::vector<const double*> ptrs(3 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        ptrs[b] = probs[b].G;
+        ptrs[B + b] = probs[b].lambda;
+        ptrs[2 * B + b] = probs[b].grad;
+    }
+    bt.upload_maps(ptrs);
+    adjoint_run(bt, probs, info);
+    return 0;
+    CG_API_END
+}
+
+size_t cgot_bcd_workspace_size(int32_t B, const cgot_bcd_problem* probs)
+{
+    try {
+        Batch bt;
+        bt.B = B;
+        return bcd_layout(bt, probs, nullptr);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int cgot_bcd_solve(int32_t B, const cgot_bcd_problem* probs, void* workspace, size_t workspace_bytes,
+                   void* stream, int32_t* niter)
+{
+    CG_API_BEGIN
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    for (int b = 0; b < B; b++) {
+        const auto& p = probs[b];
+        if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
+        if (!p.M || !p.a || !p.b) throw std::invalid_argument("null pointer");
+        if (!(p.reg > 0.0)) throw std::invalid_argument("reg must be positive");
+    }
+    Batch bt;
+    bt.B = B;
+    bt.stream = (hipStream_t)stream;
+    const size_t need = bcd_layout(bt, probs, nullptr);
+    if (workspace == nullptr || workspace_bytes < need)
+        throw std::invalid_argument("workspace too small: need " + std::to_string(need) + " bytes");
+    bcd_layout(bt, probs, workspace);
+    std::vector<const double*> ptrs(3 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        ptrs[3 * b] = probs[b].a;
+        ptrs[3 * b + 1] = probs[b].b;
+        ptrs[3 * b + 2] = probs[b].x0;
+    }
+    bt.upload_maps(ptrs);
+    bcd_run(bt, probs, niter);
+    return 0;
+    CG_API_END
+}
+
+size_t cgot_lse_workspace_size(int32_t n, int32_t m)
+{
+    cgot_bcd_problem p{};
+    p.n = n; p.m = m; p.reg = 1.0;
+    p.M = p.a = p.b = (const double*)16;
+    return cgot_bcd_workspace_size(1, &p);
+}
+
+int cgot_lse_update(const double* M, const double* a, const double* b, int32_t n, int32_t m, double reg,
+                    int32_t which, int32_t repeats, double* dual, void* workspace, size_t workspace_bytes,
+                    void* stream)
+{
+    CG_API_BEGIN
+    if (n < 1 || m < 1 || !M || !a || !b || !dual) throw std::invalid_argument("bad arguments");
+    if (!(reg > 0.0)) throw std::invalid_argument("reg must be positive");
+    cgot_bcd_problem p{};
+    p.M = M; p.a = a; p.b = b; p.n = n; p.m = m; p.reg = reg;
+    Batch bt;
+    bt.B = 1;
+    bt.stream = (hipStream_t)stream;
+    const size_t need = bcd_layout(bt, &p, nullptr);
+    if (workspace == nullptr || workspace_bytes < need) throw std::invalid_argument("workspace too small");
+    bcd_layout(bt, &p, workspace);
+    bt.h[0].gamma = dual;   // update in place
+    std::vector<const double*> ptrs = {a, b, dual};
+    bt.upload_maps(ptrs);
+    auto c = ctl_vec(1);
+    c[CTL_ACTIVE] = 1.0;
+    const double* dc = bt.ctl(c);
+    hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dc,
+                       (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
+    for (int r = 0; r < std::max(repeats, 1); r++) {
+        if (which == 0) row_lse(bt, dc, CG_PT_GAMMA);
+        else col_lse(bt, dc, CG_PT_GAMMA);
+    }
+    bt.check_launch();
+    bt.sync();
+    return 0;
+    CG_API_END
+}
+
+size_t cgot_T_sparsify_workspace_size(int32_t n, int32_t m, int64_t K)
+{
+    cgot_splr_problem p{};
+    p.n = n; p.m = m; p.reg = 1.0;
+    p.M = p.a = p.b = (const double*)16;
+    const double Te = (double)n * (double)(m - 1);
+    p.density_max = (Te > 0) ? std::min(1.0, (std::max((double)K, 1.0) + 0.5) / Te) : 1.0;
+    return cgot_splr_workspace_size(1, &p);
+}
+
+int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const double* b, int32_t n, int32_t m,
+                    double reg, double shift, int64_t K, double* rowsum, double* colsum, double* grad,
+                    double* scalars, double* csr_values, int32_t* csr_colind, int32_t* csr_rowptr,
+                    const double* rhs, double* sol, void* workspace, size_t workspace_bytes, void* stream)
+{
+    CG_API_BEGIN
+    if (n < 1 || m < 1 || !gamma || !M || !a || !b) throw std::invalid_argument("bad arguments");
+    cgot_splr_problem p{};
+    p.M = M; p.a = a; p.b = b; p.n = n; p.m = m; p.reg = reg; p.tol = 0; p.max_iter = 1;
+    const double Te = (double)n * (double)(m - 1);
+    p.density_max = (Te > 0) ? std::min(1.0, (std::max((double)K, 1.0) + 0.5) / Te) : 1.0;
+    p.shift_max = 0.0;
+    p.sparsity_pattern_cycle = 30;
+    p.candidate_sinkhorn_iter = 0;
+    p.x0 = gamma;
+    Batch bt;
+    bt.B = 1;
+    bt.stream = (hipStream_t)stream;
+    const size_t need = splr_layout(bt, &p, nullptr);
+    if (workspace == nullptr || workspace_bytes < need) throw std::invalid_argument("workspace too small");
+    splr_layout(bt, &p, workspace);
+    std::vector<const double*> ptrs = {a, b, gamma};
+    bt.upload_maps(ptrs);
+    SplrSolver s(bt);
+    s.init_host(&p);
+    s.zero_scratch();
+    std::vector<char> all(1, 1);
+    const double* dc = bt.ctl(s.mask(all));
+    hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dc,
+                       (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
+    tpass(bt, dc, CG_PT_GAMMA);
+    // top-K with exactly K' = min(max(K,1), n(m-1)) entries; shift added as given
+    HostInst& H = s.hi[0];
+    H.Kmax = std::max<long long>(1, std::min<long long>(std::max<long long>(K, 1), H.Te));
+    H.density = (H.Te > 0) ? ((double)H.Kmax + 0.5) / (double)H.Te : 0.0;
+    H.gnorm = shift / reg;
+    H.shift_max = shift / reg;
+    s.sphess(all, all);
+    const CgInst& I = bt.h[0];
+    hipLaunchKernelGGL(k_hsl_csr, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dc, csr_values,
+                       (int*)csr_colind, (int*)csr_rowptr);
+    CG_HIP(hipMemcpyAsync(rowsum, I.rowsum, sizeof(double) * n, hipMemcpyDeviceToDevice, bt.stream));
+    CG_HIP(hipMemcpyAsync(colsum, I.colsum, sizeof(double) * m, hipMemcpyDeviceToDevice, bt.stream));
+    CG_HIP(hipMemcpyAsync(grad, I.grad, sizeof(double) * I.N, hipMemcpyDeviceToDevice, bt.stream));
+    if (rhs != nullptr && sol != nullptr) {
+        CG_HIP(hipMemcpyAsync(I.grad, rhs, sizeof(double) * I.N, hipMemcpyDeviceToDevice, bt.stream));
+        CG_HIP(hipMemcpyAsync(I.yv, rhs, sizeof(double) * I.N, hipMemcpyDeviceToDevice, bt.stream));
+        s.search_direction(all, dc);
+        CG_HIP(hipMemcpyAsync(sol, I.dir, sizeof(double) * I.N, hipMemcpyDeviceToDevice, bt.stream));
+    }
+    bt.check_launch();
+    bt.read_scal();
+    if (scalars) {
+        double hs[2] = {bt.sc(0, CS_F), std::sqrt(bt.sc(0, CS_G2))};
+        CG_HIP(hipMemcpyAsync(scalars, hs, sizeof(hs), hipMemcpyHostToDevice, bt.stream));
+        CG_HIP(hipStreamSynchronize(bt.stream));
+    }
+    return 0;
+    CG_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Kernel declarations and launch-time enums shared by the kernel files and the
+// host driver.
+#pragma once
+
+#include "cgot_common.h"
+
+// Host-written per-instance control block (ctl[b*CG_NCTL + field]).
+enum {
+    CTL_ACTIVE = 0,   // 1.0: the instance takes part in this launch
+    CTL_STEP,         // trial step for CG_PT_TRIAL
+    CTL_SHIFT,        // shift*reg added to the Hessian diagonal
+    CTL_ALLOW_LR,     // 1.0 when the low-rank term may be used (iter > 0)
+    CTL_KSEL,         // K for a new sparsity pattern
+    CTL_AUX,          // kernel-specific
+    CG_NCTL = 8
+};
+
+enum { CG_PT_GAMMA = 0, CG_PT_TRIAL = 1, CG_PT_BCD = 2 };
+enum { CG_QV_MATVEC = 0, CG_QV_RHS = 1, CG_QV_SUMS = 2, CG_QV_PCG = 3 };
+
+// ---- M passes (grid = total tiles; tile_inst maps tile -> instance) ----
+__global__ void k_tpass(const CgInst*, const int*, const double*, int mode);
+__global__ void k_row_lse(const CgInst*, const int*, const double*, int mode);
+__global__ void k_col_lse(const CgInst*, const int*, const double*, int mode);
+__global__ void k_qv(const CgInst*, const int*, const double*, const double* const*,
+                     const double* const*, int mode);
+__global__ void k_plan(const CgInst*, const int*, const double*);
+__global__ void k_grad_assemble(const CgInst*, const int*, const double*, const double* const*,
+                                const double* const*, double* const*);
+
+// ---- finalize / vector kernels (grid = total fin or vec workgroups) ----
+__global__ void k_tpass_fin(const CgInst*, const int*, const double*, int mode);
+__global__ void k_rowlse_fin(const CgInst*, const int*, const double*, int mode);
+__global__ void k_collse_fin(const CgInst*, const int*, const double*, int mode);
+__global__ void k_gauge_shift(const CgInst*, const double*, int mode);      // grid = B
+__global__ void k_init_logab(const CgInst*, const int*, const double*, const double* const*,
+                             const double* const*);
+__global__ void k_low_rank(const CgInst*, const int*, const double*);
+__global__ void k_save_history(const CgInst*, const int*, const double*);
+__global__ void k_copy_bcd(const CgInst*, const int*, const double*);
+__global__ void k_gamma_from_step(const CgInst*, const int*, const double*);
+__global__ void k_woodbury_dots(const CgInst*, const int*, const double*);
+__global__ void k_woodbury_apply(const CgInst*, const int*, const double*);
+__global__ void k_output_dual(const CgInst*, const int*, const double*);
+__global__ void k_vcopy(const CgInst*, const int*, const double*, int what);
+__global__ void k_bcd_diff(const CgInst*, const int*, const double*);
+__global__ void k_hsl_csr(const CgInst*, const int*, const double*, double*, int*, int*);
+
+// ---- sparsified Hessian (top-K pattern and values) ----
+__global__ void k_hist(const CgInst*, const int*, const double*, int pass);
+__global__ void k_hist_scan(const CgInst*, const double*, int pass);              // grid = B
+__global__ void k_sel_rows_count(const CgInst*, const int* row_inst, const double*); // row grid
+__global__ void k_sel_rows_scan(const CgInst*, const double*);                       // grid = B
+__global__ void k_sel_rows_write(const CgInst*, const int* row_inst, const double*); // row grid
+__global__ void k_sel_cols_count(const CgInst*, const int*, const double*);
+__global__ void k_sel_cols_scan(const CgInst*, const double*);                    // grid = B
+__global__ void k_sel_cols_write(const CgInst*, const int*, const double*);
+__global__ void k_refresh_values(const CgInst*, const int*, const double*);       // vec grid
+
+// ---- Newton solve on the Schur complement ----
+__global__ void k_schur_build(const CgInst*, const int*, const double*);          // vec grid over P
+__global__ void k_schur_rhs(const CgInst*, const int*, const double*);            // vec grid
+__global__ void k_schur_back(const CgInst*, const int*, const double*);           // vec grid
+// compact ragged grids: pref[b]..pref[b+1] are instance b's workgroups
+__global__ void k_chol_fill(const CgInst*, int B, const int* pref);
+__global__ void k_chol_panel(const CgInst*, int B, const int* pref, int kb);
+__global__ void k_chol_update(const CgInst*, int B, const int* pref, int kb);
+__global__ void k_trsv_back(const CgInst*, int B, const int* pref, int kb);
+
+// ---- backward PCG (vec grid) ----
+__global__ void k_pcg_init(const CgInst*, const int*, const double*);
+__global__ void k_pcg_q_fin(const CgInst*, const int*, const double*);
+__global__ void k_pcg_update(const CgInst*, const int*, const double*);

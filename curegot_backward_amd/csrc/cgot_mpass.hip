@@ -1,0 +1,373 @@
+// M-sized passes: every kernel here streams the cost matrices of a ragged
+// batch once, row-major and coalesced, and writes fixed-order partials that a
+// finalize kernel combines.  These are the HBM-bound kernels of the solver.
+//
+// Tile geometry (per instance b): strips of sw = 64*c columns (c columns per
+// lane, lane l owns columns s0 + l + 64k) by row blocks of CG_RB rows (each of
+// the 4 waves owns CG_RPW consecutive rows).  Each tile writes
+//   row partials  rp[i*nstrips + strip]   (one per row and strip)
+//   col partials  cp[rowblock*m + j]      (one per row block and column)
+// so the finalize step sums a fixed number of partials in a fixed order.
+#include "cgot_common.h"
+#include "cgot_kernels.h"
+
+// Point (alpha, beta) the pass is evaluated at.
+//   CG_PT_GAMMA : gamma
+//   CG_PT_TRIAL : gamma_prev + step*dir on the first N entries, beta_{m-1} = 0
+//   CG_PT_BCD   : gamma_bcd
+CG_DEV double cg_point(const CgInst& I, int mode, double t, int k)
+{
+    if (mode == CG_PT_GAMMA) return I.gamma[k];
+    if (mode == CG_PT_BCD) return I.gamma_bcd[k];
+    return (k < I.N) ? (I.gamma_prev[k] + t * I.dir[k]) : 0.0;
+}
+
+CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const double* ctl,
+                           int& b, int& rb, int& st)
+{
+    const int t = blockIdx.x;
+    b = tile_inst[t];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
+    const CgInst& I = insts[b];
+    const int lt = t - I.tile0;
+    rb = lt / I.nstrips;
+    st = lt - rb * I.nstrips;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// T pass: T_ij = exp((alpha_i + beta_j - M_ij)/reg); row and column partial sums.
+// Reference: T_fused_kernel / line_search_fused_kernel2,
+// src/sinkhorn_splr_kernel.cu:214-338 (float atomics there; fixed-order here).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ insts,
+                                                 const int* __restrict__ tile_inst,
+                                                 const double* __restrict__ ctl, int mode)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double t = ctl[b * CG_NCTL + CTL_STEP];
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    double bj[CG_CMAX], cs[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int j = s0 + lane + 64 * k;
+        bj[k] = (k < I.c && j < m) ? cg_point(I, mode, t, n + j) : 0.0;
+        cs[k] = 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double ai = cg_point(I, mode, t, i);
+        const double* Mi = I.M + (size_t)i * m;
+        double rs = 0.0;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < m) {
+                const double v = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
+                cs[k] += v;
+                rs += v;
+            }
+        }
+        rs = cg_wave_sum(rs);
+        if (lane == 0) I.rp0[(size_t)i * I.nstrips + st] = rs;
+    }
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cs[k];
+    __syncthreads();
+    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+        const int j = s0 + q;
+        if (j < m) {
+            const double v = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+            I.cp0[(size_t)rb * m + j] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row log-sum-exp partials of (beta_j - M_ij)/reg for the alpha update.
+// Reference: optimal_alpha_kernel, src/sinkhorn_bcd_kernel.cu:115-192.
+// Per (row, strip): wave max, then sum of exp(x - max).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ tile_inst,
+                                                   const double* __restrict__ ctl, int mode)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const double* beta = (mode == CG_PT_BCD) ? (I.gamma_bcd + n) : (I.gamma + n);
+    double bj[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int j = s0 + lane + 64 * k;
+        bj[k] = (k < I.c && j < m) ? beta[j] : 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double* Mi = I.M + (size_t)i * m;
+        double x[CG_CMAX];
+        double mx = -INFINITY;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            x[k] = (k < I.c && j < m) ? (bj[k] - Mi[j]) * I.inv_reg : -INFINITY;
+            mx = fmax(mx, x[k]);
+        }
+        mx = cg_wave_max(mx);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++)
+            if (x[k] != -INFINITY) s += exp(x[k] - mx);
+        s = cg_wave_sum(s);
+        if (lane == 0) {
+            I.rp0[(size_t)i * I.nstrips + st] = mx;
+            I.rp1[(size_t)i * I.nstrips + st] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column log-sum-exp partials of (alpha_i - M_ij)/reg for the beta update.
+// Reference: optimal_beta_kernel, src/sinkhorn_bcd_kernel.cu:35-112 (strided
+// column walk there; here each lane streams its columns down the row block
+// with an online max/sum, reads stay row-major and coalesced).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ tile_inst,
+                                                   const double* __restrict__ ctl, int mode)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const double* alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
+    double mx[CG_CMAX], sm[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) { mx[k] = -INFINITY; sm[k] = 0.0; }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double ai = alpha[i];
+        const double* Mi = I.M + (size_t)i * m;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < m) {
+                const double x = (ai - Mi[j]) * I.inv_reg;
+                if (x > mx[k]) {
+                    sm[k] = (mx[k] == -INFINITY) ? 1.0 : sm[k] * exp(mx[k] - x) + 1.0;
+                    mx[k] = x;
+                } else {
+                    sm[k] += exp(x - mx[k]);
+                }
+            }
+        }
+    }
+    __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
+    __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) { lmx[w][lane + 64 * k] = mx[k]; lsm[w][lane + 64 * k] = sm[k]; }
+    __syncthreads();
+    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+        const int j = s0 + q;
+        if (j < m) {
+            double a = lmx[0][q], s = lsm[0][q];
+            cg_lse_merge(a, s, lmx[1][q], lsm[1][q]);
+            cg_lse_merge(a, s, lmx[2][q], lsm[2][q]);
+            cg_lse_merge(a, s, lmx[3][q], lsm[3][q]);
+            I.cp0[(size_t)rb * m + j] = a;
+            I.cp1[(size_t)rb * m + j] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint pass with T recomputed from the duals (gamma):
+//   CG_QV_MATVEC: row partial sum_j T_ij v_beta_j (j < m-1), col partial sum_i T_ij v_alpha_i
+//   CG_QV_PCG:    the same with v = z + beta*p_prev formed on the fly (PCG direction)
+//   CG_QV_RHS:    rp0/cp0 = sums of G_ij T_ij, rp1/cp1 = sums of T_ij
+//   CG_QV_SUMS:   rp0/cp0 = sums of T_ij
+// Reference: q_matvec / rhs_Q_scale, curegot/_backward_core.py:31-45 (3-4 torch
+// passes over a stored T there, one pass over M here).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
+                                              const int* __restrict__ tile_inst,
+                                              const double* __restrict__ ctl,
+                                              const double* const* __restrict__ vecs,
+                                              const double* const* __restrict__ Gs, int mode)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    if (mode == CG_QV_PCG && I.scal[PCG_STATUS] != 0.0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const bool mv = (mode == CG_QV_MATVEC || mode == CG_QV_PCG);
+    const double* v = (mode == CG_QV_MATVEC) ? vecs[b] : nullptr;
+    const double pbeta = (mode == CG_QV_PCG) ? I.scal[PCG_BETA] : 0.0;
+    const double* G = (mode == CG_QV_RHS) ? Gs[b] : nullptr;
+    auto vat = [&](int k) -> double {
+        return (mode == CG_QV_PCG) ? (I.sv[k] + pbeta * I.uv[k]) : v[k];
+    };
+    double bj[CG_CMAX], vb[CG_CMAX], cs[CG_CMAX], ct[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int j = s0 + lane + 64 * k;
+        const bool ok = (k < I.c && j < m);
+        bj[k] = ok ? I.gamma[n + j] : 0.0;
+        vb[k] = (ok && mv && j < m - 1) ? vat(n + j) : 0.0;
+        cs[k] = 0.0;
+        ct[k] = 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double ai = I.gamma[i];
+        const double va = mv ? vat(i) : 0.0;
+        const double* Mi = I.M + (size_t)i * m;
+        double rs = 0.0, rt = 0.0;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < m) {
+                const double tv = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
+                if (mv) {
+                    rs += tv * vb[k];
+                    cs[k] += tv * va;
+                } else if (mode == CG_QV_RHS) {
+                    const double gt = G[(size_t)i * m + j] * tv;
+                    rs += gt;
+                    cs[k] += gt;
+                    rt += tv;
+                    ct[k] += tv;
+                } else {
+                    rs += tv;
+                    cs[k] += tv;
+                }
+            }
+        }
+        rs = cg_wave_sum(rs);
+        if (mode == CG_QV_RHS) rt = cg_wave_sum(rt);
+        if (lane == 0) {
+            I.rp0[(size_t)i * I.nstrips + st] = rs;
+            if (mode == CG_QV_RHS) I.rp1[(size_t)i * I.nstrips + st] = rt;
+        }
+    }
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cs[k];
+    __syncthreads();
+    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+        const int j = s0 + q;
+        if (j < m) I.cp0[(size_t)rb * m + j] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+    }
+    if (mode == CG_QV_RHS) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = ct[k];
+        __syncthreads();
+        for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+            const int j = s0 + q;
+            if (j < m) I.cp1[(size_t)rb * m + j] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise writers
+// ---------------------------------------------------------------------------
+// P_ij = exp((alpha_i + beta_j - M_ij)/reg).  Reference: compute_transport_plan_kernel,
+// src/sinkhorn_bcd_kernel.cu:383-403.
+__global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts,
+                                                const int* __restrict__ tile_inst,
+                                                const double* __restrict__ ctl)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    if (I.plan == nullptr) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const double* dual = I.gamma;
+    double bj[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int j = s0 + lane + 64 * k;
+        bj[k] = (k < I.c && j < m) ? dual[n + j] : 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double ai = dual[i];
+        const double* Mi = I.M + (size_t)i * m;
+        double* Pi = I.plan + (size_t)i * m;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < m) Pi[j] = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
+        }
+    }
+}
+
+// dL/dM_ij = T_ij (lam_alpha_i + lam_beta_j - G_ij)/reg, lam_beta_{m-1} = 0.
+// Reference: assemble_grad, curegot/_backward_core.py:47-53.
+__global__ __launch_bounds__(CG_WG) void k_grad_assemble(const CgInst* __restrict__ insts,
+                                                         const int* __restrict__ tile_inst,
+                                                         const double* __restrict__ ctl,
+                                                         const double* const* __restrict__ lams,
+                                                         const double* const* __restrict__ Gs,
+                                                         double* const* __restrict__ outs)
+{
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const double* lam = lams[b];
+    const double* G = Gs[b];
+    double* out = outs[b];
+    double bj[CG_CMAX], lb[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int j = s0 + lane + 64 * k;
+        const bool ok = (k < I.c && j < m);
+        bj[k] = ok ? I.gamma[n + j] : 0.0;
+        lb[k] = (ok && j < m - 1) ? lam[n + j] : 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= n) break;
+        const double ai = I.gamma[i];
+        const double la = lam[i];
+        const size_t row = (size_t)i * m;
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < m) {
+                const double tv = exp((ai + bj[k] - I.M[row + j]) * I.inv_reg);
+                out[row + j] = (tv * (la + lb[k] - G[row + j])) / I.reg;
+            }
+        }
+    }
+}

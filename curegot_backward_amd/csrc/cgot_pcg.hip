@@ -1,0 +1,188 @@
+// Implicit-function adjoint: Q(T*) lambda = mu solved by preconditioned CG
+// entirely on the device, T recomputed from the converged duals in every
+// matrix-vector product (one pass over M per iteration, no stored plan).
+//
+// Reference: pcg / q_matvec / rhs_Q_scale, curegot/_backward_core.py:31-83 and
+// solve_backward, curegot/backward.py:72-144 (a Python loop of torch ops with
+// two host syncs per iteration there).
+//
+// Vector roles inside CgInst for a backward instance:
+//   dir = x (lambda), yv = r, sv = z, uv = p, grad = w = Q p,
+//   rowsum/colsum = T 1 / T^T 1, Da = inverse diagonal preconditioner (or null).
+// Per-iteration launches: k_qv(CG_QV_PCG) -> k_pcg_q_fin -> k_pcg_update.
+// Every kernel skips an instance whose status is not "running", so the host can
+// launch iterations in chunks and read the statuses once per chunk; the
+// iterates are exactly those of the sequential loop.
+#include "cgot_common.h"
+#include "cgot_kernels.h"
+
+template <int NV>
+CG_DEV bool cg_inst_reduce_p(const CgInst& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
+{
+    __shared__ double lds4[4];
+    __shared__ double tot[NV];
+    __shared__ int is_last;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = cg_block_sum(v[k], lds4);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) I.wgpart[(size_t)k * I.nvec + wgi] = v[k];
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int prev = atomicAdd(&I.counter[slot], 1u);
+        is_last = (prev == (unsigned int)(I.nvec - 1));
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    if (threadIdx.x < NV) {
+        const volatile double* wp = I.wgpart + (size_t)threadIdx.x * I.nvec;
+        double s = 0.0;
+        for (int q = 0; q < I.nvec; q++) s += wp[q];
+        tot[threadIdx.x] = s;
+    }
+    if (threadIdx.x == 0) I.counter[slot] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = tot[k];
+    return true;
+}
+
+CG_DEV bool cg_pcg_decode(const CgInst* insts, const int* wg_inst, const double* ctl, int& b, int& wgi)
+{
+    b = wg_inst[blockIdx.x];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
+    if (insts[b].scal[PCG_STATUS] != 0.0) return false;
+    wgi = blockIdx.x - insts[b].vec0;
+    return true;
+}
+
+// mu, T sums, r = mu, x = 0, z = M^{-1} r, p = 0, rz, ||mu||.
+__global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ insts,
+                                                    const int* __restrict__ wg_inst,
+                                                    const double* __restrict__ ctl)
+{
+    const int b = wg_inst[blockIdx.x];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    const int wgi = blockIdx.x - I.vec0;
+    const int q = wgi * CG_WG + threadIdx.x;
+    const int n = I.n, m = I.m;
+    double v[3] = {0.0, 0.0, 0.0};   // ||mu||^2, r.z, any(mu != 0)
+    auto setk = [&](int k, double mu) {
+        I.yv[k] = mu;
+        I.dir[k] = 0.0;
+        I.uv[k] = 0.0;
+        const double z = (I.Da != nullptr) ? I.Da[k] * mu : mu;
+        I.sv[k] = z;
+        v[0] += mu * mu;
+        v[1] += mu * z;
+        if (mu != 0.0) v[2] = 1.0;
+    };
+    if (q < n) {
+        double a = 0.0, t = 0.0;
+        for (int s = 0; s < I.nstrips; s++) {
+            a += I.rp0[(size_t)q * I.nstrips + s];
+            t += I.rp1[(size_t)q * I.nstrips + s];
+        }
+        I.rowsum[q] = t;
+        setk(q, a);
+    }
+    if (q < m) {
+        double a = 0.0, t = 0.0;
+        for (int r = 0; r < I.nrb; r++) {
+            a += I.cp0[(size_t)r * m + q];
+            t += I.cp1[(size_t)r * m + q];
+        }
+        I.colsum[q] = t;
+        if (q < m - 1) setk(n + q, a);
+    }
+    double tot[3];
+    if (cg_inst_reduce_p<3>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
+        I.scal[PCG_BNORM] = sqrt(tot[0]);
+        I.scal[PCG_RZ] = tot[1];
+        I.scal[PCG_BETA] = 0.0;
+        I.scal[PCG_ITER] = 0.0;
+        I.scal[PCG_STATUS] = (tot[2] != 0.0) ? 0.0 : PCG_ZERO_RHS;
+    }
+}
+
+// w = Q p with p = z + beta*p_prev (the M pass already holds sum_j T_ij p_beta_j
+// and sum_i T_ij p_alpha_i); store p, pw, alpha.
+__global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ insts,
+                                                     const int* __restrict__ wg_inst,
+                                                     const double* __restrict__ ctl)
+{
+    int b, wgi;
+    if (!cg_pcg_decode(insts, wg_inst, ctl, b, wgi)) return;
+    const CgInst& I = insts[b];
+    const int q = wgi * CG_WG + threadIdx.x;
+    const int n = I.n, m = I.m;
+    const double beta = I.scal[PCG_BETA];
+    double v[1] = {0.0};
+    if (q < n) {
+        double a = 0.0;
+        for (int s = 0; s < I.nstrips; s++) a += I.rp0[(size_t)q * I.nstrips + s];
+        const double p = I.sv[q] + beta * I.uv[q];
+        const double w = I.rowsum[q] * p + a;
+        I.uv[q] = p;
+        I.grad[q] = w;
+        v[0] += p * w;
+    }
+    if (q < m - 1) {
+        double a = 0.0;
+        for (int r = 0; r < I.nrb; r++) a += I.cp0[(size_t)r * m + q];
+        const int k = n + q;
+        const double p = I.sv[k] + beta * I.uv[k];
+        const double w = a + I.colsum[q] * p;
+        I.uv[k] = p;
+        I.grad[k] = w;
+        v[0] += p * w;
+    }
+    double tot[1];
+    if (cg_inst_reduce_p<1>(I, wgi, v, tot, 1) && threadIdx.x == 0) {
+        if (tot[0] <= 0.0) {
+            I.scal[PCG_STATUS] = PCG_BREAKDOWN;
+        } else {
+            I.scal[PCG_ALPHA] = I.scal[PCG_RZ] / tot[0];
+        }
+    }
+}
+
+// x += alpha p, r -= alpha w, convergence test, z = M^{-1} r, beta.
+__global__ __launch_bounds__(CG_WG) void k_pcg_update(const CgInst* __restrict__ insts,
+                                                      const int* __restrict__ wg_inst,
+                                                      const double* __restrict__ ctl)
+{
+    int b, wgi;
+    if (!cg_pcg_decode(insts, wg_inst, ctl, b, wgi)) return;
+    const CgInst& I = insts[b];
+    const int k = wgi * CG_WG + threadIdx.x;
+    const double alpha = I.scal[PCG_ALPHA];
+    double v[2] = {0.0, 0.0};
+    if (k < I.N) {
+        I.dir[k] += alpha * I.uv[k];
+        const double r = I.yv[k] - alpha * I.grad[k];
+        I.yv[k] = r;
+        const double z = (I.Da != nullptr) ? I.Da[k] * r : r;
+        I.sv[k] = z;
+        v[0] = r * r;
+        v[1] = r * z;
+    }
+    double tot[2];
+    if (cg_inst_reduce_p<2>(I, wgi, v, tot, 2) && threadIdx.x == 0) {
+        const double it = I.scal[PCG_ITER] + 1.0;
+        I.scal[PCG_ITER] = it;
+        const double tol = ctl[b * CG_NCTL + CTL_STEP];
+        const double maxit = ctl[b * CG_NCTL + CTL_AUX];
+        if (sqrt(tot[0]) < tol * I.scal[PCG_BNORM]) {
+            I.scal[PCG_STATUS] = PCG_CONVERGED;
+        } else {
+            I.scal[PCG_BETA] = tot[1] / I.scal[PCG_RZ];
+            I.scal[PCG_RZ] = tot[1];
+            if (it >= maxit) I.scal[PCG_STATUS] = PCG_MAXIT;
+        }
+    }
+}

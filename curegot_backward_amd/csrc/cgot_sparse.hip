@@ -1,0 +1,404 @@
+// Sparsified Hessian: global top-K of T[:, :m-1] (radix select over the FP64
+// bit patterns, T >= 0 so the bits order like the values), then the pattern
+// in two orders: by row (CSR of W, for the large/small elimination) and by
+// column (CSC of W).  Ties at the K-th value are taken in row-major index order
+// i*(m-1)+j, the same rule as the oracle.
+//
+// Reference: launch_H_sparsification / launch_csr_conversion /
+// recompute_nonzero_values_kernel, src/sinkhorn_splr_kernel.cu:597-906
+// (CUB DeviceTopK + Thrust sort there).
+#include "cgot_common.h"
+#include "cgot_kernels.h"
+
+CG_DEV double cg_T(const CgInst& I, int i, int j)
+{
+    return exp((I.gamma[i] + I.gamma[I.n + j] - I.M[(size_t)i * I.m + j]) * I.inv_reg);
+}
+
+CG_DEV unsigned long long cg_kth(const CgInst& I) { return (unsigned long long)__double_as_longlong(I.scal[CS_KTH_HI]); }
+
+CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const double* ctl,
+                            int& b, int& rb, int& st)
+{
+    const int t = blockIdx.x;
+    b = tile_inst[t];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
+    const CgInst& I = insts[b];
+    const int lt = t - I.tile0;
+    rb = lt / I.nstrips;
+    st = lt - rb * I.nstrips;
+    return true;
+}
+
+// Radix histogram pass.  pass 0: top 12 bits (sign+exponent); pass p>=1: the
+// next 13 bits among keys matching the prefix found so far.
+__global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts,
+                                                const int* __restrict__ tile_inst,
+                                                const double* __restrict__ ctl, int pass)
+{
+    int b, rb, st;
+    if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst& I = insts[b];
+    __shared__ unsigned int h[CG_HBINS];
+    for (int q = threadIdx.x; q < CG_HBINS; q += CG_WG) h[q] = 0u;
+    __syncthreads();
+    const int shift = (pass == 0) ? 52 : 52 - CG_HBITS * pass;
+    const unsigned long long mask = (pass == 0) ? 4095ull : (unsigned long long)(CG_HBINS - 1);
+    const unsigned long long prefix = cg_kth(I);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw, mm = I.m - 1;
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= I.n) break;
+        for (int k = 0; k < I.c; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (j < mm) {
+                const unsigned long long key = cg_key(cg_T(I, i, j));
+                if (pass == 0 || (key >> (shift + CG_HBITS)) == prefix)
+                    atomicAdd(&h[(key >> shift) & mask], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const int nb = (pass == 0) ? 4096 : CG_HBINS;
+    for (int q = threadIdx.x; q < nb; q += CG_WG) {
+        const unsigned int v = h[q];
+        if (v) atomicAdd(&I.hist[q], v);
+    }
+}
+
+// One workgroup per instance: find the bin holding the need-th largest key.
+__global__ __launch_bounds__(CG_WG) void k_hist_scan(const CgInst* __restrict__ insts,
+                                                     const double* __restrict__ ctl, int pass)
+{
+    const int b = blockIdx.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    const int nb = (pass == 0) ? 4096 : CG_HBINS;
+    const int cs = nb / CG_WG;
+    __shared__ unsigned long long csum[CG_WG];
+    __shared__ int sh_chunk;
+    __shared__ unsigned long long sh_rem;
+    unsigned long long need = (pass == 0) ? (unsigned long long)ctl[b * CG_NCTL + CTL_KSEL]
+                                          : (unsigned long long)I.scal[CS_NEED];
+    // chunk t covers bins [nb - (t+1)*cs, nb - t*cs), scanned from the top
+    const int t = threadIdx.x;
+    const int hi = nb - t * cs;
+    unsigned long long s = 0;
+    for (int q = hi - 1; q >= hi - cs; q--) s += I.hist[q];
+    csum[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long acc = 0, tot = 0;
+        for (int q = 0; q < CG_WG; q++) tot += csum[q];
+        if (need > tot) need = tot;
+        if (need == 0) need = 1;
+        int ch = CG_WG - 1;
+        for (int q = 0; q < CG_WG; q++) {
+            if (acc + csum[q] >= need) { ch = q; break; }
+            acc += csum[q];
+        }
+        sh_chunk = ch;
+        sh_rem = need - acc;
+    }
+    __syncthreads();
+    if (t == sh_chunk) {
+        unsigned long long acc = 0, rem = sh_rem;
+        int bstar = hi - cs;
+        for (int q = hi - 1; q >= hi - cs; q--) {
+            const unsigned long long c = I.hist[q];
+            if (acc + c >= rem) { bstar = q; break; }
+            acc += c;
+        }
+        const unsigned long long prefix = (pass == 0) ? 0ull : cg_kth(I);
+        const unsigned long long np = (pass == 0) ? (unsigned long long)bstar
+                                                  : ((prefix << CG_HBITS) | (unsigned long long)bstar);
+        I.scal[CS_KTH_HI] = __longlong_as_double((long long)np);
+        I.scal[CS_NEED] = (double)(rem - acc);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb; q += CG_WG) I.hist[q] = 0u;
+}
+
+// Row-wave kernels: one wave per row; blockIdx.x -> (instance, 4 rows) via row_inst.
+// Count keys above the threshold and ties, per row.
+__global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restrict__ insts,
+                                                          const int* __restrict__ row_inst,
+                                                          const double* __restrict__ ctl)
+{
+    const int b = row_inst[blockIdx.x];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
+    if (i >= I.n) return;
+    const unsigned long long kth = cg_kth(I);
+    const int mm = I.m - 1;
+    int gt = 0, tie = 0;
+    for (int j = lane; j < mm; j += 64) {
+        const unsigned long long key = cg_key(cg_T(I, i, j));
+        gt += (key > kth);
+        tie += (key == kth);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { gt += __shfl_xor(gt, o, 64); tie += __shfl_xor(tie, o, 64); }
+    if (lane == 0) { I.rcnt[i] = gt; I.rtie[i] = tie; }
+}
+
+// Exclusive block scan of per-thread integer totals (256 threads); returns the
+// exclusive prefix of this thread's total and writes the grand total.
+CG_DEV long long cg_block_excl_scan(long long v, long long* lds, long long* total)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int q = 0; q < CG_WG; q++) { const long long x = lds[q]; lds[q] = acc; acc += x; }
+        *total = acc;
+    }
+    __syncthreads();
+    const long long r = lds[threadIdx.x];
+    __syncthreads();
+    return r;
+}
+
+// One workgroup per instance: ties taken per row (row-major order) and the
+// CSR row pointer of the selected pattern.
+__global__ __launch_bounds__(CG_WG) void k_sel_rows_scan(const CgInst* __restrict__ insts,
+                                                         const double* __restrict__ ctl)
+{
+    const int b = blockIdx.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    __shared__ long long lds[CG_WG];
+    __shared__ long long total;
+    const int n = I.n;
+    const int per = (n + CG_WG - 1) / CG_WG;
+    const int r0 = threadIdx.x * per, r1 = min(n, r0 + per);
+    const long long need = (long long)I.scal[CS_NEED];
+    long long tsum = 0;
+    for (int r = r0; r < r1; r++) tsum += I.rtie[r];
+    long long tb = cg_block_excl_scan(tsum, lds, &total);
+    long long ssum = 0;
+    for (int r = r0; r < r1; r++) {
+        const long long tk = I.rtie[r];
+        long long take = need - tb;
+        take = take < 0 ? 0 : (take > tk ? tk : take);
+        tb += tk;
+        I.rtie[r] = (int)take;
+        ssum += I.rcnt[r] + take;
+    }
+    long long off = cg_block_excl_scan(ssum, lds, &total);
+    for (int r = r0; r < r1; r++) {
+        I.w_rowptr[r] = (int)off;
+        off += I.rcnt[r] + I.rtie[r];
+    }
+    if (threadIdx.x == 0) I.w_rowptr[n] = (int)total;
+}
+
+// One wave per row: write the selected columns of row i in ascending order and
+// the tie cut column (selected ties of the row are those with j < rcut[i]).
+__global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restrict__ insts,
+                                                          const int* __restrict__ row_inst,
+                                                          const double* __restrict__ ctl)
+{
+    const int b = row_inst[blockIdx.x];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
+    if (i >= I.n) return;
+    const unsigned long long kth = cg_kth(I);
+    const int mm = I.m - 1;
+    const int off = I.w_rowptr[i];
+    const int take = I.rtie[i];
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    int tseen = 0, wpos = 0, cut = 0;
+    for (int j0 = 0; j0 < mm; j0 += 64) {
+        const int j = j0 + lane;
+        const bool ok = j < mm;
+        const unsigned long long key = ok ? cg_key(cg_T(I, i, j)) : 0ull;
+        const bool isgt = ok && key > kth;
+        const bool istie = ok && key == kth;
+        const unsigned long long tmask = __ballot(istie);
+        const int trank = tseen + __popcll(tmask & lt);
+        const bool tsel = istie && trank < take;
+        const bool sel = isgt || tsel;
+        const unsigned long long smask = __ballot(sel);
+        const int pos = wpos + __popcll(smask & lt);
+        if (sel) I.w_colj[off + pos] = j;
+        if (tsel) cut = max(cut, j + 1);
+        tseen += __popcll(tmask);
+        wpos += __popcll(smask);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cut = max(cut, __shfl_xor(cut, o, 64));
+    if (lane == 0) I.rcut[i] = cut;
+}
+
+CG_DEV bool cg_selected(unsigned long long key, unsigned long long kth, int j, int cut)
+{
+    return key > kth || (key == kth && j < cut);
+}
+
+// Tile pass: selected count per (row block, column).
+__global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restrict__ insts,
+                                                          const int* __restrict__ tile_inst,
+                                                          const double* __restrict__ ctl)
+{
+    int b, rb, st;
+    if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst& I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw, mm = I.m - 1;
+    const unsigned long long kth = cg_kth(I);
+    int cnt[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) cnt[k] = 0;
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= I.n) break;
+        const int cut = I.rcut[i];
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < mm) cnt[k] += cg_selected(cg_key(cg_T(I, i, j)), kth, j, cut);
+        }
+    }
+    __shared__ int lds[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cnt[k];
+    __syncthreads();
+    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+        const int j = s0 + q;
+        if (j < mm) I.ccnt[(size_t)rb * I.m + j] = lds[0][q] + lds[1][q] + lds[2][q] + lds[3][q];
+    }
+}
+
+// One workgroup per instance: column pointers of the CSC and the first output
+// slot of every (row block, column).
+__global__ __launch_bounds__(CG_WG) void k_sel_cols_scan(const CgInst* __restrict__ insts,
+                                                         const double* __restrict__ ctl)
+{
+    const int b = blockIdx.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    __shared__ long long lds[CG_WG];
+    __shared__ long long total;
+    const int mm = I.m - 1, m = I.m;
+    const int per = (mm + CG_WG - 1) / CG_WG;
+    const int c0 = threadIdx.x * per, c1 = min(mm, c0 + per);
+    long long csum = 0;
+    for (int j = c0; j < c1; j++) {
+        int run = 0;
+        for (int r = 0; r < I.nrb; r++) {
+            const int c = I.ccnt[(size_t)r * m + j];
+            I.ccnt[(size_t)r * m + j] = run;
+            run += c;
+        }
+        I.w_colptr[j] = run;   // column total for now
+        csum += run;
+    }
+    long long off = cg_block_excl_scan(csum, lds, &total);
+    for (int j = c0; j < c1; j++) {
+        const int c = I.w_colptr[j];
+        I.w_colptr[j] = (int)off;
+        for (int r = 0; r < I.nrb; r++) I.ccnt[(size_t)r * m + j] += (int)off;
+        off += c;
+    }
+    if (threadIdx.x == 0) I.w_colptr[mm > 0 ? mm : 0] = (int)total;
+}
+
+// Tile pass: write the row indices of the CSC in ascending row order.
+__global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restrict__ insts,
+                                                          const int* __restrict__ tile_inst,
+                                                          const double* __restrict__ ctl)
+{
+    int b, rb, st;
+    if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInst& I = insts[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw, mm = I.m - 1;
+    const unsigned long long kth = cg_kth(I);
+    unsigned int bits[CG_CMAX];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) bits[k] = 0u;
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    for (int r = 0; r < CG_RPW; r++) {
+        const int i = i0 + r;
+        if (i >= I.n) break;
+        const int cut = I.rcut[i];
+#pragma unroll
+        for (int k = 0; k < CG_CMAX; k++) {
+            const int j = s0 + lane + 64 * k;
+            if (k < I.c && j < mm && cg_selected(cg_key(cg_T(I, i, j)), kth, j, cut)) bits[k] |= (1u << r);
+        }
+    }
+    __shared__ int lds[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CG_CMAX; k++) {
+        const int q = lane + 64 * k;
+        const int j = s0 + q;
+        if (k < I.c && j < mm && bits[k]) {
+            int pos = I.ccnt[(size_t)rb * I.m + j];
+            for (int ww = 0; ww < w; ww++) pos += lds[ww][q];
+            for (int r = 0; r < CG_RPW; r++)
+                if (bits[k] & (1u << r)) I.w_rowi[pos++] = i0 + r;
+        }
+    }
+}
+
+// Refresh W values on the current pattern and the diagonal blocks
+// Da = T1 + shift, Db = (T^T 1)[:m-1] + shift.  Reference:
+// recompute_nonzero_values_kernel + write_diagonal_kernel, src/sinkhorn_splr_kernel.cu:554-734.
+__global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restrict__ insts,
+                                                          const int* __restrict__ wg_inst,
+                                                          const double* __restrict__ ctl)
+{
+    const int b = wg_inst[blockIdx.x];
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInst& I = insts[b];
+    const int q = (blockIdx.x - I.vec0) * CG_WG + threadIdx.x;
+    const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
+    if (q < I.n) {
+        for (int p = I.w_rowptr[q]; p < I.w_rowptr[q + 1]; p++) I.w_rval[p] = cg_T(I, q, I.w_colj[p]);
+        I.Da[q] = I.rowsum[q] + sh;
+    }
+    if (q < I.m - 1) {
+        for (int p = I.w_colptr[q]; p < I.w_colptr[q + 1]; p++) I.w_cval[p] = cg_T(I, I.w_rowi[p], q);
+        I.Db[q] = I.colsum[q] + sh;
+    }
+}
+
+// Lower CSR of Hsl in the reference layout (test hook): rows 0..n-1 hold their
+// diagonal, row n+j holds the selected T_ij (ascending i) and then its diagonal.
+// Reference: the CSR built by launch_sphess, src/sinkhorn_splr_kernel.cu:839-906.
+__global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ wg_inst,
+                                                   const double* __restrict__ ctl,
+                                                   double* __restrict__ vals, int* __restrict__ cols,
+                                                   int* __restrict__ rowptr)
+{
+    const int b = wg_inst[blockIdx.x];
+    const CgInst& I = insts[b];
+    const int q = (blockIdx.x - I.vec0) * CG_WG + threadIdx.x;
+    const int n = I.n;
+    if (q < n) {
+        vals[q] = I.Da[q];
+        cols[q] = q;
+        rowptr[q] = q;
+    }
+    if (q < I.m - 1) {
+        int p = n + I.w_colptr[q] + q;
+        rowptr[n + q] = p;
+        for (int e = I.w_colptr[q]; e < I.w_colptr[q + 1]; e++) { vals[p] = I.w_cval[e]; cols[p] = I.w_rowi[e]; p++; }
+        vals[p] = I.Db[q];
+        cols[p] = n + q;
+    }
+    if (q == 0) rowptr[I.N] = n + I.w_colptr[I.m - 1] + (I.m - 1);
+}

@@ -1,0 +1,145 @@
+/*
+ * curegot_amd -- C ABI of the MI355X (gfx950) entropic-OT SPLR library.
+ *
+ * Plain pointers and sizes only.  All array arguments are DEVICE pointers to
+ * FP64 (or int32) memory on the current HIP device; `stream` is a hipStream_t
+ * passed as void* (NULL = default stream).  Every solve runs on that stream and
+ * takes all of its device memory from the caller's workspace (query its size
+ * first); the library allocates only a small pinned host buffer per thread.
+ *
+ * Functions return 0 on success and a negative code on error; the message is
+ * available from cgot_last_error() (thread-local).
+ *
+ * Reference interfaces replaced (benben-cd/curegot_backward):
+ *   cgot_splr_solve     cuda_sinkhorn_splr_bwd      src/sinkhorn.h:36-79, src/sinkhorn_splr.cu:1305-1611
+ *                       (and the pybind entry torch_sinkhorn_splr_bwd, src/solvers_torch.cpp:707-912);
+ *                       B > 1 replaces the sequential loop of splr_ot_batch, curegot/autograd.py:340-367
+ *   cgot_adjoint_solve  solve_backward + pcg + assemble_grad, curegot/backward.py:72-144,
+ *                       curegot/_backward_core.py:31-83, with the diagonal preconditioner of
+ *                       SPLRBackwardContext::apply_diagonal_preconditioner, src/backward_context.cu:642-655
+ *   cgot_bcd_solve      cuda_sinkhorn_bcd            src/solvers.h:4-12, src/sinkhorn_bcd_kernel.cu:429-634
+ *   cgot_T_sparsify     T_computation_sparsify_host  src/sinkhorn_splr_kernel.cu:909-1032 (test hook,
+ *                       exposed to Python as curegot.tests.test_T_computation_sparsify, src/tests.cpp:36-150)
+ */
+#ifndef CUREGOT_AMD_H
+#define CUREGOT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One OT instance of a (possibly ragged) SPLR batch. */
+typedef struct cgot_splr_problem {
+    const double* M;        /* [n*m] row-major cost */
+    const double* a;        /* [n] source marginal (> 0) */
+    const double* b;        /* [m] target marginal (> 0) */
+    int32_t n, m;
+    double reg;             /* entropic regularisation eta > 0 */
+    double tol;             /* stop when ||g||_2 < tol */
+    int32_t max_iter;
+    double density_max;     /* <= 0: default min(10/min(n,m), 1) */
+    double shift_max;       /* < 0: default 1e-3 */
+    int32_t sparsity_pattern_cycle;   /* <= 0: default 30 */
+    int32_t candidate_sinkhorn_iter;  /* < 0: default 3 */
+    const double* x0;       /* [n+m] warm start or NULL */
+    double* plan;           /* [n*m] output T or NULL */
+    double* dual;           /* [n+m] output (alpha, beta), beta[m-1] = 0, or NULL */
+} cgot_splr_problem;
+
+/* Per-instance result and backward-context view (device pointers into the
+ * workspace; valid while the workspace is alive and not reused). */
+typedef struct cgot_splr_info {
+    int32_t niter;
+    int32_t converged;
+    int32_t valid;          /* 0: non-finite objective, no usable context */
+    int32_t low_rank;       /* rank-2 BFGS term active in the context */
+    int32_t fallback;       /* niter == 0: context matrix assembled at the final point */
+    int32_t n, m;
+    double ys;
+    double f;               /* final dual objective */
+    double gnorm;           /* final ||g||_2 */
+    int64_t K;              /* selected entries of T in the context matrix */
+    const double* inv_diag; /* [n+m-1] 1/diag(A) of the context matrix */
+    const double* s;        /* [n+m-1] secant s of the last step */
+    const double* y;        /* [n+m-1] secant y of the last step */
+    const double* u;        /* [n+m-1] A^{-1} y (valid if low_rank) */
+    const double* gamma;    /* [n+m] final duals */
+} cgot_splr_info;
+
+size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
+int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
+                    void* stream, cgot_splr_info* info);
+
+/* Implicit backward of the SPLR plan w.r.t. M for one instance of a batch:
+ * solve Q(T*) lambda = mu by PCG (T* recomputed from M and the duals),
+ * then dL/dM = T*.(lambda_a 1^T + 1 lambda_b^T - G)/reg. */
+typedef struct cgot_adjoint_problem {
+    const double* M;        /* [n*m] */
+    const double* dual;     /* [n+m] converged duals, beta[m-1] = 0 */
+    const double* G;        /* [n*m] upstream gradient dL/dT */
+    int32_t n, m;
+    double reg;
+    const double* inv_diag; /* [n+m-1] Jacobi preconditioner, or NULL for plain CG */
+    double tol;             /* relative residual tolerance */
+    int32_t max_iter;
+    double* lambda;         /* [n+m-1] output */
+    double* grad;           /* [n*m] output dL/dM */
+} cgot_adjoint_problem;
+
+typedef struct cgot_adjoint_info {
+    int32_t iterations;
+    int32_t converged;
+    int32_t status;         /* 1 converged, 2 breakdown (p^T Q p <= 0), 3 max_iter, 4 zero rhs */
+} cgot_adjoint_info;
+
+size_t cgot_adjoint_workspace_size(int32_t B, const cgot_adjoint_problem* probs);
+int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* workspace,
+                       size_t workspace_bytes, void* stream, cgot_adjoint_info* info);
+
+/* Sinkhorn/BCD solve (legacy sinkhorn_bcd).  dual is not gauge-fixed. */
+typedef struct cgot_bcd_problem {
+    const double* M; const double* a; const double* b;
+    int32_t n, m;
+    double reg, tol;
+    int32_t max_iter;
+    const double* x0;       /* [n+m] or NULL */
+    double* plan;           /* [n*m] or NULL */
+    double* dual;           /* [n+m] or NULL */
+} cgot_bcd_problem;
+
+size_t cgot_bcd_workspace_size(int32_t B, const cgot_bcd_problem* probs);
+int cgot_bcd_solve(int32_t B, const cgot_bcd_problem* probs, void* workspace, size_t workspace_bytes,
+                   void* stream, int32_t* niter);
+
+/* One LSE dual update on device (BCD half-sweep), for tests and benchmarks:
+ * which = 0: alpha = reg*(log a - LSE_j((beta_j - M_ij)/reg)) using dual[n:];
+ * which = 1: beta  = reg*(log b - LSE_i((alpha_i - M_ij)/reg)) using dual[:n].
+ * dual is updated in place. */
+size_t cgot_lse_workspace_size(int32_t n, int32_t m);
+int cgot_lse_update(const double* M, const double* a, const double* b, int32_t n, int32_t m,
+                    double reg, int32_t which, int32_t repeats, double* dual, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/* Test hook: T pass + top-K sparsification at gamma = (alpha, beta).
+ * Outputs rowsum [n], colsum [m], grad [n+m-1], scalars {f, gnorm} and the lower
+ * CSR of Hsl = [[diag(T1)+shift, 0], [T_Omega^T, diag(T^T 1)+shift]]
+ * (values/colind [K'+n+m-1], rowptr [n+m]), K' = min(max(K,1), n(m-1)).
+ * If rhs != NULL also solves Hsl_sym x = rhs ([n+m-1]) into sol. */
+size_t cgot_T_sparsify_workspace_size(int32_t n, int32_t m, int64_t K);
+int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const double* b,
+                    int32_t n, int32_t m, double reg, double shift, int64_t K,
+                    double* rowsum, double* colsum, double* grad, double* scalars,
+                    double* csr_values, int32_t* csr_colind, int32_t* csr_rowptr,
+                    const double* rhs, double* sol,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+const char* cgot_last_error(void);
+const char* cgot_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
