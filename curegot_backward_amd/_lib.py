@@ -22,7 +22,7 @@ class SplrProblem(ctypes.Structure):
         ("max_iter", ctypes.c_int32),
         ("density_max", ctypes.c_double), ("shift_max", ctypes.c_double),
         ("sparsity_pattern_cycle", ctypes.c_int32), ("candidate_sinkhorn_iter", ctypes.c_int32),
-        ("x0", _dp), ("plan", _dp), ("dual", _dp),
+        ("x0", _dp), ("plan", _dp), ("dual", _dp), ("trace", _dp),
     ]
 
 
@@ -69,6 +69,7 @@ EXPORTS = (
     "cgot_bcd_workspace_size", "cgot_bcd_solve",
     "cgot_lse_workspace_size", "cgot_lse_update",
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
+    "cgot_profile_enable", "cgot_profile_reset", "cgot_profile_read",
     "cgot_last_error", "cgot_version",
 )
 
@@ -108,6 +109,12 @@ def load():
     L.cgot_T_sparsify.argtypes = [vp, vp, vp, vp, i32, i32, dbl, dbl, i64, vp, vp, vp, vp, vp, vp, vp,
                                   vp, vp, vp, sz, vp]
     L.cgot_T_sparsify.restype = ctypes.c_int
+    L.cgot_profile_enable.argtypes = [i32]
+    L.cgot_profile_enable.restype = None
+    L.cgot_profile_reset.argtypes = []
+    L.cgot_profile_reset.restype = None
+    L.cgot_profile_read.argtypes = [i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(i64)]
+    L.cgot_profile_read.restype = ctypes.c_int
     L.cgot_last_error.restype = ctypes.c_char_p
     L.cgot_version.restype = ctypes.c_char_p
     _lib = L
@@ -120,3 +127,23 @@ def check(rc: int, what: str):
         if rc == -2:
             raise ValueError(f"{what}: {msg}")
         raise RuntimeError(f"{what}: {msg}")
+
+
+PROFILE_CLASSES = {"tpass": 0, "row_lse": 1, "col_lse": 2, "qv": 3, "chol_update": 4,
+                   "chol_panel": 5, "trsv": 6}
+
+
+def profile_enable(on: bool):
+    load().cgot_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    load().cgot_profile_reset()
+
+
+def profile_read(name: str):
+    """(total ms, total algorithmic work, launches) of a kernel class."""
+    ms, work, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)
+    check(load().cgot_profile_read(PROFILE_CLASSES[name], ctypes.byref(ms), ctypes.byref(work),
+                                   ctypes.byref(n)), "profile_read")
+    return ms.value, work.value, n.value

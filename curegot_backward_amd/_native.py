@@ -169,6 +169,7 @@ def _make_problem(M, a, b, reg, tol, max_iter, kwargs, plan, dual, x0):
     p.x0 = x0.data_ptr() if x0 is not None else None
     p.plan = plan.data_ptr()
     p.dual = dual.data_ptr()
+    p.trace = None
     return p
 
 
@@ -188,7 +189,8 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
     """Ragged batch of backward-capable SPLR solves in one native call.
 
     Ms/as_/bs are sequences of [n_i, m_i], [n_i], [m_i] tensors (shapes may
-    differ); reg may be a float or one value per instance.  Returns one result
+    differ); reg may be a float or one value per instance.  allow_invalid=True
+    returns a non-finite instance with context None instead of raising.  Returns one result
     dict per instance, each like sinkhorn_splr_bwd's.  Every instance's result
     is bitwise the same as solving it alone.
     """
@@ -205,6 +207,8 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
         raise RuntimeError('context_mode must be "adopt" or "snapshot"')
     device = _target_device(Ms[0])
     x0s = kwargs.get("x0", None)
+    want_trace = bool(kwargs.get("trace", False))
+    traces = []
     L = _lib.load()
     probs = (_lib.SplrProblem * B)()
     keep = []
@@ -224,6 +228,10 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
         plan = torch.empty((n, m), dtype=torch.float64, device=device)
         dual = torch.empty(n + m, dtype=torch.float64, device=device)
         probs[i] = _make_problem(M, a, b, regs[i], tol, max_iter, kwargs, plan, dual, x0)
+        if want_trace:
+            tr = (ctypes.c_double * (8 * max(int(max_iter), 1)))()
+            traces.append(tr)
+            probs[i].trace = ctypes.cast(tr, ctypes.c_void_p).value
         keep.append((M, a, b, x0))
         outs.append((plan, dual))
     with torch.cuda.device(device):
@@ -233,14 +241,15 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
         rc = L.cgot_splr_solve(B, probs, ws.data_ptr(), nbytes, _stream(device), infos)
         _lib.check(rc, "sinkhorn_splr_bwd")
     results = []
+    allow_invalid = bool(kwargs.get("allow_invalid", False))
     for i in range(B):
         info = infos[i]
         plan, dual = outs[i]
-        if not info.valid:
+        if not info.valid and not allow_invalid:
             raise RuntimeError(
                 "torch_sinkhorn_splr_bwd: forward pass failed (non-finite objective value); "
-                "no backward context available")
-        ctx = SPLRBackwardContext(keep[i][0], dual, regs[i], info, ws)
+                f"no backward context available (instance {i}, shape {info.n}x{info.m})")
+        ctx = SPLRBackwardContext(keep[i][0], dual, regs[i], info, ws) if info.valid else None
         src = Ms[i]
         res = {
             "niter": int(info.niter),
@@ -248,7 +257,13 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
             "plan": plan if (src.is_cuda and src.dtype == torch.float64) else plan.to(src.device, src.dtype),
             "dual": dual if (src.is_cuda and src.dtype == torch.float64) else dual.to(src.device, src.dtype),
             "context": ctx,
+            "valid": bool(info.valid),
+            "gnorm": float(info.gnorm),
+            "objective": float(info.f),
         }
+        if want_trace:
+            import numpy as _np
+            res["trace"] = _np.frombuffer(traces[i], dtype=_np.float64).reshape(-1, 8)[: int(info.niter)].copy()
         if verbose > 0:
             print(f"sinkhorn_splr_bwd (HIP) [{info.n} x {info.m}] niter={info.niter} "
                   f"converged={bool(info.converged)} gnorm={info.gnorm:.3e}")

@@ -78,6 +78,7 @@ struct CgInst {
     int P;                 // min(n, m-1) Schur size
     int beta_side;         // 1: Schur onto beta (P = m-1), 0: onto alpha (P = n)
     double* S;             // [P*P]
+    double* Ldg;           // [nblk*CG_NB*CG_NB] Cholesky factors of the diagonal blocks
     double* rhs;           // [2*P] Schur right-hand sides / solutions
     double* ltmp;          // [2*max(n,m)] Schur solutions
     double* inv_diag;      // [N] 1/diag(A) of the context matrix

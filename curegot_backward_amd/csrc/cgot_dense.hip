@@ -7,8 +7,9 @@
 // S is symmetric, strictly diagonally dominant (margin >= shift*reg) with a
 // non-positive off-diagonal, so an unpivoted Cholesky is stable.
 //
-// Dense S is row-major P x P (lower triangle used).  The factorization is a
-// right-looking blocked Cholesky with CG_NB = 64:
+// Dense S is row-major P x P (lower triangle used); the factors of the
+// diagonal blocks live in Ldg, the panels below them in S.  The factorization
+// is a right-looking blocked Cholesky with CG_NB = 64:
 //   k_chol_panel(kb):  factor the diagonal block in LDS, solve the panel rows
 //                      below it, and the forward substitution of block kb;
 //   k_chol_update(kb): trailing update S22 -= L21 L21^T on lower tiles, plus the
@@ -213,10 +214,13 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
     }
     const bool ok = cg_lds_potrf(L, nb, dsq);
     if (rt == 0) {
+        // The factor goes to a side buffer, never into S: other workgroups of
+        // this launch may still be loading the unfactored block from S.
         if (!ok && threadIdx.x == 0) I.scal[CS_PIVOT] = 1.0;
-        for (int e = threadIdx.x; e < nb * nb; e += CG_WG) {
-            const int r = e / nb, c = e % nb;
-            if (c <= r) S[(size_t)(c0 + r) * P + c0 + c] = L[r][c];
+        double* Ld = I.Ldg + (size_t)kb * CG_NB * CG_NB;
+        for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
+            const int r = e / CG_NB, c = e % CG_NB;
+            Ld[e] = L[r][c];
         }
         // forward substitution of the block: y = L11^{-1} b (two right-hand sides)
         if (threadIdx.x < 2) {
@@ -321,11 +325,12 @@ __global__ __launch_bounds__(CG_WG) void k_trsv_back(const CgInst* __restrict__ 
     const int c0 = kb * CG_NB;
     const int nb = min(CG_NB, P - c0);
     const double* S = I.S;
+    const double* Ld = I.Ldg + (size_t)kb * CG_NB * CG_NB;
     __shared__ double L[CG_NB][CG_NB + 1];
     __shared__ double xs[2][CG_NB];
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = e / CG_NB, c = e % CG_NB;
-        L[r][c] = (r < nb && c <= r) ? S[(size_t)(c0 + r) * P + c0 + c] : 0.0;
+        L[r][c] = (r < nb && c <= r) ? Ld[e] : 0.0;
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;

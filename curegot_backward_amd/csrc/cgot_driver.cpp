@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -78,6 +79,59 @@ struct Bump {
 
 constexpr int kCtlSlots = 16;
 
+// ---------------------------------------------------------------------------
+// Optional per-kernel-class timing with HIP events on the solve stream
+// (enabled by cgot_profile_enable; used by the benchmark for roofline figures).
+// ---------------------------------------------------------------------------
+enum { PROF_TPASS = 0, PROF_ROWLSE, PROF_COLLSE, PROF_QV, PROF_CHOL_UPDATE, PROF_CHOL_PANEL,
+       PROF_TRSV, PROF_NCLS = 8 };
+struct ProfRec { hipEvent_t a, b; int cls; double work; };
+bool g_prof = false;
+std::vector<ProfRec> g_prof_pending;
+std::vector<hipEvent_t> g_prof_pool;
+double g_prof_ms[PROF_NCLS] = {0};
+double g_prof_work[PROF_NCLS] = {0};
+long long g_prof_count[PROF_NCLS] = {0};
+
+hipEvent_t prof_event()
+{
+    if (!g_prof_pool.empty()) { hipEvent_t e = g_prof_pool.back(); g_prof_pool.pop_back(); return e; }
+    hipEvent_t e;
+    CG_HIP(hipEventCreate(&e));
+    return e;
+}
+
+struct ProfScope {
+    hipStream_t s; ProfRec r; bool on;
+    ProfScope(hipStream_t st, int cls, double work) : s(st), on(g_prof)
+    {
+        if (!on) return;
+        r.a = prof_event(); r.b = prof_event(); r.cls = cls; r.work = work;
+        CG_HIP(hipEventRecord(r.a, s));
+    }
+    ~ProfScope()
+    {
+        if (!on) return;
+        (void)hipEventRecord(r.b, s);
+        g_prof_pending.push_back(r);
+    }
+};
+
+void prof_collect()
+{
+    for (auto& r : g_prof_pending) {
+        float ms = 0.f;
+        CG_HIP(hipEventSynchronize(r.b));
+        CG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        g_prof_ms[r.cls] += ms;
+        g_prof_work[r.cls] += r.work;
+        g_prof_count[r.cls] += 1;
+        g_prof_pool.push_back(r.a);
+        g_prof_pool.push_back(r.b);
+    }
+    g_prof_pending.clear();
+}
+
 int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // Geometry common to every instance kind.
@@ -117,6 +171,7 @@ struct Batch {
     int slot = 0;              // next ctl slot
     double* hctl = nullptr;    // pinned, kCtlSlots * B * CG_NCTL
     double* hscal = nullptr;   // pinned, B * CG_NSCAL
+    std::map<const double*, const double*> hmap;   // device ctl slot -> host copy
 
     void layout_common(Bump& bp, int nblk_max)
     {
@@ -170,6 +225,8 @@ struct Batch {
         double* hs = hctl + (size_t)slot * B * CG_NCTL;
         double* ds = d_ctl + (size_t)slot * B * CG_NCTL;
         std::memcpy(hs, c.data(), sizeof(double) * B * CG_NCTL);
+        last_hctl = hs;
+        hmap[ds] = hs;
         CG_HIP(hipMemcpyAsync(ds, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
         slot++;
         return ds;
@@ -179,7 +236,20 @@ struct Batch {
     {
         CG_HIP(hipStreamSynchronize(stream));
         slot = 0;
+        if (!g_prof_pending.empty()) prof_collect();
     }
+
+    // Algorithmic bytes of one M pass over the active instances of control c:
+    // 8nm for M plus `vec_bytes` per (n+m) entry.
+    double mpass_bytes(const double* hc, double vec_bytes) const
+    {
+        double t = 0.0;
+        for (int b = 0; b < B; b++)
+            if (hc[(size_t)b * CG_NCTL + CTL_ACTIVE] != 0.0)
+                t += 8.0 * (double)h[b].n * (double)h[b].m + vec_bytes * (double)(h[b].n + h[b].m);
+        return t;
+    }
+    const double* last_hctl = nullptr;
 
     // Read every instance's scalars (one copy + one sync for the batch).
     void read_scal()
@@ -190,9 +260,16 @@ struct Batch {
     double sc(int b, int k) const { return hscal[(size_t)b * CG_NSCAL + k]; }
 
     // ---- launches ----
-    void mpass(void (*k)(const CgInst*, const int*, const double*, int), const double* c, int mode)
+    void mpass(void (*k)(const CgInst*, const int*, const double*, int), const double* c, int mode,
+               int cls = -1, double vec_bytes = 0.0)
     {
-        if (ntiles) hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c, mode);
+        if (!ntiles) return;
+        if (g_prof && cls >= 0) {
+            ProfScope ps(stream, cls, mpass_bytes(hmap[c], vec_bytes));
+            hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c, mode);
+        } else {
+            hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c, mode);
+        }
     }
     void vec(void (*k)(const CgInst*, const int*, const double*, int), const double* c, int mode)
     {
@@ -226,19 +303,19 @@ std::vector<double> ctl_vec(int B) { return std::vector<double>((size_t)B * CG_N
 // T pass (M pass + finalize) for the instances marked active in c.
 void tpass(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_tpass, c, mode);
+    bt.mpass(k_tpass, c, mode, PROF_TPASS, mode == CG_PT_TRIAL ? 24.0 : 16.0);
     bt.vec(k_tpass_fin, c, mode);
 }
 
 void row_lse(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_row_lse, c, mode);
+    bt.mpass(k_row_lse, c, mode, PROF_ROWLSE, 16.0);
     bt.vec(k_rowlse_fin, c, mode);
 }
 
 void col_lse(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_col_lse, c, mode);
+    bt.mpass(k_col_lse, c, mode, PROF_COLLSE, 16.0);
     bt.vec(k_collse_fin, c, mode);
 }
 
@@ -420,6 +497,7 @@ struct HostInst {
     bool analyze = false, update = false;
     Wolfe ls;
     bool in_ls = false;
+    double* trace = nullptr;
 };
 
 void resolve_defaults(const cgot_splr_problem& p, double& dmax, double& shift_max, int& cycle, int& cand)
@@ -501,6 +579,7 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.rcut = bp.take<int>(n);
         I.ccnt = bp.take<int>((size_t)I.nrb * m);
         I.S = bp.take<double>((size_t)I.P * I.P);
+        I.Ldg = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
         I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
         I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
         I.inv_diag = bp.take<double>(N);
@@ -624,18 +703,31 @@ struct SplrSolver {
         bt.vec0(k_schur_rhs, dc);
         for (int kb = 0; kb < nblk_max; kb++) {
             const int tp = total(1 + kb);
-            if (tp > 0)
+            if (tp > 0) {
+                ProfScope ps(bt.stream, PROF_CHOL_PANEL, 0.0);
                 hipLaunchKernelGGL(k_chol_panel, dim3(tp), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(1 + kb), kb);
+            }
             const int tu = total(1 + nblk_max + kb);
-            if (tu > 0)
+            if (tu > 0) {
+                // algorithmic flops: 2*64^3 per full lower tile (diagonal tiles count half)
+                double fl = 0.0;
+                for (int b = 0; b < B; b++) {
+                    const int nb = on[b] ? hi[b].nblk : 0;
+                    const int T = nb - kb - 1;
+                    if (T > 0) fl += 2.0 * CG_NB * CG_NB * CG_NB * (T * (T - 1) / 2 + 0.5 * T);
+                }
+                ProfScope ps(bt.stream, PROF_CHOL_UPDATE, fl);
                 hipLaunchKernelGGL(k_chol_update, dim3(tu), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
                                    dtab(1 + nblk_max + kb), kb);
+            }
         }
         for (int kb = nblk_max - 1; kb >= 0; kb--) {
             const int tt = total(1 + 2 * nblk_max + kb);
-            if (tt > 0)
+            if (tt > 0) {
+                ProfScope ps(bt.stream, PROF_TRSV, 0.0);
                 hipLaunchKernelGGL(k_trsv_back, dim3(tt), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
                                    dtab(1 + 2 * nblk_max + kb), kb);
+            }
         }
         bt.vec0(k_schur_back, dc);
     }
@@ -663,6 +755,7 @@ struct SplrSolver {
             H.K = 0;
             H.P = bt.h[b].P;
             H.nblk = cdiv(H.P, CG_NB);
+            H.trace = p.trace;
         }
     }
 
@@ -814,6 +907,12 @@ struct SplrSolver {
                     H.density *= bad ? 1.1 : 0.99;
                     H.density = std::min(H.dmax, std::max(H.dmin, H.density));
                 }
+                if (H.trace) {
+                    double* tr = H.trace + 8 * (size_t)iter;
+                    tr[0] = H.f; tr[1] = H.gnorm; tr[2] = H.ls.result; tr[3] = bt.sc(b, CS_LOWRANK);
+                    tr[4] = use_bcd[b]; tr[5] = (double)H.K; tr[6] = H.ls.trials;
+                    tr[7] = (H.ls.recompute && H.ls.result != H.ls.last_eval) ? 1.0 : 0.0;
+                }
                 const bool conv_after = std::isfinite(H.gnorm) && (H.gnorm < H.tol);
                 const bool max_reached = (iter + 1 >= H.max_iter);
                 const bool inv_after = !std::isfinite(H.f) || !std::isfinite(H.gnorm);
@@ -949,6 +1048,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         for (int b = 0; b < B; b++) c2[(size_t)b * CG_NCTL + CTL_ACTIVE] = running[b] ? 1.0 : 0.0;
         const double* d2 = bt.ctl(c2);
         for (int k = 0; k < chunk; k++) {
+            ProfScope ps(bt.stream, PROF_QV, bt.mpass_bytes(bt.hmap[d2], 24.0));
             hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, d2,
                                (const double* const*)nullptr, (const double* const*)nullptr, (int)CG_QV_PCG);
             bt.vec0(k_pcg_q_fin, d2);
@@ -1095,6 +1195,26 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
 extern "C" {
 
 const char* cgot_last_error(void) { return g_last_error.c_str(); }
+
+void cgot_profile_enable(int32_t on)
+{
+    g_prof = (on != 0);
+}
+
+void cgot_profile_reset(void)
+{
+    for (int k = 0; k < PROF_NCLS; k++) { g_prof_ms[k] = 0; g_prof_work[k] = 0; g_prof_count[k] = 0; }
+}
+
+int cgot_profile_read(int32_t cls, double* ms_total, double* work_total, int64_t* launches)
+{
+    if (cls < 0 || cls >= PROF_NCLS) return -2;
+    try { prof_collect(); } catch (const std::exception& e) { g_last_error = e.what(); return -1; }
+    *ms_total = g_prof_ms[cls];
+    *work_total = g_prof_work[cls];
+    *launches = g_prof_count[cls];
+    return 0;
+}
 const char* cgot_version(void) { return "curegot_amd 0.1.0 gfx950"; }
 
 size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs)

@@ -47,6 +47,9 @@ typedef struct cgot_splr_problem {
     const double* x0;       /* [n+m] warm start or NULL */
     double* plan;           /* [n*m] output T or NULL */
     double* dual;           /* [n+m] output (alpha, beta), beta[m-1] = 0, or NULL */
+    double* trace;          /* HOST [8*max_iter] per-iteration record or NULL:
+                               f, ||g||, step, low_rank, sinkhorn candidate taken, K,
+                               line-search trials, recomputed */
 } cgot_splr_problem;
 
 /* Per-instance result and backward-context view (device pointers into the
@@ -135,6 +138,15 @@ int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const
                     double* csr_values, int32_t* csr_colind, int32_t* csr_rowptr,
                     const double* rhs, double* sol,
                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Kernel timing (HIP events around each launch on the solve stream), for
+ * roofline figures.  Classes: 0 T pass (bytes), 1 row LSE (bytes), 2 column LSE
+ * (bytes), 3 adjoint Q.v pass (bytes), 4 Cholesky trailing update (flops),
+ * 5 Cholesky panel, 6 backward substitution.  Not thread-safe; no counterpart
+ * in the reference (its Timer, src/timer.h, prints host wall times). */
+void cgot_profile_enable(int32_t on);
+void cgot_profile_reset(void);
+int cgot_profile_read(int32_t cls, double* ms_total, double* work_total, int64_t* launches);
 
 const char* cgot_last_error(void);
 const char* cgot_version(void);

@@ -1,0 +1,61 @@
+"""Ragged batching: every instance's result is bitwise the same alone and in a
+batch, and splr_ot_batch equals per-slice splr_ot (reference validate_stage5_api.py:269-298)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cg():
+    import curegot_backward_amd
+    return curegot_backward_amd
+
+
+def test_ragged_batch_bitwise_equals_single(cg):
+    from curegot_backward_amd.workloads import ragged_batch
+    probs = ragged_batch(B=10, seed=3, lo=40, hi=400)
+    Ms = [p[0].cuda() for p in probs]
+    As = [p[1].cuda() for p in probs]
+    Bs = [p[2].cuda() for p in probs]
+    etas = [p[3] for p in probs]
+    outs = cg.sinkhorn_splr_bwd_batch(Ms, As, Bs, etas, 1e-8, 1000)
+    for i in range(len(probs)):
+        print("instance", i, tuple(Ms[i].shape), etas[i], flush=True)
+        single = cg.sinkhorn_splr_bwd(Ms[i], As[i], Bs[i], etas[i], 1e-8, 1000)
+        assert outs[i]["niter"] == single["niter"]
+        assert torch.equal(outs[i]["plan"], single["plan"])
+        assert torch.equal(outs[i]["dual"], single["dual"])
+
+
+def test_splr_ot_batch_equals_per_slice(cg):
+    from curegot_backward_amd.workloads import make_problem
+    B, n, m = 4, 50, 60
+    Ms, As, Bs = zip(*[make_problem(n, m, s) for s in range(B)])
+    M = torch.stack(Ms).cuda().requires_grad_(True)
+    a, b = torch.stack(As).cuda(), torch.stack(Bs).cuda()
+    T = cg.torch.splr_ot_batch(M, a, b, eta=0.05, tol=1e-8)
+    (T ** 2).sum().backward()
+    for i in range(B):
+        Mi = M[i].detach().clone().requires_grad_(True)
+        Ti = cg.torch.splr_ot(Mi, a[i], b[i], eta=0.05, tol=1e-8)
+        (Ti ** 2).sum().backward()
+        assert torch.equal(T[i].detach(), Ti.detach())
+        assert (M.grad[i] - Mi.grad).abs().max() <= 1e-12 * Mi.grad.abs().max()
+
+
+def test_ragged_list_batch_with_backward(cg):
+    from curegot_backward_amd.workloads import make_problem
+    shapes = [(30, 45), (64, 20), (17, 17)]
+    Ms = [make_problem(n, m, i)[0].cuda().requires_grad_(True) for i, (n, m) in enumerate(shapes)]
+    As = [make_problem(n, m, i)[1].cuda() for i, (n, m) in enumerate(shapes)]
+    Bs = [make_problem(n, m, i)[2].cuda() for i, (n, m) in enumerate(shapes)]
+    Ts = cg.torch.splr_ot_batch(Ms, As, Bs, eta=[0.05, 0.02, 0.1], tol=1e-8)
+    sum((T ** 2).sum() for T in Ts).backward()
+    for i, (n, m) in enumerate(shapes):
+        Mi = Ms[i].detach().clone().requires_grad_(True)
+        Ti = cg.torch.splr_ot(Mi, As[i], Bs[i], eta=[0.05, 0.02, 0.1][i], tol=1e-8)
+        (Ti ** 2).sum().backward()
+        assert torch.equal(Ts[i].detach(), Ti.detach())
+        assert torch.equal(Ms[i].grad, Mi.grad)

@@ -1,0 +1,144 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle on
+the same seeded inputs, and against the committed fixture."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests import _dense
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def cg():
+    import curegot_backward_amd
+    return curegot_backward_amd
+
+
+def dev(x):
+    return torch.as_tensor(x, dtype=torch.float64).cuda()
+
+
+def rel(x, y):
+    x = x.detach().cpu().numpy() if torch.is_tensor(x) else np.asarray(x)
+    return float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))
+
+
+def test_golden_readme_problem(cg):
+    z = np.load(os.path.join(HERE, "golden", "readme_c1.npz"))
+    out = cg.sinkhorn_splr_bwd(dev(z["M"]), dev(z["a"]), dev(z["b"]), float(z["eta"]), float(z["tol"]), 1000)
+    print("readme problem: niter", out["niter"], flush=True)
+    assert out["converged"] and out["niter"] == int(z["niter"])
+    assert rel(out["plan"], z["plan"]) < 1e-10
+    assert rel(out["dual"], z["dual"]) < 1e-10
+    res = cg.solve_backward(out["plan"], dev(z["G"]), out["context"], float(z["eta"]))
+    assert res["converged"] and res["iterations"] == int(z["bwd_iters"])
+    assert rel(res["grad"], z["grad"]) < 1e-10
+    assert rel(res["lambda"], z["lam"]) < 1e-9
+
+
+CASES = [(40, 30, 0.05, 1), (30, 40, 0.01, 2), (100, 120, 0.05, 3), (129, 65, 0.02, 4), (65, 200, 0.1, 5),
+         (7, 1, 0.05, 6), (1, 9, 0.05, 7), (300, 257, 0.01, 8), (2, 2, 0.5, 9)]
+
+
+@pytest.mark.parametrize("n,m,eta,seed", CASES)
+def test_forward_backward_vs_oracle(cg, n, m, eta, seed):
+    from curegot_backward_amd.workloads import make_problem
+    print(f"case n={n} m={m} eta={eta} seed={seed}", flush=True)
+    M, a, b = make_problem(n, m, seed)
+    ref = oracle.splr_bwd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-8, max_iter=500)
+    out = cg.sinkhorn_splr_bwd(M.cuda(), a.cuda(), b.cuda(), eta, 1e-8, 500)
+    assert out["niter"] == ref["niter"] and out["converged"] == ref["converged"]
+    assert rel(out["plan"], ref["plan"]) < 1e-10
+    ctx = out["context"]
+    assert ctx.low_rank_enabled == ref["low_rank"] and ctx.is_fallback == ref["fallback"]
+    np.testing.assert_allclose(ctx.inv_diag().cpu().numpy(), ref["inv_diag"], rtol=1e-10)
+    G = torch.randn(n, m, dtype=torch.float64, generator=torch.Generator().manual_seed(seed))
+    for pre in ("diag", "none"):
+        res = cg.solve_backward(out["plan"], G.cuda(), ctx, eta, preconditioner=pre)
+        rb = oracle.backward(M.numpy(), ref["dual"], G.numpy(), eta, ctx=ref, preconditioner=pre)
+        assert res["converged"] == rb["converged"]
+        # scale of dL/dM: |G| |T| / eta (n == 1 or m == 1 give an identically zero gradient)
+        scale = float((np.abs(G.numpy()) * ref["plan"]).max() / eta)
+        err = float(np.abs(res["grad"].cpu().numpy() - rb["grad"]).max())
+        assert err <= 1e-10 * max(np.abs(rb["grad"]).max(), 1e-4 * scale), (err, scale)
+        if min(n, m) > 1:
+            assert res["iterations"] == rb["iterations"]
+
+
+def test_sparsify_hook_matches_dense(cg):
+    # reference test/test_csr.py:16-157 through the device hook
+    from scipy.sparse import csr_matrix
+    np.random.seed(123)
+    n, m, K, reg, shift = 6, 3, 10, 0.1, 0.01
+    alpha = np.random.normal(scale=0.1, size=n)
+    beta = np.random.normal(scale=0.1, size=m)
+    M = np.abs(np.random.normal(scale=0.1, size=(n, m)))
+    a = np.random.uniform(size=n); a /= a.sum()
+    b = np.random.uniform(size=m); b /= b.sum()
+    T = np.exp((alpha[:, None] + beta[None, :] - M) / reg)
+    rhs = np.random.default_rng(1).normal(size=n + m - 1)
+    r = cg.tests.test_T_computation_sparsify(alpha, beta, torch.tensor(M), a, b, reg, shift, K, rhs=rhs)
+    np.testing.assert_allclose(r["Trowsums"].cpu().numpy(), T.sum(1), atol=1e-12)
+    np.testing.assert_allclose(r["Tcolsums"].cpu().numpy(), T.sum(0), atol=1e-12)
+    assert abs(r["objfn"] - (reg * T.sum() - alpha @ a - beta @ b)) < 1e-12
+    Tt = T.copy(); Tt[:, -1] = 0
+    topk = np.argpartition(-Tt.flatten(), K)[:K]
+    Tsp = np.zeros(n * m); Tsp[topk] = Tt.flatten()[topk]; Tsp = Tsp.reshape(n, m)
+    H = np.diag(np.concatenate((T.sum(1), T.sum(0)[:-1])) + shift)
+    H[n:, :n] = Tsp[:, :-1].T
+    Hc = csr_matrix(H)
+    np.testing.assert_allclose(r["csr_val"].cpu().numpy(), Hc.data, atol=1e-12)
+    np.testing.assert_array_equal(r["csr_colind"].cpu().numpy(), Hc.indices)
+    np.testing.assert_array_equal(r["csr_rowptr"].cpu().numpy(), Hc.indptr)
+    Hs = _dense.csr_dense_sym(Hc.data, Hc.indices, Hc.indptr, n + m - 1)
+    np.testing.assert_allclose(r["sol"].cpu().numpy(), np.linalg.solve(Hs, rhs), rtol=1e-10)
+
+
+@pytest.mark.parametrize("n,m,K", [(200, 150, 600), (150, 200, 600), (300, 300, 3000), (70, 130, 1)])
+def test_newton_solve_vs_dense(cg, n, m, K):
+    # multi-block dense Cholesky (P > 64) on the Schur complement vs numpy.linalg.solve
+    print(f"newton n={n} m={m} K={K}", flush=True)
+    rng = np.random.default_rng(n + m)
+    M = rng.random((n, m))
+    alpha = rng.normal(scale=0.01, size=n)
+    beta = rng.normal(scale=0.01, size=m)
+    a = np.full(n, 1 / n); b = np.full(m, 1 / m)
+    rhs = rng.normal(size=n + m - 1)
+    r = cg.tests.test_T_computation_sparsify(alpha, beta, torch.tensor(M), a, b, 0.05, 1e-4, K, rhs=rhs)
+    A = _dense.csr_dense_sym(r["csr_val"].cpu().numpy(), r["csr_colind"].cpu().numpy(),
+                             r["csr_rowptr"].cpu().numpy(), n + m - 1)
+    x = np.linalg.solve(A, rhs)
+    assert rel(r["sol"], x) < 1e-9
+
+
+@pytest.mark.parametrize("n,m,which", [(100, 700, 0), (700, 100, 1), (33, 1031, 1), (1031, 33, 0)])
+def test_lse_update_vs_oracle(cg, n, m, which):
+    from curegot_backward_amd import _native
+    rng = np.random.default_rng(n * m)
+    M = rng.random((n, m))
+    a = rng.random(n) + 0.5; a /= a.sum()
+    b = rng.random(m) + 0.5; b /= b.sum()
+    dual = rng.normal(scale=0.05, size=n + m)
+    d = dev(dual)
+    _native.lse_update(dev(M), dev(a), dev(b), d, 0.02, which)
+    if which == 0:
+        ref = oracle.row_lse_update(M, dual[n:], np.log(a), 0.02)
+        assert rel(d[:n], ref) < 1e-12
+    else:
+        ref = oracle.col_lse_update(M, dual[:n], np.log(b), 0.02)
+        assert rel(d[n:], ref) < 1e-12
+
+
+@pytest.mark.parametrize("n,m,eta", [(80, 60, 0.01), (60, 80, 0.05)])
+def test_sinkhorn_bcd_vs_oracle(cg, n, m, eta):
+    from curegot_backward_amd.workloads import make_problem
+    M, a, b = make_problem(n, m, 3)
+    ref = oracle.sinkhorn_bcd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-9, max_iter=20000)
+    out = cg.sinkhorn_bcd(M.cuda(), a.cuda(), b.cuda(), eta, 1e-9, 20000)
+    assert abs(out["niter"] - ref["niter"]) <= 1
+    assert rel(out["plan"], ref["plan"]) < 1e-9

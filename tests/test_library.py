@@ -1,0 +1,42 @@
+"""The HIP library loads and exports every symbol include/curegot_amd.h declares
+(no compute calls: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+from curegot_backward_amd import _lib
+from curegot_backward_amd.workloads import ragged_sizes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "curegot_amd.h")).read()
+    return sorted(set(re.findall(r"\b(cgot_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def test_exports_match_header():
+    L = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(_lib.EXPORTS) <= set(syms)
+    assert L.cgot_version().decode().startswith("curegot_amd")
+
+
+def test_workspace_size_is_host_only():
+    L = _lib.load()
+    sizes = ragged_sizes(8)
+    probs = (_lib.SplrProblem * len(sizes))()
+    for i, (n, m) in enumerate(sizes):
+        probs[i].M = probs[i].a = probs[i].b = 16
+        probs[i].n, probs[i].m, probs[i].reg, probs[i].tol, probs[i].max_iter = n, m, 0.05, 1e-8, 100
+        probs[i].density_max = -1.0
+        probs[i].shift_max = -1.0
+    ws = L.cgot_splr_workspace_size(len(sizes), probs)
+    # at least the dense Schur blocks
+    assert ws > sum(8 * min(n, m - 1) ** 2 for n, m in sizes)
+    probs[0].n = 0
+    assert L.cgot_splr_workspace_size(len(sizes), probs) == 0
+    assert b"n and m" in L.cgot_last_error()
