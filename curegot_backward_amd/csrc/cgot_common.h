@@ -33,7 +33,7 @@ struct CgInst {
     int ntiles;            // nrb*nstrips
     int vec0;              // first vector workgroup (ceil((n+m)/CG_WG) of them)
     int nvec;
-    int row0;              // first row-wave workgroup (ceil(n/CG_WAVES) of them)
+    int row0;              // first wave-map workgroup (ceil((n+m)/CG_WAVES) of them, one wave per index)
     int nrow;
     double reg, inv_reg;
     // dual / gradient state, all FP64
@@ -79,6 +79,7 @@ struct CgInst {
     int beta_side;         // 1: Schur onto beta (P = m-1), 0: onto alpha (P = n)
     double* S;             // [P*P]
     double* Ldg;           // [nblk*CG_NB*CG_NB] Cholesky factors of the diagonal blocks
+    double* Winv;          // [nblk*CG_NB*CG_NB] their inverses
     double* rhs;           // [2*P] Schur right-hand sides / solutions
     double* ltmp;          // [2*max(n,m)] Schur solutions
     double* inv_diag;      // [N] 1/diag(A) of the context matrix

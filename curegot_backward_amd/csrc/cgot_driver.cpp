@@ -140,13 +140,19 @@ void set_geometry(CgInst& I, int n, int m, double reg)
     I.n = n;
     I.m = m;
     I.N = n + m - 1;
-    I.c = std::min(CG_CMAX, std::max(1, cdiv(m, 64)));
+    {
+        // columns per lane: the next power of two >= ceil(m/64), at most CG_CMAX
+        const int need = std::max(1, cdiv(m, 64));
+        int c = 1;
+        while (c < need && c < CG_CMAX) c *= 2;
+        I.c = c;
+    }
     I.sw = 64 * I.c;
     I.nstrips = cdiv(m, I.sw);
     I.nrb = cdiv(n, CG_RB);
     I.ntiles = I.nrb * I.nstrips;
     I.nvec = cdiv(n + m, CG_WG);
-    I.nrow = cdiv(n, CG_WAVES);
+    I.nrow = cdiv(n + m, CG_WAVES);
     I.reg = reg;
     I.inv_reg = 1.0 / reg;
 }
@@ -580,6 +586,7 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.ccnt = bp.take<int>((size_t)I.nrb * m);
         I.S = bp.take<double>((size_t)I.P * I.P);
         I.Ldg = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
+        I.Winv = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
         I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
         I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
         I.inv_diag = bp.take<double>(N);
@@ -659,7 +666,7 @@ struct SplrSolver {
             const double shift = std::min(hi[b].gnorm, hi[b].shift_max);
             r[(size_t)b * CG_NCTL + CTL_SHIFT] = shift * hi[b].reg;
         }
-        if (any) bt.vec0(k_refresh_values, bt.ctl(r));
+        if (any) bt.rows0(k_refresh_values, bt.ctl(r));
     }
 
     // Search direction for the instances in `on`.
@@ -669,20 +676,21 @@ struct SplrSolver {
         int nblk_max = 0;
         for (int b = 0; b < B; b++)
             if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
-        // dense prefix tables: fill | panel[kb] | update[kb] | trsv[kb]
-        std::vector<int> pref((size_t)(3 * nblk_max + 1) * (B + 1), 0);
+        // dense prefix tables: fill | panel[kb] | update[kb] | trsv[kb] | diag0
+        std::vector<int> pref((size_t)(3 * nblk_max + 2) * (B + 1), 0);
         auto tab = [&](int t) { return pref.data() + (size_t)t * (B + 1); };
         for (int b = 0; b < B; b++) {
             const int nb = on[b] ? hi[b].nblk : 0;
             tab(0)[b + 1] = nb * (nb + 1) / 2;
             for (int kb = 0; kb < nblk_max; kb++) {
                 const int T = nb - kb - 1;
-                tab(1 + kb)[b + 1] = (kb < nb) ? (nb - kb) : 0;
+                tab(1 + kb)[b + 1] = (kb < nb) ? (nb - kb - 1) : 0;
                 tab(1 + nblk_max + kb)[b + 1] = (T > 0) ? T * (T + 1) / 2 : 0;
                 tab(1 + 2 * nblk_max + kb)[b + 1] = (kb < nb) ? (kb + 1) : 0;
             }
+            tab(1 + 3 * nblk_max)[b + 1] = (nb > 0) ? 1 : 0;
         }
-        for (int t = 0; t < 3 * nblk_max + 1; t++)
+        for (int t = 0; t < 3 * nblk_max + 2; t++)
             for (int b = 0; b < B; b++) tab(t)[b + 1] += tab(t)[b];
         // The table is staged through a pinned buffer.  Every iteration reads the
         // scalars back (a stream sync) before the next search direction, so the
@@ -699,8 +707,11 @@ struct SplrSolver {
         auto total = [&](int t) { return tab(t)[B]; };
         if (nblk_max > 0 && total(0) > 0)
             hipLaunchKernelGGL(k_chol_fill, dim3(total(0)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(0));
-        bt.vec0(k_schur_build, dc);
-        bt.vec0(k_schur_rhs, dc);
+        bt.rows0(k_schur_build, dc);
+        bt.rows0(k_schur_rhs, dc);
+        if (nblk_max > 0 && total(1 + 3 * nblk_max) > 0)
+            hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 3 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
+                               dtab(1 + 3 * nblk_max));
         for (int kb = 0; kb < nblk_max; kb++) {
             const int tp = total(1 + kb);
             if (tp > 0) {
@@ -721,15 +732,12 @@ struct SplrSolver {
                                    dtab(1 + nblk_max + kb), kb);
             }
         }
-        for (int kb = nblk_max - 1; kb >= 0; kb--) {
-            const int tt = total(1 + 2 * nblk_max + kb);
-            if (tt > 0) {
-                ProfScope ps(bt.stream, PROF_TRSV, 0.0);
-                hipLaunchKernelGGL(k_trsv_back, dim3(tt), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                                   dtab(1 + 2 * nblk_max + kb), kb);
-            }
+        if (nblk_max > 0 && total(1 + 3 * nblk_max) > 0) {
+            ProfScope ps(bt.stream, PROF_TRSV, 0.0);
+            hipLaunchKernelGGL(k_trsv_all, dim3(total(1 + 3 * nblk_max)), dim3(1024), 0, bt.stream, bt.d_insts, B,
+                               dtab(1 + 3 * nblk_max));
         }
-        bt.vec0(k_schur_back, dc);
+        bt.rows0(k_schur_back, dc);
     }
 
     void woodbury(const double* dc)

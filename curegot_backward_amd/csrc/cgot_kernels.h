@@ -55,17 +55,19 @@ __global__ void k_sel_rows_write(const CgInst*, const int* row_inst, const doubl
 __global__ void k_sel_cols_count(const CgInst*, const int*, const double*);
 __global__ void k_sel_cols_scan(const CgInst*, const double*);                    // grid = B
 __global__ void k_sel_cols_write(const CgInst*, const int*, const double*);
-__global__ void k_refresh_values(const CgInst*, const int*, const double*);       // vec grid
+__global__ void k_refresh_values(const CgInst*, const int*, const double*);       // wave grid
 
 // ---- Newton solve on the Schur complement ----
-__global__ void k_schur_build(const CgInst*, const int*, const double*);          // vec grid over P
-__global__ void k_schur_rhs(const CgInst*, const int*, const double*);            // vec grid
-__global__ void k_schur_back(const CgInst*, const int*, const double*);           // vec grid
+__global__ void k_schur_build(const CgInst*, const int*, const double*);          // wave grid
+__global__ void k_schur_rhs(const CgInst*, const int*, const double*);            // wave grid
+__global__ void k_schur_back(const CgInst*, const int*, const double*);           // wave grid
 // compact ragged grids: pref[b]..pref[b+1] are instance b's workgroups
 __global__ void k_chol_fill(const CgInst*, int B, const int* pref);
+__global__ void k_chol_diag0(const CgInst*, int B, const int* pref);
 __global__ void k_chol_panel(const CgInst*, int B, const int* pref, int kb);
 __global__ void k_chol_update(const CgInst*, int B, const int* pref, int kb);
 __global__ void k_trsv_back(const CgInst*, int B, const int* pref, int kb);
+__global__ void k_trsv_all(const CgInst*, int B, const int* pref);
 
 // ---- backward PCG (vec grid) ----
 __global__ void k_pcg_init(const CgInst*, const int*, const double*);

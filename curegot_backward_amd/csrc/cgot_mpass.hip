@@ -39,54 +39,74 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
 // T pass: T_ij = exp((alpha_i + beta_j - M_ij)/reg); row and column partial sums.
 // Reference: T_fused_kernel / line_search_fused_kernel2,
 // src/sinkhorn_splr_kernel.cu:214-338 (float atomics there; fixed-order here).
+// Each wave takes its CG_RPW rows in groups of 4 and issues the loads of a
+// whole group before the exponentials, so 4*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
+template <int C>
+CG_DEV void cg_tpass_tile(const CgInst& I, int mode, double t, int rb, int st)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    double bj[C], cs[C];
+    bool ok[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int j = s0 + lane + 64 * k;
+        ok[k] = j < m;
+        bj[k] = ok[k] ? cg_point(I, mode, t, n + j) : 0.0;
+        cs[k] = 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    const double ir = I.inv_reg;
+#pragma unroll
+    for (int r0 = 0; r0 < CG_RPW; r0 += 4) {
+        double mv[4][C], ai[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
+            ai[r] = cg_point(I, mode, t, i);
+            const double* Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = i0 + r0 + r;
+            double rs = 0.0;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                const double v = ok[k] ? exp((ai[r] + bj[k] - mv[r][k]) * ir) : 0.0;
+                rs += v;
+                if (i < n) cs[k] += v;
+            }
+            rs = cg_wave_sum(rs);
+            if (lane == 0 && i < n) I.rp0[(size_t)i * I.nstrips + st] = rs;
+        }
+    }
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
+#pragma unroll
+    for (int k = 0; k < C; k++) lds[w][lane + 64 * k] = cs[k];
+    __syncthreads();
+    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
+        const int j = s0 + q;
+        if (j < m) I.cp0[(size_t)rb * m + j] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+    }
+}
+
 __global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ insts,
                                                  const int* __restrict__ tile_inst,
                                                  const double* __restrict__ ctl, int mode)
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const CgInst& I = insts[b];
     const double t = ctl[b * CG_NCTL + CTL_STEP];
-    const int s0 = st * I.sw;
-    const int m = I.m, n = I.n;
-    double bj[CG_CMAX], cs[CG_CMAX];
-#pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) {
-        const int j = s0 + lane + 64 * k;
-        bj[k] = (k < I.c && j < m) ? cg_point(I, mode, t, n + j) : 0.0;
-        cs[k] = 0.0;
-    }
-    const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= n) break;
-        const double ai = cg_point(I, mode, t, i);
-        const double* Mi = I.M + (size_t)i * m;
-        double rs = 0.0;
-#pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < m) {
-                const double v = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
-                cs[k] += v;
-                rs += v;
-            }
-        }
-        rs = cg_wave_sum(rs);
-        if (lane == 0) I.rp0[(size_t)i * I.nstrips + st] = rs;
-    }
-    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
-#pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cs[k];
-    __syncthreads();
-    for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
-        const int j = s0 + q;
-        if (j < m) {
-            const double v = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
-            I.cp0[(size_t)rb * m + j] = v;
-        }
+    switch (I.c) {
+        case 1: cg_tpass_tile<1>(I, mode, t, rb, st); break;
+        case 2: cg_tpass_tile<2>(I, mode, t, rb, st); break;
+        case 4: cg_tpass_tile<4>(I, mode, t, rb, st); break;
+        default: cg_tpass_tile<8>(I, mode, t, rb, st); break;
     }
 }
 

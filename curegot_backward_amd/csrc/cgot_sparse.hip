@@ -354,24 +354,26 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
 }
 
 // Refresh W values on the current pattern and the diagonal blocks
-// Da = T1 + shift, Db = (T^T 1)[:m-1] + shift.  Reference:
-// recompute_nonzero_values_kernel + write_diagonal_kernel, src/sinkhorn_splr_kernel.cu:554-734.
+// Da = T1 + shift, Db = (T^T 1)[:m-1] + shift; one wave per row i (q < n) or
+// column j = q - n.  Reference: recompute_nonzero_values_kernel +
+// write_diagonal_kernel, src/sinkhorn_splr_kernel.cu:554-734.
 __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restrict__ insts,
-                                                          const int* __restrict__ wg_inst,
+                                                          const int* __restrict__ wave_inst,
                                                           const double* __restrict__ ctl)
 {
-    const int b = wg_inst[blockIdx.x];
+    const int b = wave_inst[blockIdx.x];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInst& I = insts[b];
-    const int q = (blockIdx.x - I.vec0) * CG_WG + threadIdx.x;
+    const int q = (blockIdx.x - I.row0) * CG_WAVES + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
     if (q < I.n) {
-        for (int p = I.w_rowptr[q]; p < I.w_rowptr[q + 1]; p++) I.w_rval[p] = cg_T(I, q, I.w_colj[p]);
-        I.Da[q] = I.rowsum[q] + sh;
-    }
-    if (q < I.m - 1) {
-        for (int p = I.w_colptr[q]; p < I.w_colptr[q + 1]; p++) I.w_cval[p] = cg_T(I, I.w_rowi[p], q);
-        I.Db[q] = I.colsum[q] + sh;
+        for (int p = I.w_rowptr[q] + lane; p < I.w_rowptr[q + 1]; p += 64) I.w_rval[p] = cg_T(I, q, I.w_colj[p]);
+        if (lane == 0) I.Da[q] = I.rowsum[q] + sh;
+    } else if (q < I.N) {
+        const int j = q - I.n;
+        for (int p = I.w_colptr[j] + lane; p < I.w_colptr[j + 1]; p += 64) I.w_cval[p] = cg_T(I, I.w_rowi[p], j);
+        if (lane == 0) I.Db[j] = I.colsum[j] + sh;
     }
 }
 
