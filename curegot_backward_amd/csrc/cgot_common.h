@@ -70,6 +70,7 @@ struct CgInst {
     double* Db;            // [m-1]
     // top-K selection scratch
     unsigned int* hist;    // [CG_HBINS]
+    int* flags;            // [nblk] backward-substitution block flags (epoch numbers)
     int* rcnt;             // [n]   selected per row
     int* rtie;             // [n]   ties per row
     int* rcut;             // [n]   tie cut column per row
@@ -78,8 +79,7 @@ struct CgInst {
     int P;                 // min(n, m-1) Schur size
     int beta_side;         // 1: Schur onto beta (P = m-1), 0: onto alpha (P = n)
     double* S;             // [P*P]
-    double* Ldg;           // [nblk*CG_NB*CG_NB] Cholesky factors of the diagonal blocks
-    double* Winv;          // [nblk*CG_NB*CG_NB] their inverses
+    double* Winv;          // [nblk*CG_NB*CG_NB] inverses of the diagonal-block Cholesky factors
     double* rhs;           // [2*P] Schur right-hand sides / solutions
     double* ltmp;          // [2*max(n,m)] Schur solutions
     double* inv_diag;      // [N] 1/diag(A) of the context matrix

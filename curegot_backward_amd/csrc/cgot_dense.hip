@@ -7,17 +7,20 @@
 // S is symmetric, strictly diagonally dominant (margin >= shift*reg) with a
 // non-positive off-diagonal, so an unpivoted Cholesky is stable.
 //
-// Dense S is row-major P x P (lower triangle used); the factors of the
-// diagonal blocks live in Ldg, the panels below them in S.  The factorization
-// is a right-looking blocked Cholesky with CG_NB = 64.  Every diagonal block
-// is factored once, together with the inverse W of its factor, so the panel
-// solve and both substitutions become small dense products:
+// Dense S is row-major P x P (lower triangle used); the panels below the
+// diagonal blocks are stored in S, the inverses of the diagonal-block factors
+// in Winv.  The factorization is a right-looking blocked Cholesky with
+// CG_NB = 64.  Every diagonal block is factored once, together with the
+// inverse W of its factor, so the panel solve and both substitutions become
+// small dense products:
 //   k_chol_diag0:      factor/invert diagonal block 0, forward-substitute it;
-//   k_chol_panel(kb):  panel rows below block kb: X = A21 W_kb^T;
+//   k_chol_panel(0):   panel rows below block 0: X = A21 W_0^T;
 //   k_chol_update(kb): trailing update S22 -= L21 L21^T on lower tiles, the
-//                      forward-substitution update of the right-hand sides, and
-//                      the factor/inverse of diagonal block kb+1 (final here);
-//   k_trsv_back(kb):   backward substitution L^T x = y, one block per launch.
+//                      forward-substitution update of the right-hand sides,
+//                      the factor/inverse of diagonal block kb+1 (final here)
+//                      and, once that is published, the panel of block kb+1;
+//   k_trsv_flow:       backward substitution L^T x = y, one workgroup per block,
+//                      chained through per-block flags within one launch.
 // Grids are compact over a ragged batch: pref[b]..pref[b+1] are instance b's
 // workgroups for the launch (host computed, inactive instances get none).
 //
@@ -229,144 +232,157 @@ __global__ __launch_bounds__(CG_WG) void k_schur_back(const CgInst* __restrict__
 // ---------------------------------------------------------------------------
 // Blocked Cholesky
 // ---------------------------------------------------------------------------
+// Broadcast lane `src` (wave-uniform) of a double.
+CG_DEV double cg_readlane_d(double v, int src)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// sqrt(x) and 1/sqrt(x) for x > 0: the hardware reciprocal-square-root
+// estimate refined by two steps of the coupled Goldschmidt iteration
+// (g -> sqrt(x), h -> 1/(2 sqrt(x)); each step squares the relative error).
+CG_DEV void cg_sqrt_rsqrt(double x, double& sq, double& rs)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    sq = g;
+    rs = 2.0 * h;
+}
+
+// Inverse of the 16x16 lower diagonal block p of L into W, lanes 0..15 of the
+// calling wave (one column each, forward substitution with broadcast LDS reads).
+CG_DEV void cg_diag16_inverse(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], const double* dinv, int p)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane >= 16) return;
+    const int b0 = 16 * p, j = lane;
+    double w[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; k++) v = fma(-L[b0 + i][b0 + k], w[k], v);
+        w[i] = (i >= j) ? v * dinv[b0 + i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) W[b0 + i][b0 + j] = w[i];
+}
+
+// 16x16x16 product on one wave: acc += A B with A[i][k] = pa[i*lda + k] and
+// B[k][j] = pb[k*ldb + j] in LDS (v_mfma_f64_16x16x4f64, four k-steps).
+CG_DEV void cg_mfma16(const double* pa, int lda, const double* pb, int ldb, cg_d4& acc)
+{
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[i * lda + 4 * s + kq], pb[(4 * s + kq) * ldb + i], acc,
+                                                   0, 0, 0);
+}
+
 // Factor the 64 x 64 lower block held in L (LDS, row stride CG_NB+1) in place
-// and form W = L^{-1}.  Rows/columns >= nb hold the identity (so the padded
-// block factors trivially).  Blocked by 16:
-//   for each 16-column panel p: lanes 0..15 of wave 0 factor the 16x16 diagonal
-//   block left-looking, one row per lane in registers (the pivot row is shared
-//   through LDS within the wave, no barrier); the rows below solve against it;
-//   the trailing block takes the rank-16 update;
-//   W: the four 16x16 diagonal inverses, one column per lane of wave 0 in
-//   registers; then the off-diagonal blocks by distance d = 1, 2, 3:
-//   W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj.
+// and form the lower triangle of W = L^{-1} (the upper triangle of W is left
+// undefined).  Rows/columns >= nb hold the identity (so the padded block
+// factors trivially).  Blocked by 16:
+//   panel p (columns 16p..16p+15, rows 16p..63): wave 0 holds one row per lane
+//   in registers and runs the 16 right-looking column steps, broadcasting the
+//   pivot and the column entries with readlane (the rows below the diagonal
+//   block are solved in the same pass); meanwhile wave 1 inverts diagonal
+//   block p-1.  The trailing lower block then takes the rank-16 update as
+//   16x16 MFMA tiles spread over the four waves.
+//   W: after the four 16x16 diagonal inverses, the off-diagonal blocks by
+//   distance d = 1, 2, 3 on MFMA: W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj.
 CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int nb, double* dinv)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     __shared__ int bad;
     if (tid == 0) bad = 0;
-    for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) {
-        const int r = e / CG_NB, c = e % CG_NB;
-        if (r >= nb || c >= nb) L[r][c] = (r == c) ? 1.0 : 0.0;
+    if (nb < CG_NB) {
+        for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) {
+            const int r = e / CG_NB, c = e % CG_NB;
+            if (r >= nb || c >= nb) L[r][c] = (r == c) ? 1.0 : 0.0;
+        }
     }
     __syncthreads();
 #pragma unroll 1
     for (int p = 0; p < 4; p++) {
         const int b0 = 16 * p;
-        if (wv == 0 && lane < 16) {
-            const int r = lane;
+        const int R = CG_NB - b0;          // rows of this panel
+        if (wv == 0) {
+            const bool act = lane < R;
             double a[16];
 #pragma unroll
-            for (int k = 0; k < 16; k++) a[k] = L[b0 + r][b0 + k];
+            for (int k = 0; k < 16; k++) a[k] = act ? L[b0 + lane][b0 + k] : 0.0;
+            bool badp = false;
+            double myd = 0.0;
 #pragma unroll
             for (int c = 0; c < 16; c++) {
-                if (r == c) {
-                    double d = a[c];
+                const double piv = cg_readlane_d(a[c], c);
+                badp = badp || !(piv > 0.0);
+                double sq, rs;
+                cg_sqrt_rsqrt(piv, sq, rs);
+                a[c] = (lane == c) ? sq : a[c] * rs;
+                myd = (lane == c) ? rs : myd;
 #pragma unroll
-                    for (int k = 0; k < c; k++) d -= a[k] * a[k];
-                    if (!(d > 0.0)) bad = 1;
-                    const double sd = sqrt(d);
-                    const double id = 1.0 / sd;
-                    a[c] = sd;
-                    L[b0 + c][b0 + c] = sd;
-                    dinv[b0 + c] = id;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (r > c) {
-                    double v = a[c];
-#pragma unroll
-                    for (int k = 0; k < c; k++) v -= a[k] * L[b0 + c][b0 + k];
-                    v *= dinv[b0 + c];
-                    a[c] = v;
-                    L[b0 + r][b0 + c] = v;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int k = c + 1; k < 16; k++) a[k] = fma(-a[c], cg_readlane_d(a[c], k), a[k]);
             }
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    if (lane >= 16 || k <= lane) L[b0 + lane][b0 + k] = a[k];
+            }
+            if (lane < 16) dinv[b0 + lane] = myd;
+            if (badp && lane == 0) bad = 1;
+        } else if (wv == 1 && p > 0) {
+            cg_diag16_inverse(L, W, dinv, p - 1);
         }
         __syncthreads();
-        const int rows = CG_NB - b0 - 16;
-        if (rows == 0) break;
-        // rows below: X L_pp^T = A  (one thread per row r, 16 sequential columns)
-        if (tid < rows) {
-            const int r = b0 + 16 + tid;
-            double x[16];
+        const int T = (R - 16) / 16;       // trailing tiles per side
+        if (T == 0) break;
+        // lower tiles (ti >= tj) of the trailing block, wave-strided
+        for (int t = wv; t < T * (T + 1) / 2; t += CG_WAVES) {
+            int ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+            const int tj = t - ti * (ti + 1) / 2;
+            const int i0 = b0 + 16 + 16 * ti, j0 = b0 + 16 + 16 * tj;
+            cg_d4 acc = {0.0, 0.0, 0.0, 0.0};
+            // C -= L[i0.., b0..b0+15] L[j0.., b0..b0+15]^T: B[k][j] = L[j0 + j][b0 + k]
+            const int i = lane & 15, kq = lane >> 4;
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                double v = L[r][b0 + c];
+            for (int s = 0; s < 4; s++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L[i0 + i][b0 + 4 * s + kq], L[j0 + i][b0 + 4 * s + kq],
+                                                           acc, 0, 0, 0);
 #pragma unroll
-                for (int q = 0; q < c; q++) v -= x[q] * L[b0 + c][b0 + q];
-                x[c] = v * dinv[b0 + c];
-            }
-#pragma unroll
-            for (int c = 0; c < 16; c++) L[r][b0 + c] = x[c];
-        }
-        __syncthreads();
-        // trailing update of rows/cols >= b0+16 (lower part): wave wv takes rows
-        // r = b0+16+wv+4t, lane takes column q = b0+16+lane
-        {
-            const int q = b0 + 16 + lane;
-            double lq[16];
-            if (q < CG_NB) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) lq[k] = L[q][b0 + k];
-            }
-            for (int r = b0 + 16 + wv; r < CG_NB; r += CG_WAVES) {
-                if (q < CG_NB && q <= r) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 16; k++) v += L[r][b0 + k] * lq[k];
-                    L[r][q] -= v;
-                }
-            }
+            for (int r = 0; r < 4; r++) L[i0 + 4 * r + kq][j0 + i] -= acc[r];
         }
         __syncthreads();
     }
-    // ---- inverse: diagonal 16x16 blocks, lane = (block, column) of wave 0 ----
-    for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) W[e / CG_NB][e % CG_NB] = 0.0;
-    __syncthreads();
-    if (wv == 0) {
-        const int b0 = 16 * (lane >> 4), j = lane & 15;
-        double w[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            double v = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < i; k++) v -= L[b0 + i][b0 + k] * w[k];
-            w[i] = (i >= j) ? v * dinv[b0 + i] : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < 16; i++) W[b0 + i][b0 + j] = w[i];
-    }
+    if (wv == 0) cg_diag16_inverse(L, W, dinv, 3);
     __syncthreads();
     for (int d = 1; d < 4; d++) {
-        // blocks (i = j + d, j): first T_ij = sum_{k=j}^{i-1} L_ik W_kj, stored in
-        // the (i, j) block of W (not read at this level), then W_ij = -W_ii T_ij
-        // with all reads before the barrier and all writes after it.
-        const int nbk = 4 - d;
-        for (int e = tid; e < nbk * 256; e += CG_WG) {
-            const int jb = e >> 8, ib = jb + d, rr = (e >> 4) & 15, cc = e & 15;
-            double v = 0.0;
+        const int jb = wv;
+        if (jb < 4 - d) {
+            const int ib = jb + d;
+            cg_d4 t = {0.0, 0.0, 0.0, 0.0};
             for (int kb = jb; kb < ib; kb++)
+                cg_mfma16(&L[16 * ib][16 * kb], CG_NB + 1, &W[16 * kb][16 * jb], CG_NB + 1, t);
+            // W_ij = -W_ii T: the accumulator layout of T (row 4r + l/16, column
+            // l%16 in register r) is the B-operand layout of k-step r
+            const int i = lane & 15, kq = lane >> 4;
+            cg_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k = 0; k < 16; k++) v += L[16 * ib + rr][16 * kb + k] * W[16 * kb + k][16 * jb + cc];
-            W[16 * ib + rr][16 * jb + cc] = v;
-        }
-        __syncthreads();
-        double out[3];
-        for (int e = tid, u = 0; e < nbk * 256; e += CG_WG, u++) {
-            const int jb = e >> 8, ib = jb + d, rr = (e >> 4) & 15, cc = e & 15;
-            double v = 0.0;
+            for (int s = 0; s < 4; s++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(W[16 * ib + i][16 * ib + 4 * s + kq], t[s], acc, 0, 0, 0);
 #pragma unroll
-            for (int k = 0; k < 16; k++) v += W[16 * ib + rr][16 * ib + k] * W[16 * ib + k][16 * jb + cc];
-            out[u] = -v;
-        }
-        __syncthreads();
-        for (int e = tid, u = 0; e < nbk * 256; e += CG_WG, u++) {
-            const int jb = e >> 8, ib = jb + d, rr = (e >> 4) & 15, cc = e & 15;
-            W[16 * ib + rr][16 * jb + cc] = out[u];
+            for (int r = 0; r < 4; r++) W[16 * ib + 4 * r + kq][16 * jb + i] = -acc[r];
         }
         __syncthreads();
     }
@@ -378,20 +394,22 @@ CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int
 // y_kb = W_kb b_kb for the two right-hand sides (b already holds the updates).
 // L and W are LDS blocks.
 CG_DEV void cg_diag_block(const CgInst& I, int kb, double (*L)[CG_NB + 1], double (*W)[CG_NB + 1],
-                          double* dinv, double* ytmp)
+                          double* dinv, double* ytmp, int fepoch)
 {
     const int P = I.P;
     const int c0 = kb * CG_NB;
     const int nb = min(CG_NB, P - c0);
     const bool ok = cg_lds_potrf_inv(L, W, nb, dinv);
     if (!ok && threadIdx.x == 0) I.scal[CS_PIVOT] = 1.0;
-    double* Ld = I.Ldg + (size_t)kb * CG_NB * CG_NB;
     double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = e / CG_NB, c = e % CG_NB;
-        Ld[e] = (r < nb && c <= r) ? L[r][c] : 0.0;
-        Wd[e] = (r < nb && c < nb) ? W[r][c] : 0.0;
+        Wd[e] = (r < nb && c <= r) ? W[r][c] : 0.0;
     }
+    // publish W_kb to the panel workgroups of the same launch
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&I.flags[kb], fepoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < 2 * CG_NB) {
         const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
         ytmp[threadIdx.x] = (q < nb) ? I.rhs[(size_t)r * P + c0 + q] : 0.0;
@@ -401,15 +419,34 @@ CG_DEV void cg_diag_block(const CgInst& I, int kb, double (*L)[CG_NB + 1], doubl
         const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
         if (q < nb) {
             double v = 0.0;
-            for (int k = 0; k <= q; k++) v += W[q][k] * ytmp[r * CG_NB + k];
+            for (int k = 0; k <= q; k++) v = fma(W[q][k], ytmp[r * CG_NB + k], v);
             I.rhs[(size_t)r * P + c0 + q] = v;
         }
     }
 }
 
+// Bounded wait (thread 0) until flags[k] >= target, then a workgroup barrier
+// and an acquire fence so the data published before the flag is visible.  A
+// wait that exceeds the bound flags CS_PIVOT = 2 and returns.
+CG_DEV void cg_wait_flag(const CgInst& I, int k, int target)
+{
+    if (threadIdx.x == 0) {
+        int spins = 0;
+        while (__hip_atomic_load(&I.flags[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1 << 25)) {
+                I.scal[CS_PIVOT] = 2.0;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // Block 0: factor/invert the first diagonal block (one workgroup per instance).
 __global__ __launch_bounds__(CG_WG) void k_chol_diag0(const CgInst* __restrict__ insts, int B,
-                                                      const int* __restrict__ pref)
+                                                      const int* __restrict__ pref, int fepoch)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
     const CgInst& I = insts[b];
@@ -423,7 +460,7 @@ __global__ __launch_bounds__(CG_WG) void k_chol_diag0(const CgInst* __restrict__
         const int r = e / CG_NB, c = e % CG_NB;
         L[r][c] = (r < nb && c <= r) ? I.S[(size_t)r * P + c] : 0.0;
     }
-    cg_diag_block(I, 0, L, W, dinv, ytmp);
+    cg_diag_block(I, 0, L, W, dinv, ytmp, fepoch);
 }
 
 // Panel rows of block column kb (row tile rt + 1 below the diagonal):
@@ -461,11 +498,16 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
 }
 
 // Trailing update of tile (TI, TJ), kb < TJ <= TI: C -= A B^T with A, B the
-// panel rows of blocks TI and TJ.  Diagonal tiles also apply the forward-
-// substitution update rhs_TI -= A y_kb; the tile (kb+1, kb+1) then factors and
-// inverts itself (it is final after this update) and solves its rhs block.
+// panel rows of blocks TI and TJ (C is loaded before the product).  Diagonal
+// tiles also apply the forward-substitution update rhs_TI -= A y_kb.  The tile
+// (kb+1, kb+1), the first workgroup of the instance's range, is final after
+// this update: it factors and inverts itself, publishes W_{kb+1}
+// (flags[kb+1] = fepoch) and solves its rhs block.  The tiles (TI, kb+1) below
+// it are the next panel: after their update they wait for W_{kb+1} and store
+// X = C W_{kb+1}^T in place (they wait only on the first workgroup of their
+// range, which waits on nothing).
 __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict__ insts, int B,
-                                                       const int* __restrict__ pref, int kb)
+                                                       const int* __restrict__ pref, int kb, int fepoch)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
     const CgInst& I = insts[b];
@@ -480,15 +522,29 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
     __shared__ double Bs[CG_NB][CG_NB + 1];
     __shared__ double dinv[CG_NB];
     __shared__ double ytmp[2 * CG_NB];
+    // current C in the accumulator layout (rows beyond P read as 0; the
+    // diagonal tile keeps only its lower triangle)
+    double cold[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gr = ri + cg_mfma_row(a, r), gc = rj + cg_mfma_col(c);
+                cold[a][c][r] = (gr < P && gc < P && (TI != TJ || gc <= gr)) ? S[(size_t)gr * P + gc] : 0.0;
+            }
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = e / CG_NB, c = e % CG_NB;
         As[r][c] = (ri + r < P) ? S[(size_t)(ri + r) * P + c0 + c] : 0.0;
         Bs[r][c] = (rj + r < P) ? S[(size_t)(rj + r) * P + c0 + c] : 0.0;
     }
+    if (TI == TJ && threadIdx.x < 2 * CG_NB)   // y_kb (block kb is full width)
+        ytmp[threadIdx.x] = I.rhs[(size_t)(threadIdx.x / CG_NB) * P + c0 + threadIdx.x % CG_NB];
     __syncthreads();
     cg_d4 acc[2][2];
     cg_mfma_abt64(As, Bs, acc);
-    if (TI != TJ) {
+    if (TI != TJ && TJ != kb + 1) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -496,36 +552,38 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int gr = ri + cg_mfma_row(a, r), gc = rj + cg_mfma_col(c);
-                    if (gr < P && gc < P) S[(size_t)gr * P + gc] -= acc[a][c][r];
+                    if (gr < P && gc < P) S[(size_t)gr * P + gc] = cold[a][c][r] - acc[a][c][r];
                 }
         return;
     }
-    // diagonal tile: forward-substitution update of rhs block TI
-    if (threadIdx.x < 2 * CG_NB) {
-        const int rhsi = threadIdx.x / CG_NB, r = threadIdx.x % CG_NB;
-        if (ri + r < P) {
-            const double* y = I.rhs + (size_t)rhsi * P + c0;
-            double v = 0.0;
-            for (int k = 0; k < CG_NB; k++) v += As[r][k] * y[k];
-            I.rhs[(size_t)rhsi * P + ri + r] -= v;
+    if (TI == TJ) {
+        // forward-substitution update of rhs block TI
+        if (threadIdx.x < 2 * CG_NB) {
+            const int rhsi = threadIdx.x / CG_NB, r = threadIdx.x % CG_NB;
+            if (ri + r < P) {
+                double v = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < CG_NB; k++) v = fma(As[r][k], ytmp[rhsi * CG_NB + k], v);
+                I.rhs[(size_t)rhsi * P + ri + r] -= v;
+            }
+        }
+        if (TI != kb + 1) {
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int gr = ri + cg_mfma_row(a, r), gc = rj + cg_mfma_col(c);
+                        if (gr < P && gc <= gr) S[(size_t)gr * P + gc] = cold[a][c][r] - acc[a][c][r];
+                    }
+            return;
         }
     }
-    if (TI != kb + 1) {
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int c = 0; c < 2; c++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int gr = ri + cg_mfma_row(a, r), gc = rj + cg_mfma_col(c);
-                    if (gr < P && gc <= gr) S[(size_t)gr * P + gc] -= acc[a][c][r];
-                }
-        return;
-    }
-    // tile (kb+1, kb+1): the updated block goes to LDS (reusing As/Bs), then is
-    // factored, inverted and used for its forward substitution
+    // tile (kb+1, kb+1) or a tile of the next panel: the updated tile goes to
+    // LDS (As; padding rows/columns are 0)
     const int nb = min(CG_NB, P - ri);
-    __syncthreads();   // As no longer read (rhs update above)
+    __syncthreads();   // As/Bs no longer read (product, rhs update)
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -533,100 +591,118 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int lr = cg_mfma_row(a, r), lc = cg_mfma_col(c);
-                As[lr][lc] = (lr < nb && lc <= lr) ? (S[(size_t)(ri + lr) * P + rj + lc] - acc[a][c][r]) : 0.0;
+                As[lr][lc] = (lr < nb) ? cold[a][c][r] - acc[a][c][r] : 0.0;
             }
-    __threadfence_block();
-    cg_diag_block(I, kb + 1, As, Bs, dinv, ytmp);   // Bs is free now: it holds W
-}
-
-// Backward substitution step kb: x_kb = W_kb^T y_kb (every workgroup computes
-// it), workgroup 0 stores x_kb, workgroup t >= 1 updates block t-1:
-// y_{t-1} -= L[kb, t-1]^T x_kb.  y lives in rhs, x in ltmp.
-__global__ __launch_bounds__(CG_WG) void k_trsv_back(const CgInst* __restrict__ insts, int B,
-                                                     const int* __restrict__ pref, int kb)
-{
-    const int b = cg_find_inst(pref, B, blockIdx.x);
-    const CgInst& I = insts[b];
-    const int t = blockIdx.x - pref[b];
-    const int P = I.P;
-    const int c0 = kb * CG_NB;
-    const int nb = min(CG_NB, P - c0);
-    const double* S = I.S;
-    const double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
-    __shared__ double ys[2][CG_NB];
-    __shared__ double xs[2][CG_NB];
-    if (threadIdx.x < 2 * CG_NB) {
-        const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
-        ys[r][q] = (q < nb) ? I.rhs[(size_t)r * P + c0 + q] : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2 * CG_NB) {
-        // x[q] = sum_{c >= q} W[c][q] y[c]
-        const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
-        double v = 0.0;
-        for (int c = q; c < nb; c++) v += Wd[c * CG_NB + q] * ys[r][c];
-        xs[r][q] = v;
-    }
-    __syncthreads();
-    if (t == 0) {
-        if (threadIdx.x < 2 * CG_NB) {
-            const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
-            if (q < nb) I.ltmp[(size_t)r * P + c0 + q] = xs[r][q];
-        }
+    if (TI == TJ) {
+        __syncthreads();
+        cg_diag_block(I, kb + 1, As, Bs, dinv, ytmp, fepoch);   // Bs is free now: it holds W
         return;
     }
-    const int blk = t - 1;
-    if (threadIdx.x < 2 * CG_NB) {
-        const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
-        const int col = blk * CG_NB + q;
-        double v = 0.0;
-        for (int c = 0; c < nb; c++) v += S[(size_t)(c0 + c) * P + col] * xs[r][c];
-        I.rhs[(size_t)r * P + col] -= v;
-    }
+    // next panel: X = C W^T, W = W_{kb+1} once published
+    cg_wait_flag(I, kb + 1, fepoch);
+    const double* Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
+    for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) Bs[e / CG_NB][e % CG_NB] = Wd[e];
+    __syncthreads();
+    cg_mfma_abt64(As, Bs, acc);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gr = ri + cg_mfma_row(a, r);
+                if (gr < P) S[(size_t)gr * P + rj + cg_mfma_col(c)] = acc[a][c][r];
+            }
 }
 
-// Whole backward substitution L^T x = y in one workgroup of 1024 threads per
-// instance (the block steps are a dependent chain, so one launch replaces one
-// launch per block).  Step kb: x_kb = W_kb^T y_kb, then y_col -= sum_c
-// L[kb*64 + c][col] x_kb[c] for every col < kb*64 (one thread per column,
-// coalesced reads of block row kb).  y lives in rhs, x goes to ltmp.
-__global__ __launch_bounds__(1024) void k_trsv_all(const CgInst* __restrict__ insts, int B,
-                                                   const int* __restrict__ pref)
+// Backward substitution L^T x = y as a dataflow over the block columns: one
+// workgroup per (instance, block kb).  Workgroups are numbered level by level,
+// level t = nblk-1-kb, with the instances of each level in the order `order`
+// (sorted by decreasing block count, so level t holds the first lvl_pref[t+1]
+// - lvl_pref[t] of them): every instance's chain starts at once.  Workgroup kb
+// accumulates L_{j,kb}^T x_j for j = nblk-1 .. kb+1 as each x_j is published
+// (flags[j] == epoch; the tile is loaded before the wait), then forms
+// x_kb = W_kb^T (y_kb - sum) and publishes it.  A workgroup waits only on
+// lower levels, i.e. on workgroups with smaller launch indices, which were
+// dispatched before it and themselves wait only on earlier ones, so every wait
+// ends; the spin is also bounded and flags CS_PIVOT = 2 instead of hanging.
+// y lives in rhs, x goes to ltmp.
+__global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ insts, const int* __restrict__ order,
+                                                     const int* __restrict__ lvl_pref, int nlvl, int epoch)
 {
-    const int b = cg_find_inst(pref, B, blockIdx.x);
+    const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
+    const int b = order[blockIdx.x - lvl_pref[t]];
     const CgInst& I = insts[b];
     const int P = I.P;
     const int nblk = (P + CG_NB - 1) / CG_NB;
+    const int kb = nblk - 1 - t;
+    const int c0 = kb * CG_NB;
+    const int nb = min(CG_NB, P - c0);
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
     const double* S = I.S;
-    double* y0 = I.rhs;
-    double* y1 = I.rhs + P;
+    __shared__ double Ws[CG_NB][CG_NB + 1];
     __shared__ double xs[2][CG_NB];
-    const int tid = threadIdx.x;
-    for (int kb = nblk - 1; kb >= 0; kb--) {
-        const int c0 = kb * CG_NB;
-        const int nb = min(CG_NB, P - c0);
-        const double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
+    __shared__ double part[2][CG_WAVES][CG_NB];
+    const double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
+    for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) Ws[e / CG_NB][e % CG_NB] = Wd[e];
+    // y_kb (thread < 128: right-hand side tid/64, entry tid%64)
+    const double yv = (tid < 2 * CG_NB && (tid % CG_NB) < nb) ? I.rhs[(size_t)(tid / CG_NB) * P + c0 + tid % CG_NB]
+                                                               : 0.0;
+    double v0 = 0.0, v1 = 0.0;
+    for (int j = nblk - 1; j > kb; j--) {
+        // block kb is full width here (kb < nblk-1); tile rows g, g+4, ...
+        const int r0 = j * CG_NB;
+        const int nbj = min(CG_NB, P - r0);
+        double sv[CG_NB / CG_WAVES];
+#pragma unroll
+        for (int q = 0; q < CG_NB / CG_WAVES; q++) {
+            const int rr = g + CG_WAVES * q;
+            sv[q] = (rr < nbj) ? S[(size_t)(r0 + rr) * P + c0 + c] : 0.0;
+        }
+        cg_wait_flag(I, j, epoch);
         if (tid < 2 * CG_NB) {
             const int r = tid / CG_NB, q = tid % CG_NB;
-            const double* y = r ? y1 : y0;
-            double v = 0.0;
-            for (int c = q; c < nb; c++) v += Wd[c * CG_NB + q] * y[c0 + c];
-            xs[r][q] = v;
-            if (q < nb) I.ltmp[(size_t)r * P + c0 + q] = v;
+            xs[r][q] = (q < nbj) ? I.ltmp[(size_t)r * P + r0 + q] : 0.0;
         }
         __syncthreads();
-        for (int col = tid; col < c0; col += 1024) {
-            double v0 = 0.0, v1 = 0.0;
-            const double* Sc = S + (size_t)c0 * P + col;
-#pragma unroll 4
-            for (int c = 0; c < nb; c++) {
-                const double sv = Sc[(size_t)c * P];
-                v0 += sv * xs[0][c];
-                v1 += sv * xs[1][c];
-            }
-            y0[col] -= v0;
-            y1[col] -= v1;
+#pragma unroll
+        for (int q = 0; q < CG_NB / CG_WAVES; q++) {
+            const int rr = g + CG_WAVES * q;
+            v0 = fma(sv[q], xs[0][rr], v0);
+            v1 = fma(sv[q], xs[1][rr], v1);
         }
         __syncthreads();
     }
+    part[0][g][c] = v0;
+    part[1][g][c] = v1;
+    __syncthreads();
+    if (tid < 2 * CG_NB) {
+        const int r = tid / CG_NB, q = tid % CG_NB;
+        const double acc = (part[r][0][q] + part[r][1][q]) + (part[r][2][q] + part[r][3][q]);
+        xs[r][q] = (q < nb) ? yv - acc : 0.0;
+    }
+    __syncthreads();
+    {
+        // x[q] = sum_{cc >= q} W[cc][q] r[cc]: thread (q, quarter g) sums cc in
+        // [16g, 16g + 16), both right-hand sides; partials reduced through LDS
+        double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int cc = 16 * g + u;
+            const double w = (cc >= c) ? Ws[cc][c] : 0.0;
+            x0 = fma(w, xs[0][cc], x0);
+            x1 = fma(w, xs[1][cc], x1);
+        }
+        part[0][g][c] = x0;
+        part[1][g][c] = x1;
+    }
+    __syncthreads();
+    if (tid < 2 * CG_NB) {
+        const int r = tid / CG_NB, q = tid % CG_NB;
+        const double x = (part[r][0][q] + part[r][1][q]) + (part[r][2][q] + part[r][3][q]);
+        if (q < nb) I.ltmp[(size_t)r * P + c0 + q] = x;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&I.flags[kb], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -187,7 +187,7 @@ struct Batch {
         d_row = bp.take<int>(nrow);
         d_ctl = bp.take<double>((size_t)kCtlSlots * B * CG_NCTL);
         d_scal = bp.take<double>((size_t)B * CG_NSCAL);
-        pref_cap = (3 * nblk_max + 2) * (B + 1);
+        pref_cap = (3 * nblk_max + 3) * (B + 1) + nblk_max + 1;
         d_pref = bp.take<int>(pref_cap);
         d_ptrs = bp.take<const double*>(3 * (size_t)B);
     }
@@ -546,13 +546,16 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
     Bump bp(base);
     bt.layout_common(bp, nblk_max);
     // scratch that must start at zero, contiguous for one clear
-    bt.zero_base = bp.take<unsigned int>((size_t)B * (4 + CG_HBINS));
-    bt.zero_bytes = (size_t)B * (4 + CG_HBINS) * sizeof(unsigned int);
+    // per instance: 4 counters, the top-K histogram, the block flags
+    const size_t zstride = 4 + CG_HBINS + (size_t)std::max(nblk_max, 1);
+    bt.zero_base = bp.take<unsigned int>((size_t)B * zstride);
+    bt.zero_bytes = (size_t)B * zstride * sizeof(unsigned int);
     for (int b = 0; b < B; b++) {
         CgInst& I = bt.h[b];
         const int n = I.n, m = I.m, N = I.N;
-        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * (4 + CG_HBINS) : nullptr;
+        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * zstride : nullptr;
         I.hist = bt.zero_base ? I.counter + 4 : nullptr;
+        I.flags = bt.zero_base ? (int*)(I.hist + CG_HBINS) : nullptr;
         I.ab = bp.take<double>(n + m);
         I.logab = bp.take<double>(n + m);
         I.gamma = bp.take<double>(n + m);
@@ -585,7 +588,6 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.rcut = bp.take<int>(n);
         I.ccnt = bp.take<int>((size_t)I.nrb * m);
         I.S = bp.take<double>((size_t)I.P * I.P);
-        I.Ldg = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
         I.Winv = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
         I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
         I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
@@ -597,6 +599,8 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
 struct SplrSolver {
     Batch& bt;
     std::vector<HostInst> hi;
+
+    int epoch = 0;   // search-direction launches so far (block flags of this solve)
 
     SplrSolver(Batch& b) : bt(b) {}
 
@@ -676,8 +680,11 @@ struct SplrSolver {
         int nblk_max = 0;
         for (int b = 0; b < B; b++)
             if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
-        // dense prefix tables: fill | panel[kb] | update[kb] | trsv[kb] | diag0
-        std::vector<int> pref((size_t)(3 * nblk_max + 2) * (B + 1), 0);
+        // dense prefix tables: fill | panel[kb] | update[kb] | (unused)[kb] | diag0,
+        // then the backward-substitution order (instances by decreasing block
+        // count) and its level prefix (level t: instances with more than t blocks)
+        const size_t lvl_off = (size_t)(3 * nblk_max + 3) * (B + 1);
+        std::vector<int> pref(lvl_off + nblk_max + 1, 0);
         auto tab = [&](int t) { return pref.data() + (size_t)t * (B + 1); };
         for (int b = 0; b < B; b++) {
             const int nb = on[b] ? hi[b].nblk : 0;
@@ -692,6 +699,19 @@ struct SplrSolver {
         }
         for (int t = 0; t < 3 * nblk_max + 2; t++)
             for (int b = 0; b < B; b++) tab(t)[b + 1] += tab(t)[b];
+        int nact = 0;
+        {
+            int* ord = tab(2 + 3 * nblk_max);
+            for (int b = 0; b < B; b++)
+                if (on[b] && hi[b].nblk > 0) ord[nact++] = b;
+            std::stable_sort(ord, ord + nact, [&](int x, int y) { return hi[x].nblk > hi[y].nblk; });
+            int* lp = pref.data() + lvl_off;
+            for (int t = 0; t < nblk_max; t++) {
+                int cnt = 0;
+                while (cnt < nact && hi[ord[cnt]].nblk > t) cnt++;
+                lp[t + 1] = lp[t] + cnt;
+            }
+        }
         // The table is staged through a pinned buffer.  Every iteration reads the
         // scalars back (a stream sync) before the next search direction, so the
         // previous copy out of this buffer has completed when it is rewritten.
@@ -709,11 +729,17 @@ struct SplrSolver {
             hipLaunchKernelGGL(k_chol_fill, dim3(total(0)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(0));
         bt.rows0(k_schur_build, dc);
         bt.rows0(k_schur_rhs, dc);
+        // block flags: factor of diagonal block k published = 2*epoch-1, solution
+        // block k published = 2*epoch (flags start at 0 for the solve)
+        epoch++;
+        const int fepoch = 2 * epoch - 1;
         if (nblk_max > 0 && total(1 + 3 * nblk_max) > 0)
             hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 3 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                               dtab(1 + 3 * nblk_max));
+                               dtab(1 + 3 * nblk_max), fepoch);
         for (int kb = 0; kb < nblk_max; kb++) {
-            const int tp = total(1 + kb);
+            // panel of block 0 on its own; later panels are formed by the
+            // previous step's update launch
+            const int tp = (kb == 0) ? total(1 + kb) : 0;
             if (tp > 0) {
                 ProfScope ps(bt.stream, PROF_CHOL_PANEL, 0.0);
                 hipLaunchKernelGGL(k_chol_panel, dim3(tp), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(1 + kb), kb);
@@ -729,13 +755,14 @@ struct SplrSolver {
                 }
                 ProfScope ps(bt.stream, PROF_CHOL_UPDATE, fl);
                 hipLaunchKernelGGL(k_chol_update, dim3(tu), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                                   dtab(1 + nblk_max + kb), kb);
+                                   dtab(1 + nblk_max + kb), kb, fepoch);
             }
         }
-        if (nblk_max > 0 && total(1 + 3 * nblk_max) > 0) {
+        const int ntrsv = (nblk_max > 0) ? pref[lvl_off + nblk_max] : 0;
+        if (ntrsv > 0) {
             ProfScope ps(bt.stream, PROF_TRSV, 0.0);
-            hipLaunchKernelGGL(k_trsv_all, dim3(total(1 + 3 * nblk_max)), dim3(1024), 0, bt.stream, bt.d_insts, B,
-                               dtab(1 + 3 * nblk_max));
+            hipLaunchKernelGGL(k_trsv_flow, dim3(ntrsv), dim3(CG_WG), 0, bt.stream, bt.d_insts,
+                               dtab(2 + 3 * nblk_max), dtab0 + lvl_off, nblk_max, 2 * epoch);
         }
         bt.rows0(k_schur_back, dc);
     }
@@ -862,6 +889,9 @@ struct SplrSolver {
                 bt.read_scal();
                 for (int b = 0; b < B; b++) {
                     if (!(act[b] && hi[b].in_ls)) continue;
+                    if (first && bt.sc(b, CS_PIVOT) >= 2.0)
+                        throw std::runtime_error("internal: a block-flag wait of the backward substitution "
+                                                 "exceeded its bound");
                     if (first && cand[b]) { fbcd[b] = bt.sc(b, CS_FBCD); g2bcd[b] = bt.sc(b, CS_G2BCD); }
                     g2[b] = bt.sc(b, CS_G2);
                     hi[b].ls.feed(bt.sc(b, CS_F), bt.sc(b, CS_DG), bt.sc(b, CS_DG0));

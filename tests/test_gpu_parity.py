@@ -74,7 +74,9 @@ def test_forward_backward_vs_oracle(cg, n, m, eta, seed):
         err = float(np.abs(res["grad"].cpu().numpy() - rb["grad"]).max())
         assert err <= 1e-10 * max(np.abs(rb["grad"]).max(), 1e-4 * scale), (err, scale)
         if min(n, m) > 1:
-            assert res["iterations"] == rb["iterations"]
+            # PCG stops at a relative residual of 1e-12: rounding-level differences in
+            # the converged duals can move the stop by one iteration
+            assert abs(res["iterations"] - rb["iterations"]) <= 1
 
 
 def test_sparsify_hook_matches_dense(cg):
