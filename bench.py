@@ -8,7 +8,8 @@ batch (forward, plans written); inputs are resident in HBM before timing.
 
 Also measured: the same instances solved one at a time with the plain entry
 (per-solve latency p50/p99), the T-pass kernel's achieved HBM bandwidth (HIP
-events on the solve stream), and the CPU oracle on a bounded sample.
+events around each launch in one extra, single-group solve), and the CPU oracle
+on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 """
@@ -67,6 +68,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (instances are independent; a multi-GPU "
@@ -91,25 +93,33 @@ def main():
     def batched():
         return cg.sinkhorn_splr_bwd_batch(Ms, As, Bs, etas, 1e-8, 1000)
 
+    _lib.set_max_groups(args.groups)
     for w in range(args.warmup):
         t = time.perf_counter()
         outs = batched()
         torch.cuda.synchronize()
         log(f"warmup {w}: {time.perf_counter() - t:.3f} s")
 
-    _lib.profile_reset()
-    _lib.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         outs = batched()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    _lib.profile_enable(False)
     niters = [o["niter"] for o in outs]
     conv = sum(1 for o in outs if o["converged"])
     log(f"timed: {args.steps} steps in {el:.3f} s; converged {conv}/{len(outs)}; "
         f"niter mean {np.mean(niters):.1f} max {max(niters)}")
+
+    # kernel figures: one more solve with HIP events around every launch, as a
+    # single lockstep group so that an event interval holds only its own kernel
+    _lib.set_max_groups(1)
+    _lib.profile_reset()
+    _lib.profile_enable(True)
+    batched()
+    torch.cuda.synchronize()
+    _lib.profile_enable(False)
+    _lib.set_max_groups(args.groups)
     prof = {k: _lib.profile_read(k) for k in _lib.PROFILE_CLASSES}
     for k, (ms, work, n) in prof.items():
         if n:
@@ -135,7 +145,7 @@ def main():
         "config": {
             "workload": f"ragged SPLR batch B={len(Ms)}, n,m log-uniform in [128,4096], d in {{2,8,32}}, "
                         "M=|x-y|^2/max, Dirichlet(1) marginals floored 1e-4, eta in {0.01,0.05}, tol 1e-8",
-            "B": len(Ms), "tol": 1e-8, "seed": args.seed, "cells": cells,
+            "B": len(Ms), "tol": 1e-8, "seed": args.seed, "cells": cells, "groups": args.groups,
             "converged": conv, "niter_mean": float(np.mean(niters)), "niter_max": int(max(niters)),
         },
         "roofline": {
@@ -163,7 +173,8 @@ def main():
         seq_el = time.perf_counter() - t1
         seq = len(Ms) / seq_el
         result["sequential"] = {"solves_per_s": seq, "p50_ms": float(np.percentile(lat, 50)),
-                                "p99_ms": float(np.percentile(lat, 99)), "total_s": seq_el}
+                                "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(max(lat)),
+                                "total_s": seq_el}
         result["speedup_vs_sequential"] = value / seq
         log(f"sequential: {seq:.2f} solves/s, p50 {np.percentile(lat, 50):.1f} ms, "
             f"p99 {np.percentile(lat, 99):.1f} ms")

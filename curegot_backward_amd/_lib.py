@@ -64,7 +64,7 @@ class BcdProblem(ctypes.Structure):
 
 
 EXPORTS = (
-    "cgot_splr_workspace_size", "cgot_splr_solve",
+    "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
     "cgot_bcd_workspace_size", "cgot_bcd_solve",
     "cgot_lse_workspace_size", "cgot_lse_update",
@@ -84,13 +84,15 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"curegot_backward_amd: {LIB_PATH} not found; build it with "
-            "`python -m curegot_backward_amd.build` (hipcc, gfx950)")
+            "`python curegot_backward_amd/build.py` (hipcc, gfx950)")
     L = ctypes.CDLL(LIB_PATH)
     sz, i32, i64, dbl, vp = ctypes.c_size_t, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
     L.cgot_splr_workspace_size.argtypes = [i32, ctypes.POINTER(SplrProblem)]
     L.cgot_splr_workspace_size.restype = sz
     L.cgot_splr_solve.argtypes = [i32, ctypes.POINTER(SplrProblem), vp, sz, vp, ctypes.POINTER(SplrInfo)]
     L.cgot_splr_solve.restype = ctypes.c_int
+    L.cgot_set_max_groups.argtypes = [i32]
+    L.cgot_set_max_groups.restype = ctypes.c_int
     L.cgot_adjoint_workspace_size.argtypes = [i32, ctypes.POINTER(AdjointProblem)]
     L.cgot_adjoint_workspace_size.restype = sz
     L.cgot_adjoint_solve.argtypes = [i32, ctypes.POINTER(AdjointProblem), vp, sz, vp,
@@ -131,6 +133,12 @@ def check(rc: int, what: str):
 
 PROFILE_CLASSES = {"tpass": 0, "row_lse": 1, "col_lse": 2, "qv": 3, "chol_update": 4,
                    "chol_panel": 5, "trsv": 6}
+
+
+def set_max_groups(groups: int):
+    """Bound the number of concurrent size classes a batched SPLR solve is split
+    into (1..4; 1 = one lockstep group on the caller's stream)."""
+    check(load().cgot_set_max_groups(int(groups)), "set_max_groups")
 
 
 def profile_enable(on: bool):

@@ -20,73 +20,91 @@
 #define CG_NB 64           // dense Cholesky block size
 
 #define CG_DEV __device__ __forceinline__
+#define CG_AS1 __attribute__((address_space(1)))   // global memory
 
-// Per-instance descriptor (device resident, written once per solve).
+// Per-instance descriptor (device resident, written once per solve).  The
+// field list is shared by the host view (plain pointers) and the device view
+// CgInstD, whose buffers are global-memory pointers so that loads and stores
+// through them compile to global rather than flat instructions.  Both views
+// have the same layout.
+#define CG_INST_FIELDS(Q) \
+                                                                                                 \
+    const Q double* M; /* [n*m] */                                                               \
+    int n, m, N; /* N = n+m-1 */                                                                 \
+    int c; /* columns per lane (1..CG_CMAX) */                                                   \
+    int sw; /* strip width 64*c */                                                               \
+    int nstrips; /* ceil(m/sw) */                                                                \
+    int nrb; /* ceil(n/CG_RB) */                                                                 \
+    int tile0; /* first global tile of this instance */                                          \
+    int ntiles; /* nrb*nstrips */                                                                \
+    int vec0; /* first vector workgroup (ceil((n+m)/CG_WG) of them) */                           \
+    int nvec;                                                                                    \
+    int row0; /* first wave-map workgroup (ceil((n+m)/CG_WAVES) of them, one wave per index) */  \
+    int nrow;                                                                                    \
+    double reg, inv_reg;                                                                         \
+    /* dual / gradient state, all FP64 */                                                        \
+    Q double* ab; /* [n+m]   (a, b) */                                                           \
+    Q double* logab; /* [n+m] */                                                                 \
+    Q double* gamma; /* [n+m] */                                                                 \
+    Q double* gamma_prev; /* [n+m] */                                                            \
+    Q double* gamma_bcd; /* [n+m] */                                                             \
+    Q double* grad; /* [N] */                                                                    \
+    Q double* grad_prev; /* [N] */                                                               \
+    Q double* grad_bcd; /* [N] */                                                                \
+    Q double* dir; /* [N]  Newton / quasi-Newton direction */                                    \
+    Q double* yv; /* [N]  secant y */                                                            \
+    Q double* sv; /* [N]  secant s */                                                            \
+    Q double* uv; /* [N]  A^{-1} y */                                                            \
+    Q double* rowsum; /* [n] */                                                                  \
+    Q double* colsum; /* [m] */                                                                  \
+    Q double* rp0; /* [n*nstrips] row partials (sum, or LSE max) */                              \
+    Q double* rp1; /* [n*nstrips] row partials (LSE sum) */                                      \
+    Q double* cp0; /* [nrb*m]     column partials */                                             \
+    Q double* cp1; /* [nrb*m] */                                                                 \
+    Q double* wgpart; /* [4*max(nfin,nvec)] per-workgroup scalar partials */                     \
+    Q unsigned int* counter; /* [4] last-workgroup counters */                                   \
+    Q double* scal; /* [CG_NSCAL] per-instance scalars */                                        \
+    /* sparsified Hessian (pattern + values), W_ij = T_ij on the pattern */                      \
+    int64_t Kmax;                                                                                \
+    Q int* w_rowptr; /* [n+1]   CSR of W by row i: column j */                                   \
+    Q int* w_colj; /* [Kmax] */                                                                  \
+    Q double* w_rval; /* [Kmax] */                                                               \
+    Q int* w_colptr; /* [m]     CSC of W by column j (m-1 used): row i */                        \
+    Q int* w_rowi; /* [Kmax] */                                                                  \
+    Q double* w_cval; /* [Kmax] */                                                               \
+    Q double* Da; /* [n]  diag(T1) + shift*reg */                                                \
+    Q double* Db; /* [m-1] */                                                                    \
+    /* top-K selection scratch */                                                                \
+    Q unsigned int* hist; /* [CG_HBINS] */                                                       \
+    Q int* flags; /* [nblk] backward-substitution block flags (epoch numbers) */                 \
+    Q int* rcnt; /* [n]   selected per row */                                                    \
+    Q int* rtie; /* [n]   ties per row */                                                        \
+    Q int* rcut; /* [n]   tie cut column per row */                                              \
+    Q int* ccnt; /* [nrb*m] selected per (row block, column) */                                  \
+    /* dense Schur complement (row-major P x P, lower triangle used) */                          \
+    int P; /* min(n, m-1) Schur size */                                                          \
+    int beta_side; /* 1: Schur onto beta (P = m-1), 0: onto alpha (P = n) */                     \
+    Q double* S; /* [P*P] */                                                                     \
+    Q double* Winv; /* [nblk*CG_NB*CG_NB] inverses of the diagonal-block Cholesky factors */     \
+    Q double* rhs; /* [2*P] Schur right-hand sides / solutions */                                \
+    Q double* ltmp; /* [2*max(n,m)] Schur solutions */                                           \
+    Q double* inv_diag; /* [N] 1/diag(A) of the context matrix */                                \
+    /* outputs */                                                                                \
+    Q double* plan; /* [n*m] or null */                                                          \
+    Q double* dual; /* [n+m] or null */                                                         
 struct CgInst {
-    const double* M;       // [n*m]
-    int n, m, N;           // N = n+m-1
-    int c;                 // columns per lane (1..CG_CMAX)
-    int sw;                // strip width 64*c
-    int nstrips;           // ceil(m/sw)
-    int nrb;               // ceil(n/CG_RB)
-    int tile0;             // first global tile of this instance
-    int ntiles;            // nrb*nstrips
-    int vec0;              // first vector workgroup (ceil((n+m)/CG_WG) of them)
-    int nvec;
-    int row0;              // first wave-map workgroup (ceil((n+m)/CG_WAVES) of them, one wave per index)
-    int nrow;
-    double reg, inv_reg;
-    // dual / gradient state, all FP64
-    double* ab;            // [n+m]   (a, b)
-    double* logab;         // [n+m]
-    double* gamma;         // [n+m]
-    double* gamma_prev;    // [n+m]
-    double* gamma_bcd;     // [n+m]
-    double* grad;          // [N]
-    double* grad_prev;     // [N]
-    double* grad_bcd;      // [N]
-    double* dir;           // [N]  Newton / quasi-Newton direction
-    double* yv;            // [N]  secant y
-    double* sv;            // [N]  secant s
-    double* uv;            // [N]  A^{-1} y
-    double* rowsum;        // [n]
-    double* colsum;        // [m]
-    double* rp0;           // [n*nstrips] row partials (sum, or LSE max)
-    double* rp1;           // [n*nstrips] row partials (LSE sum)
-    double* cp0;           // [nrb*m]     column partials
-    double* cp1;           // [nrb*m]
-    double* wgpart;        // [4*max(nfin,nvec)] per-workgroup scalar partials
-    unsigned int* counter; // [4] last-workgroup counters
-    double* scal;          // [CG_NSCAL] per-instance scalars
-    // sparsified Hessian (pattern + values), W_ij = T_ij on the pattern
-    int64_t Kmax;
-    int* w_rowptr;         // [n+1]   CSR of W by row i: column j
-    int* w_colj;           // [Kmax]
-    double* w_rval;        // [Kmax]
-    int* w_colptr;         // [m]     CSC of W by column j (m-1 used): row i
-    int* w_rowi;           // [Kmax]
-    double* w_cval;        // [Kmax]
-    double* Da;            // [n]  diag(T1) + shift*reg
-    double* Db;            // [m-1]
-    // top-K selection scratch
-    unsigned int* hist;    // [CG_HBINS]
-    int* flags;            // [nblk] backward-substitution block flags (epoch numbers)
-    int* rcnt;             // [n]   selected per row
-    int* rtie;             // [n]   ties per row
-    int* rcut;             // [n]   tie cut column per row
-    int* ccnt;             // [nrb*m] selected per (row block, column)
-    // dense Schur complement (row-major P x P, lower triangle used)
-    int P;                 // min(n, m-1) Schur size
-    int beta_side;         // 1: Schur onto beta (P = m-1), 0: onto alpha (P = n)
-    double* S;             // [P*P]
-    double* Winv;          // [nblk*CG_NB*CG_NB] inverses of the diagonal-block Cholesky factors
-    double* rhs;           // [2*P] Schur right-hand sides / solutions
-    double* ltmp;          // [2*max(n,m)] Schur solutions
-    double* inv_diag;      // [N] 1/diag(A) of the context matrix
-    // outputs
-    double* plan;          // [n*m] or null
-    double* dual;          // [n+m] or null
+    CG_INST_FIELDS()
 };
+struct CgInstD {
+    CG_INST_FIELDS(CG_AS1)
+};
+static_assert(sizeof(CgInst) == sizeof(CgInstD), "descriptor views differ");
+
+CG_DEV const CgInstD& cg_inst(const CgInst* insts, int b)
+{
+    return reinterpret_cast<const CgInstD*>(insts)[b];
+}
+
 
 // Scalar slots in CgInst::scal
 enum {

@@ -51,6 +51,46 @@ CG_DEV void cg_tri_index(int t, int& I, int& J)
 
 typedef double cg_d4 __attribute__((ext_vector_type(4)));
 
+// Load a 64 x 64 tile src[r*ld + c] into LDS with all of a thread's loads in
+// flight before the first LDS write; entries with r >= nrows or c >= ncols (or
+// c > r when `lower`) read as 0.
+CG_DEV void cg_tile_to_lds(double (*T)[CG_NB + 1], const CG_AS1 double* src, size_t ld, int nrows, int ncols,
+                           bool lower)
+{
+    constexpr int NQ = CG_NB * CG_NB / CG_WG;
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int e = threadIdx.x + CG_WG * q, r = e / CG_NB, c = e % CG_NB;
+        v[q] = (r < nrows && c < ncols && (!lower || c <= r)) ? src[(size_t)r * ld + c] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int e = threadIdx.x + CG_WG * q;
+        T[e / CG_NB][e % CG_NB] = v[q];
+    }
+}
+
+// Two full-width tiles at once (rows >= nrows_a / nrows_b read as 0).
+CG_DEV void cg_tiles2_to_lds(double (*Ta)[CG_NB + 1], const CG_AS1 double* a, size_t lda, int nrows_a,
+                             double (*Tb)[CG_NB + 1], const CG_AS1 double* b, size_t ldb, int nrows_b)
+{
+    constexpr int NQ = CG_NB * CG_NB / CG_WG;
+    double va[NQ], vb[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int e = threadIdx.x + CG_WG * q, r = e / CG_NB, c = e % CG_NB;
+        va[q] = (r < nrows_a) ? a[(size_t)r * lda + c] : 0.0;
+        vb[q] = (r < nrows_b) ? b[(size_t)r * ldb + c] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int e = threadIdx.x + CG_WG * q;
+        Ta[e / CG_NB][e % CG_NB] = va[q];
+        Tb[e / CG_NB][e % CG_NB] = vb[q];
+    }
+}
+
 // 64x64 tile product C = A B^T from LDS (A, B: 64 x 64, row stride CG_NB+1) on
 // FP64 matrix cores: wave w owns rows 32*(w/2).. and columns 32*(w%2).., as
 // 2 x 2 blocks of v_mfma_f64_16x16x4f64.  Operand layout of that instruction:
@@ -86,7 +126,7 @@ __global__ __launch_bounds__(CG_WG) void k_chol_fill(const CgInst* __restrict__ 
                                                      const int* __restrict__ pref)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     int TI, TJ;
     cg_tri_index(blockIdx.x - pref[b], TI, TJ);
     const int P = I.P;
@@ -116,13 +156,13 @@ __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict_
 {
     int b, s2;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, s2)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int P = I.P;
     if (s2 >= P) return;
     const int lane = threadIdx.x & 63;
-    const int* sp; const int* sidx; const double* sval;
-    const int* lp; const int* lidx; const double* lval;
-    const double* Ds; const double* Dl;
+    const CG_AS1 int* sp; const CG_AS1 int* sidx; const CG_AS1 double* sval;
+    const CG_AS1 int* lp; const CG_AS1 int* lidx; const CG_AS1 double* lval;
+    const CG_AS1 double* Ds; const CG_AS1 double* Dl;
     if (I.beta_side) {
         sp = I.w_colptr; sidx = I.w_rowi; sval = I.w_cval;
         lp = I.w_rowptr; lidx = I.w_colj; lval = I.w_rval;
@@ -132,7 +172,7 @@ __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict_
         lp = I.w_colptr; lidx = I.w_rowi; lval = I.w_cval;
         Ds = I.Da; Dl = I.Db;
     }
-    double* Srow = I.S + (size_t)s2 * P;
+    auto Srow = I.S + (size_t)s2 * P;
     if (lane == 0) Srow[s2] = Ds[s2];
     for (int p = sp[s2]; p < sp[s2 + 1]; p++) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -153,7 +193,7 @@ __global__ __launch_bounds__(CG_WG) void k_schur_rhs(const CgInst* __restrict__ 
 {
     int b, s;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, s)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int P = I.P, n = I.n;
     if (s >= P) return;
     const int lane = threadIdx.x & 63;
@@ -188,12 +228,12 @@ __global__ __launch_bounds__(CG_WG) void k_schur_back(const CgInst* __restrict__
 {
     int b, q;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, q)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int P = I.P, n = I.n;
     if (q >= I.N) return;
     const int lane = threadIdx.x & 63;
-    const double* xs0 = I.ltmp;
-    const double* xs1 = I.ltmp + P;
+    const auto xs0 = I.ltmp;
+    const auto xs1 = I.ltmp + P;
     const bool large = I.beta_side ? (q < n) : (q >= n);
     if (!large) {
         if (lane == 0) {
@@ -393,7 +433,7 @@ CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int
 // factor and its inverse, and run the forward substitution of that block:
 // y_kb = W_kb b_kb for the two right-hand sides (b already holds the updates).
 // L and W are LDS blocks.
-CG_DEV void cg_diag_block(const CgInst& I, int kb, double (*L)[CG_NB + 1], double (*W)[CG_NB + 1],
+CG_DEV void cg_diag_block(const CgInstD& I, int kb, double (*L)[CG_NB + 1], double (*W)[CG_NB + 1],
                           double* dinv, double* ytmp, int fepoch)
 {
     const int P = I.P;
@@ -401,7 +441,7 @@ CG_DEV void cg_diag_block(const CgInst& I, int kb, double (*L)[CG_NB + 1], doubl
     const int nb = min(CG_NB, P - c0);
     const bool ok = cg_lds_potrf_inv(L, W, nb, dinv);
     if (!ok && threadIdx.x == 0) I.scal[CS_PIVOT] = 1.0;
-    double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
+    auto Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = e / CG_NB, c = e % CG_NB;
         Wd[e] = (r < nb && c <= r) ? W[r][c] : 0.0;
@@ -428,7 +468,7 @@ CG_DEV void cg_diag_block(const CgInst& I, int kb, double (*L)[CG_NB + 1], doubl
 // Bounded wait (thread 0) until flags[k] >= target, then a workgroup barrier
 // and an acquire fence so the data published before the flag is visible.  A
 // wait that exceeds the bound flags CS_PIVOT = 2 and returns.
-CG_DEV void cg_wait_flag(const CgInst& I, int k, int target)
+CG_DEV void cg_wait_flag(const CgInstD& I, int k, int target)
 {
     if (threadIdx.x == 0) {
         int spins = 0;
@@ -449,17 +489,15 @@ __global__ __launch_bounds__(CG_WG) void k_chol_diag0(const CgInst* __restrict__
                                                       const int* __restrict__ pref, int fepoch)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int P = I.P;
     const int nb = min(CG_NB, P);
     __shared__ double L[CG_NB][CG_NB + 1];
     __shared__ double W[CG_NB][CG_NB + 1];
     __shared__ double dinv[CG_NB];
     __shared__ double ytmp[2 * CG_NB];
-    for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
-        const int r = e / CG_NB, c = e % CG_NB;
-        L[r][c] = (r < nb && c <= r) ? I.S[(size_t)r * P + c] : 0.0;
-    }
+    cg_tile_to_lds(L, I.S, P, nb, nb, true);
+    __syncthreads();
     cg_diag_block(I, 0, L, W, dinv, ytmp, fepoch);
 }
 
@@ -469,20 +507,17 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
                                                       const int* __restrict__ pref, int kb)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int rt = blockIdx.x - pref[b] + 1;
     const int P = I.P;
     const int c0 = kb * CG_NB;
     const int r0 = c0 + rt * CG_NB;
-    double* S = I.S;
-    const double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
+    auto S = I.S;
+    const auto Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
     __shared__ double As[CG_NB][CG_NB + 1];
     __shared__ double Ws[CG_NB][CG_NB + 1];
-    for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
-        const int r = e / CG_NB, c = e % CG_NB;
-        As[r][c] = (r0 + r < P) ? S[(size_t)(r0 + r) * P + c0 + c] : 0.0;   // block kb is full
-        Ws[r][c] = Wd[e];
-    }
+    // block kb is full width
+    cg_tiles2_to_lds(As, S + (size_t)r0 * P + c0, P, P - r0, Ws, Wd, CG_NB, CG_NB);
     __syncthreads();
     cg_d4 acc[2][2];
     cg_mfma_abt64(As, Ws, acc);
@@ -510,14 +545,14 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
                                                        const int* __restrict__ pref, int kb, int fepoch)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     int ti, tj;
     cg_tri_index(blockIdx.x - pref[b], ti, tj);
     const int TI = kb + 1 + ti, TJ = kb + 1 + tj;
     const int P = I.P;
     const int c0 = kb * CG_NB;
     const int ri = TI * CG_NB, rj = TJ * CG_NB;
-    double* S = I.S;
+    auto S = I.S;
     __shared__ double As[CG_NB][CG_NB + 1];
     __shared__ double Bs[CG_NB][CG_NB + 1];
     __shared__ double dinv[CG_NB];
@@ -534,11 +569,7 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
                 const int gr = ri + cg_mfma_row(a, r), gc = rj + cg_mfma_col(c);
                 cold[a][c][r] = (gr < P && gc < P && (TI != TJ || gc <= gr)) ? S[(size_t)gr * P + gc] : 0.0;
             }
-    for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
-        const int r = e / CG_NB, c = e % CG_NB;
-        As[r][c] = (ri + r < P) ? S[(size_t)(ri + r) * P + c0 + c] : 0.0;
-        Bs[r][c] = (rj + r < P) ? S[(size_t)(rj + r) * P + c0 + c] : 0.0;
-    }
+    cg_tiles2_to_lds(As, S + (size_t)ri * P + c0, P, P - ri, Bs, S + (size_t)rj * P + c0, P, P - rj);
     if (TI == TJ && threadIdx.x < 2 * CG_NB)   // y_kb (block kb is full width)
         ytmp[threadIdx.x] = I.rhs[(size_t)(threadIdx.x / CG_NB) * P + c0 + threadIdx.x % CG_NB];
     __syncthreads();
@@ -600,8 +631,8 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
     }
     // next panel: X = C W^T, W = W_{kb+1} once published
     cg_wait_flag(I, kb + 1, fepoch);
-    const double* Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
-    for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) Bs[e / CG_NB][e % CG_NB] = Wd[e];
+    const auto Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
+    cg_tile_to_lds(Bs, Wd, CG_NB, CG_NB, CG_NB, false);
     __syncthreads();
     cg_mfma_abt64(As, Bs, acc);
 #pragma unroll
@@ -632,19 +663,19 @@ __global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ 
 {
     const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
     const int b = order[blockIdx.x - lvl_pref[t]];
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int P = I.P;
     const int nblk = (P + CG_NB - 1) / CG_NB;
     const int kb = nblk - 1 - t;
     const int c0 = kb * CG_NB;
     const int nb = min(CG_NB, P - c0);
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    const double* S = I.S;
+    const auto S = I.S;
     __shared__ double Ws[CG_NB][CG_NB + 1];
     __shared__ double xs[2][CG_NB];
     __shared__ double part[2][CG_WAVES][CG_NB];
-    const double* Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
-    for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) Ws[e / CG_NB][e % CG_NB] = Wd[e];
+    const auto Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
+    cg_tile_to_lds(Ws, Wd, CG_NB, CG_NB, CG_NB, false);
     // y_kb (thread < 128: right-hand side tid/64, entry tid%64)
     const double yv = (tid < 2 * CG_NB && (tid % CG_NB) < nb) ? I.rhs[(size_t)(tid / CG_NB) * P + c0 + tid % CG_NB]
                                                                : 0.0;

@@ -16,6 +16,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -87,8 +92,9 @@ enum { PROF_TPASS = 0, PROF_ROWLSE, PROF_COLLSE, PROF_QV, PROF_CHOL_UPDATE, PROF
        PROF_TRSV, PROF_NCLS = 8 };
 struct ProfRec { hipEvent_t a, b; int cls; double work; };
 bool g_prof = false;
-std::vector<ProfRec> g_prof_pending;
-std::vector<hipEvent_t> g_prof_pool;
+thread_local std::vector<ProfRec> g_prof_pending;   // recorded by this thread, not yet read
+thread_local std::vector<hipEvent_t> g_prof_pool;
+std::mutex g_prof_mu;                               // guards the totals below
 double g_prof_ms[PROF_NCLS] = {0};
 double g_prof_work[PROF_NCLS] = {0};
 long long g_prof_count[PROF_NCLS] = {0};
@@ -123,6 +129,7 @@ void prof_collect()
         float ms = 0.f;
         CG_HIP(hipEventSynchronize(r.b));
         CG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        std::lock_guard<std::mutex> lk(g_prof_mu);
         g_prof_ms[r.cls] += ms;
         g_prof_work[r.cls] += r.work;
         g_prof_count[r.cls] += 1;
@@ -1211,8 +1218,6 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
     bt.sync();
 }
 
-}  // namespace
-
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1230,6 +1235,146 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
         return -1;                         \
     }
 
+
+// ---------------------------------------------------------------------------
+// Concurrent lockstep groups of one ragged batch.
+//
+// The per-iteration critical path of an instance is its Schur factorization
+// chain (one launch per 64-column block), so a lockstep batch advances at the
+// pace of its largest instance while the GPU is mostly idle.  A batch is
+// therefore split by block count into up to kMaxGroups classes (block count in
+// (max/2, max], (max/4, max/2], ...); each class runs the lockstep solver on
+// its own HIP stream from its own persistent host thread, so the classes
+// overlap on the device.  Results are per instance and do not depend on the
+// grouping.
+// ---------------------------------------------------------------------------
+constexpr int kMaxGroups = 4;
+int g_max_groups = kMaxGroups;   // cgot_set_max_groups
+
+struct GroupPlan {
+    std::vector<std::vector<int>> idx;                 // instance indices per group
+    std::vector<std::vector<cgot_splr_problem>> probs;
+    std::vector<size_t> off;                           // workspace offset of each group
+    size_t total = 0;
+};
+
+GroupPlan plan_splr_groups(int B, const cgot_splr_problem* probs)
+{
+    GroupPlan g;
+    std::vector<int> nblk(B);
+    int mx = 1;
+    for (int b = 0; b < B; b++) {
+        const int P = (probs[b].m - 1 <= probs[b].n) ? probs[b].m - 1 : probs[b].n;
+        nblk[b] = std::max(1, cdiv(P, CG_NB));
+        mx = std::max(mx, nblk[b]);
+    }
+    const int G = (B >= 8) ? std::max(1, std::min(g_max_groups, kMaxGroups)) : 1;
+    std::vector<std::vector<int>> cls(G);
+    for (int b = 0; b < B; b++) {
+        int k = 0;
+        while (k + 1 < G && (long long)nblk[b] * (2LL << k) <= mx) k++;
+        cls[k].push_back(b);
+    }
+    size_t off = 0;
+    for (auto& c : cls) {
+        if (c.empty()) continue;
+        std::vector<cgot_splr_problem> sub;
+        for (int b : c) sub.push_back(probs[b]);
+        Batch tmp;
+        tmp.B = (int)sub.size();
+        const size_t need = splr_layout(tmp, sub.data(), nullptr);
+        off = (off + 255) & ~(size_t)255;
+        g.off.push_back(off);
+        off += need;
+        g.idx.push_back(c);
+        g.probs.push_back(std::move(sub));
+    }
+    g.total = off + 256;
+    return g;
+}
+
+// A pool of persistent host threads (their pinned staging buffers and streams
+// live as long as the process).  Callers may run concurrently: each call
+// queues its jobs and waits for its own completion count; the pool grows so
+// that every queued job has a thread, up to kMaxWorkers.
+class GroupPool {
+public:
+    static GroupPool& get()
+    {
+        static GroupPool* p = new GroupPool();   // never destroyed: workers are detached
+        return *p;
+    }
+    void run(std::vector<std::function<void()>>& jobs)
+    {
+        Call call;
+        call.remaining = (int)jobs.size();
+        std::unique_lock<std::mutex> lk(mu_);
+        for (auto& j : jobs) q_.push_back({&j, &call});
+        while (idle_ < (int)q_.size() && workers_ < kMaxWorkers) {
+            std::thread(&GroupPool::loop, this).detach();
+            workers_++;
+            idle_++;
+        }
+        cv_.notify_all();
+        call.cv.wait(lk, [&] { return call.remaining == 0; });
+    }
+
+private:
+    static constexpr int kMaxWorkers = 16;
+    struct Call { int remaining = 0; std::condition_variable cv; };
+    struct Item { std::function<void()>* job; Call* call; };
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return !q_.empty(); });
+            Item it = q_.front();
+            q_.pop_front();
+            idle_--;
+            lk.unlock();
+            (*it.job)();
+            lk.lock();
+            idle_++;
+            if (--it.call->remaining == 0) it.call->cv.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Item> q_;
+    int workers_ = 0, idle_ = 0;
+};
+
+// Stream of the calling worker thread on `device` (created once, default priority).
+hipStream_t worker_stream(int device)
+{
+    thread_local std::map<int, hipStream_t> streams;
+    auto it = streams.find(device);
+    if (it != streams.end()) return it->second;
+    hipStream_t st;
+    CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    streams[device] = st;
+    return st;
+}
+
+void splr_solve_group(const cgot_splr_problem* probs, int B, void* ws, hipStream_t stream, cgot_splr_info* info)
+{
+    Batch bt;
+    bt.B = B;
+    bt.stream = stream;
+    splr_layout(bt, probs, ws);
+    std::vector<const double*> ptrs(3 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        ptrs[3 * b] = probs[b].a;
+        ptrs[3 * b + 1] = probs[b].b;
+        ptrs[3 * b + 2] = probs[b].x0;
+    }
+    bt.upload_maps(ptrs);
+    SplrSolver s(bt);
+    s.run(probs, info);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* cgot_last_error(void) { return g_last_error.c_str(); }
@@ -1241,6 +1386,7 @@ void cgot_profile_enable(int32_t on)
 
 void cgot_profile_reset(void)
 {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     for (int k = 0; k < PROF_NCLS; k++) { g_prof_ms[k] = 0; g_prof_work[k] = 0; g_prof_count[k] = 0; }
 }
 
@@ -1248,6 +1394,7 @@ int cgot_profile_read(int32_t cls, double* ms_total, double* work_total, int64_t
 {
     if (cls < 0 || cls >= PROF_NCLS) return -2;
     try { prof_collect(); } catch (const std::exception& e) { g_last_error = e.what(); return -1; }
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     *ms_total = g_prof_ms[cls];
     *work_total = g_prof_work[cls];
     *launches = g_prof_count[cls];
@@ -1258,10 +1405,9 @@ const char* cgot_version(void) { return "curegot_amd 0.1.0 gfx950"; }
 size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs)
 {
     try {
-        Batch bt;
-        bt.B = B;
+        if (B < 1) throw std::invalid_argument("B must be >= 1");
         for (int b = 0; b < B; b++) check_problem(probs[b]);
-        return splr_layout(bt, probs, nullptr);
+        return plan_splr_groups(B, probs).total;
     } catch (const std::exception& e) {
         g_last_error = e.what();
         return 0;
@@ -1274,24 +1420,73 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
     CG_API_BEGIN
     if (B < 1) throw std::invalid_argument("B must be >= 1");
     for (int b = 0; b < B; b++) check_problem(probs[b]);
-    Batch bt;
-    bt.B = B;
-    bt.stream = (hipStream_t)stream;
-    const size_t need = splr_layout(bt, probs, nullptr);
-    if (workspace == nullptr || workspace_bytes < need)
-        throw std::invalid_argument("workspace too small: need " + std::to_string(need) + " bytes");
-    splr_layout(bt, probs, workspace);
-    std::vector<const double*> ptrs(3 * (size_t)B);
-    for (int b = 0; b < B; b++) {
-        ptrs[3 * b] = probs[b].a;
-        ptrs[3 * b + 1] = probs[b].b;
-        ptrs[3 * b + 2] = probs[b].x0;
+    const GroupPlan g = plan_splr_groups(B, probs);
+    if (workspace == nullptr || workspace_bytes < g.total)
+        throw std::invalid_argument("workspace too small: need " + std::to_string(g.total) + " bytes");
+    const int G = (int)g.idx.size();
+    if (G == 1) {
+        splr_solve_group(probs, B, (char*)workspace + g.off[0], (hipStream_t)stream, info);
+        return 0;
     }
-    bt.upload_maps(ptrs);
-    SplrSolver s(bt);
-    s.run(probs, info);
+    // group 0 on the caller's stream in this thread, the others on worker
+    // streams (ordered after the caller's stream on entry and before its later
+    // work on exit), so that each group has its own hardware queue
+    int device = 0;
+    CG_HIP(hipGetDevice(&device));
+    hipEvent_t start;
+    CG_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+    CG_HIP(hipEventRecord(start, (hipStream_t)stream));
+    std::vector<std::vector<cgot_splr_info>> sub(G);
+    std::vector<std::string> err(G);
+    std::vector<hipEvent_t> done(G, nullptr);
+    std::vector<std::function<void()>> jobs;
+    for (int k = 0; k < G; k++) sub[k].assign(g.idx[k].size(), cgot_splr_info{});
+    for (int k = 1; k < G; k++) {
+        jobs.push_back([&, k]() {
+            try {
+                CG_HIP(hipSetDevice(device));
+                hipStream_t st = worker_stream(device);
+                CG_HIP(hipStreamWaitEvent(st, start, 0));
+                splr_solve_group(g.probs[k].data(), (int)g.probs[k].size(), (char*)workspace + g.off[k], st,
+                                 sub[k].data());
+                CG_HIP(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
+                CG_HIP(hipEventRecord(done[k], st));
+            } catch (const std::exception& e) {
+                err[k] = e.what();
+            }
+        });
+    }
+    std::thread host([&]() { GroupPool::get().run(jobs); });
+    try {
+        splr_solve_group(g.probs[0].data(), (int)g.probs[0].size(), (char*)workspace + g.off[0],
+                         (hipStream_t)stream, sub[0].data());
+    } catch (const std::exception& e) {
+        err[0] = e.what();
+    }
+    host.join();
+    for (int k = 1; k < G; k++) {
+        if (done[k]) {
+            (void)hipStreamWaitEvent((hipStream_t)stream, done[k], 0);
+            (void)hipEventDestroy(done[k]);
+        }
+    }
+    (void)hipEventDestroy(start);
+    for (int k = 0; k < G; k++)
+        if (!err[k].empty()) throw std::runtime_error(err[k]);
+    for (int k = 0; k < G; k++)
+        for (size_t q = 0; q < g.idx[k].size(); q++) info[g.idx[k][q]] = sub[k][q];
     return 0;
     CG_API_END
+}
+
+int cgot_set_max_groups(int32_t groups)
+{
+    if (groups < 1 || groups > kMaxGroups) {
+        g_last_error = "max groups must be in [1, " + std::to_string(kMaxGroups) + "]";
+        return -2;
+    }
+    g_max_groups = groups;
+    return 0;
 }
 
 

@@ -15,7 +15,7 @@
 //   CG_PT_GAMMA : gamma
 //   CG_PT_TRIAL : gamma_prev + step*dir on the first N entries, beta_{m-1} = 0
 //   CG_PT_BCD   : gamma_bcd
-CG_DEV double cg_point(const CgInst& I, int mode, double t, int k)
+CG_DEV double cg_point(const CgInstD& I, int mode, double t, int k)
 {
     if (mode == CG_PT_GAMMA) return I.gamma[k];
     if (mode == CG_PT_BCD) return I.gamma_bcd[k];
@@ -28,7 +28,7 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
     const int t = blockIdx.x;
     b = tile_inst[t];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lt = t - I.tile0;
     rb = lt / I.nstrips;
     st = lt - rb * I.nstrips;
@@ -43,7 +43,7 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
 // whole group before the exponentials, so 4*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
 template <int C>
-CG_DEV void cg_tpass_tile(const CgInst& I, int mode, double t, int rb, int st)
+CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
@@ -66,7 +66,7 @@ CG_DEV void cg_tpass_tile(const CgInst& I, int mode, double t, int rb, int st)
         for (int r = 0; r < 4; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = cg_point(I, mode, t, i);
-            const double* Mi = I.M + (size_t)i * m + s0 + lane;
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
 #pragma unroll
             for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
         }
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ inst
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     switch (I.c) {
         case 1: cg_tpass_tile<1>(I, mode, t, rb, st); break;
@@ -121,11 +121,11 @@ __global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ in
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
+    const CgInstD I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
-    const double* beta = (mode == CG_PT_BCD) ? (I.gamma_bcd + n) : (I.gamma + n);
+    const auto beta = (mode == CG_PT_BCD) ? (I.gamma_bcd + n) : (I.gamma + n);
     double bj[CG_CMAX];
 #pragma unroll
     for (int k = 0; k < CG_CMAX; k++) {
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ in
     for (int r = 0; r < CG_RPW; r++) {
         const int i = i0 + r;
         if (i >= n) break;
-        const double* Mi = I.M + (size_t)i * m;
+        const auto Mi = I.M + (size_t)i * m;
         double x[CG_CMAX];
         double mx = -INFINITY;
 #pragma unroll
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ in
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
+    const CgInstD I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
-    const double* alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
+    const auto alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     double mx[CG_CMAX], sm[CG_CMAX];
 #pragma unroll
     for (int k = 0; k < CG_CMAX; k++) { mx[k] = -INFINITY; sm[k] = 0.0; }
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ in
         const int i = i0 + r;
         if (i >= n) break;
         const double ai = alpha[i];
-        const double* Mi = I.M + (size_t)i * m;
+        const auto Mi = I.M + (size_t)i * m;
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
@@ -233,15 +233,15 @@ __global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
+    const CgInstD I = cg_inst(insts, b);
     if (mode == CG_QV_PCG && I.scal[PCG_STATUS] != 0.0) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
     const bool mv = (mode == CG_QV_MATVEC || mode == CG_QV_PCG);
-    const double* v = (mode == CG_QV_MATVEC) ? vecs[b] : nullptr;
+    const CG_AS1 double* v = (mode == CG_QV_MATVEC) ? (const CG_AS1 double*)vecs[b] : nullptr;
     const double pbeta = (mode == CG_QV_PCG) ? I.scal[PCG_BETA] : 0.0;
-    const double* G = (mode == CG_QV_RHS) ? Gs[b] : nullptr;
+    const CG_AS1 double* G = (mode == CG_QV_RHS) ? (const CG_AS1 double*)Gs[b] : nullptr;
     auto vat = [&](int k) -> double {
         return (mode == CG_QV_PCG) ? (I.sv[k] + pbeta * I.uv[k]) : v[k];
     };
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
         if (i >= n) break;
         const double ai = I.gamma[i];
         const double va = mv ? vat(i) : 0.0;
-        const double* Mi = I.M + (size_t)i * m;
+        const auto Mi = I.M + (size_t)i * m;
         double rs = 0.0, rt = 0.0;
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
@@ -321,12 +321,12 @@ __global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
+    const CgInstD I = cg_inst(insts, b);
     if (I.plan == nullptr) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
-    const double* dual = I.gamma;
+    const auto dual = I.gamma;
     double bj[CG_CMAX];
 #pragma unroll
     for (int k = 0; k < CG_CMAX; k++) {
@@ -338,8 +338,8 @@ __global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts
         const int i = i0 + r;
         if (i >= n) break;
         const double ai = dual[i];
-        const double* Mi = I.M + (size_t)i * m;
-        double* Pi = I.plan + (size_t)i * m;
+        const auto Mi = I.M + (size_t)i * m;
+        auto Pi = I.plan + (size_t)i * m;
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
@@ -359,13 +359,13 @@ __global__ __launch_bounds__(CG_WG) void k_grad_assemble(const CgInst* __restric
 {
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst I = insts[b];
+    const CgInstD I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
-    const double* lam = lams[b];
-    const double* G = Gs[b];
-    double* out = outs[b];
+    const CG_AS1 double* lam = (const CG_AS1 double*)lams[b];
+    const CG_AS1 double* G = (const CG_AS1 double*)Gs[b];
+    CG_AS1 double* out = (CG_AS1 double*)outs[b];
     double bj[CG_CMAX], lb[CG_CMAX];
 #pragma unroll
     for (int k = 0; k < CG_CMAX; k++) {

@@ -17,7 +17,7 @@
 #include "cgot_kernels.h"
 
 template <int NV>
-CG_DEV bool cg_inst_reduce_p(const CgInst& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
+CG_DEV bool cg_inst_reduce_p(const CgInstD& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
 {
     __shared__ double lds4[4];
     __shared__ double tot[NV];
@@ -31,14 +31,14 @@ CG_DEV bool cg_inst_reduce_p(const CgInst& I, int wgi, double (&v)[NV], double (
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(&I.counter[slot], 1u);
+        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (prev == (unsigned int)(I.nvec - 1));
     }
     __syncthreads();
     if (!is_last) return false;
     __threadfence();
     if (threadIdx.x < NV) {
-        const volatile double* wp = I.wgpart + (size_t)threadIdx.x * I.nvec;
+        const volatile CG_AS1 double* wp = I.wgpart + (size_t)threadIdx.x * I.nvec;
         double s = 0.0;
         for (int q = 0; q < I.nvec; q++) s += wp[q];
         tot[threadIdx.x] = s;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ i
 {
     const int b = wg_inst[blockIdx.x];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int wgi = blockIdx.x - I.vec0;
     const int q = wgi * CG_WG + threadIdx.x;
     const int n = I.n, m = I.m;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ 
 {
     int b, wgi;
     if (!cg_pcg_decode(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     const int n = I.n, m = I.m;
     const double beta = I.scal[PCG_BETA];
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_update(const CgInst* __restrict__
 {
     int b, wgi;
     if (!cg_pcg_decode(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     const double alpha = I.scal[PCG_ALPHA];
     double v[2] = {0.0, 0.0};

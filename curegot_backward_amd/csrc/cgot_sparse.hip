@@ -10,12 +10,12 @@
 #include "cgot_common.h"
 #include "cgot_kernels.h"
 
-CG_DEV double cg_T(const CgInst& I, int i, int j)
+CG_DEV double cg_T(const CgInstD& I, int i, int j)
 {
     return exp((I.gamma[i] + I.gamma[I.n + j] - I.M[(size_t)i * I.m + j]) * I.inv_reg);
 }
 
-CG_DEV unsigned long long cg_kth(const CgInst& I) { return (unsigned long long)__double_as_longlong(I.scal[CS_KTH_HI]); }
+CG_DEV unsigned long long cg_kth(const CgInstD& I) { return (unsigned long long)__double_as_longlong(I.scal[CS_KTH_HI]); }
 
 CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const double* ctl,
                             int& b, int& rb, int& st)
@@ -23,7 +23,7 @@ CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const dou
     const int t = blockIdx.x;
     b = tile_inst[t];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lt = t - I.tile0;
     rb = lt / I.nstrips;
     st = lt - rb * I.nstrips;
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
 {
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     __shared__ unsigned int h[CG_HBINS];
     for (int q = threadIdx.x; q < CG_HBINS; q += CG_WG) h[q] = 0u;
     __syncthreads();
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
     const int nb = (pass == 0) ? 4096 : CG_HBINS;
     for (int q = threadIdx.x; q < nb; q += CG_WG) {
         const unsigned int v = h[q];
-        if (v) atomicAdd(&I.hist[q], v);
+        if (v) __hip_atomic_fetch_add(&I.hist[q], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(CG_WG) void k_hist_scan(const CgInst* __restrict__ 
 {
     const int b = blockIdx.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int nb = (pass == 0) ? 4096 : CG_HBINS;
     const int cs = nb / CG_WG;
     __shared__ unsigned long long csum[CG_WG];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restri
 {
     const int b = row_inst[blockIdx.x];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
     if (i >= I.n) return;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_scan(const CgInst* __restric
 {
     const int b = blockIdx.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     __shared__ long long lds[CG_WG];
     __shared__ long long total;
     const int n = I.n;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restri
 {
     const int b = row_inst[blockIdx.x];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
     if (i >= I.n) return;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
 {
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw, mm = I.m - 1;
     const unsigned long long kth = cg_kth(I);
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_scan(const CgInst* __restric
 {
     const int b = blockIdx.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     __shared__ long long lds[CG_WG];
     __shared__ long long total;
     const int mm = I.m - 1, m = I.m;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
 {
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw, mm = I.m - 1;
     const unsigned long long kth = cg_kth(I);
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restri
 {
     const int b = wave_inst[blockIdx.x];
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = (blockIdx.x - I.row0) * CG_WAVES + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ in
                                                    int* __restrict__ rowptr)
 {
     const int b = wg_inst[blockIdx.x];
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = (blockIdx.x - I.vec0) * CG_WG + threadIdx.x;
     const int n = I.n;
     if (q < n) {

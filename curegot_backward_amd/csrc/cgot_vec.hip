@@ -19,7 +19,7 @@ CG_DEV bool cg_decode_vec(const CgInst* insts, const int* wg_inst, const double*
 // Reduce NV per-thread values over all vec workgroups of the instance.  Returns
 // true in the last workgroup, where out[] holds the totals (all threads).
 template <int NV>
-CG_DEV bool cg_inst_reduce(const CgInst& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
+CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
 {
     __shared__ double lds4[4];
     __shared__ double tot[NV];
@@ -33,7 +33,7 @@ CG_DEV bool cg_inst_reduce(const CgInst& I, int wgi, double (&v)[NV], double (&o
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(&I.counter[slot], 1u);
+        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (prev == (unsigned int)(I.nvec - 1));
     }
     __syncthreads();
@@ -41,7 +41,7 @@ CG_DEV bool cg_inst_reduce(const CgInst& I, int wgi, double (&v)[NV], double (&o
     __threadfence();
     if (threadIdx.x < NV) {
         const int k = threadIdx.x;
-        const volatile double* wp = I.wgpart + (size_t)k * I.nvec;
+        const volatile CG_AS1 double* wp = I.wgpart + (size_t)k * I.nvec;
         double s = 0.0;
         for (int q = 0; q < I.nvec; q++) s += wp[q];
         tot[k] = s;
@@ -65,12 +65,12 @@ __global__ __launch_bounds__(CG_WG) void k_tpass_fin(const CgInst* __restrict__ 
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int n = I.n, m = I.m, N = I.N;
     const int q = wgi * CG_WG + threadIdx.x;
     const double t = ctl[b * CG_NCTL + CTL_STEP];
-    double* grad = (mode == CG_PT_BCD) ? I.grad_bcd : I.grad;
-    const double* gam = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
+    auto grad = (mode == CG_PT_BCD) ? I.grad_bcd : I.grad;
+    const auto gam = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // tsum, dot, g2, dg, dg0, g2 over alpha
     if (q < n) {
         double rs = 0.0;
@@ -126,13 +126,13 @@ __global__ __launch_bounds__(CG_WG) void k_rowlse_fin(const CgInst* __restrict__
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.n) return;
     double mx = -INFINITY, s = 0.0;
     for (int st = 0; st < I.nstrips; st++)
         cg_lse_merge(mx, s, I.rp0[(size_t)q * I.nstrips + st], I.rp1[(size_t)q * I.nstrips + st]);
-    double* alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
+    auto alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     alpha[q] = I.reg * (I.logab[q] - (mx + log(s)));
 }
 
@@ -144,13 +144,13 @@ __global__ __launch_bounds__(CG_WG) void k_collse_fin(const CgInst* __restrict__
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.m) return;
     double mx = -INFINITY, s = 0.0;
     for (int r = 0; r < I.nrb; r++)
         cg_lse_merge(mx, s, I.cp0[(size_t)r * I.m + q], I.cp1[(size_t)r * I.m + q]);
-    double* beta = ((mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma) + I.n;
+    auto beta = ((mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma) + I.n;
     beta[q] = I.reg * (I.logab[I.n + q] - (mx + log(s)));
 }
 
@@ -162,8 +162,8 @@ __global__ __launch_bounds__(CG_WG) void k_gauge_shift(const CgInst* __restrict_
 {
     const int b = blockIdx.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
-    const CgInst& I = insts[b];
-    double* g = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
+    const CgInstD& I = cg_inst(insts, b);
+    auto g = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     const double sh = g[I.N];
     __syncthreads();
     for (int k = threadIdx.x; k < I.N; k += CG_WG) g[k] += (k < I.n) ? sh : -sh;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(CG_WG) void k_init_logab(const CgInst* __restrict__
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     const int n = I.n, m = I.m;
     if (k >= n + m) return;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(CG_WG) void k_low_rank(const CgInst* __restrict__ i
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (k < I.N) {
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(CG_WG) void k_save_history(const CgInst* __restrict
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k < I.N) {
         I.gamma_prev[k] = I.gamma[k];
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(CG_WG) void k_copy_bcd(const CgInst* __restrict__ i
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k < I.n + I.m) I.gamma[k] = I.gamma_bcd[k];
 }
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(CG_WG) void k_gamma_from_step(const CgInst* __restr
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k < I.N) I.gamma[k] = I.gamma_prev[k] + ctl[b * CG_NCTL + CTL_STEP] * I.dir[k];
 }
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(CG_WG) void k_woodbury_dots(const CgInst* __restric
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     if (I.scal[CS_LOWRANK] == 0.0) return;   // uniform per instance
     const int k = wgi * CG_WG + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(CG_WG) void k_woodbury_apply(const CgInst* __restri
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k >= I.N) return;
     double d = I.dir[k];
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(CG_WG) void k_output_dual(const CgInst* __restrict_
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k < I.n + I.m && I.dual != nullptr) I.dual[k] = I.gamma[k];
 }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(CG_WG) void k_vcopy(const CgInst* __restrict__ inst
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (what == 0) {
         if (k < I.n + I.m) I.gamma_prev[k] = I.gamma[k];
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(CG_WG) void k_bcd_diff(const CgInst* __restrict__ i
 {
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInst& I = insts[b];
+    const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (q < I.n) {

@@ -73,6 +73,14 @@ typedef struct cgot_splr_info {
 } cgot_splr_info;
 
 size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
+/* A batch of B >= 8 instances is split by Schur block count into up to 4
+ * classes that run concurrently (each on its own internal HIP stream and host
+ * thread, ordered after `stream` on entry and before its later work on exit).
+ * Results are per instance and independent of the split.  cgot_set_max_groups
+ * (1..4, default 4) bounds the number of classes; 1 runs the whole batch in one
+ * lockstep group on `stream`.  No counterpart in the reference, whose
+ * splr_ot_batch loops over instances (curegot/autograd.py:340-367). */
+int cgot_set_max_groups(int32_t groups);
 int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
                     void* stream, cgot_splr_info* info);
 
@@ -139,11 +147,12 @@ int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const
                     const double* rhs, double* sol,
                     void* workspace, size_t workspace_bytes, void* stream);
 
-/* Kernel timing (HIP events around each launch on the solve stream), for
+/* Kernel timing (HIP events around each launch on the stream it runs on), for
  * roofline figures.  Classes: 0 T pass (bytes), 1 row LSE (bytes), 2 column LSE
  * (bytes), 3 adjoint Q.v pass (bytes), 4 Cholesky trailing update (flops),
- * 5 Cholesky panel, 6 backward substitution.  Not thread-safe; no counterpart
- * in the reference (its Timer, src/timer.h, prints host wall times). */
+ * 5 Cholesky panel, 6 backward substitution.  While concurrent groups run
+ * (below), a launch's event interval includes the other groups' overlap.  No
+ * counterpart in the reference (its Timer, src/timer.h, prints host wall times). */
 void cgot_profile_enable(int32_t on);
 void cgot_profile_reset(void);
 int cgot_profile_read(int32_t cls, double* ms_total, double* work_total, int64_t* launches);

@@ -29,6 +29,36 @@ def test_ragged_batch_bitwise_equals_single(cg):
         assert torch.equal(outs[i]["dual"], single["dual"])
 
 
+def test_concurrent_groups_bitwise_equal_one_group(cg):
+    """The batch is split into concurrent size classes (own streams and host
+    threads); the split must not change any result."""
+    from curegot_backward_amd import _lib
+    from curegot_backward_amd.workloads import ragged_batch
+    probs = ragged_batch(B=24, seed=5, lo=64, hi=1200)
+    Ms = [p[0].cuda() for p in probs]
+    As = [p[1].cuda() for p in probs]
+    Bs = [p[2].cuda() for p in probs]
+    etas = [p[3] for p in probs]
+    runs = {}
+    try:
+        for g in (1, 4, 2):
+            _lib.set_max_groups(g)
+            print("groups", g, flush=True)
+            runs[g] = cg.sinkhorn_splr_bwd_batch(Ms, As, Bs, etas, 1e-8, 1000)
+            torch.cuda.synchronize()
+    finally:
+        _lib.set_max_groups(4)
+    for g in (4, 2):
+        for i in range(len(probs)):
+            assert runs[g][i]["niter"] == runs[1][i]["niter"]
+            assert torch.equal(runs[g][i]["dual"], runs[1][i]["dual"])
+            assert torch.equal(runs[g][i]["plan"], runs[1][i]["plan"])
+            # the context views of each group's workspace region
+            assert torch.equal(runs[g][i]["context"].inv_diag(), runs[1][i]["context"].inv_diag())
+    with pytest.raises(ValueError):
+        _lib.set_max_groups(0)
+
+
 def test_splr_ot_batch_equals_per_slice(cg):
     from curegot_backward_amd.workloads import make_problem
     B, n, m = 4, 50, 60
