@@ -190,5 +190,61 @@ CG_DEV void cg_lse_merge(double& mx, double& s, double mx2, double s2)
     else { s = s + s2 * exp(mx2 - mx); }
 }
 
+// sum_{k < count} p[k*stride] in index order (the same rounding as a plain
+// loop), with CG_SUM_U loads in flight per round trip.
+#define CG_SUM_U 16
+CG_DEV double cg_ordered_sum(const CG_AS1 double* p, int count, size_t stride)
+{
+    double s = 0.0;
+    for (int k = 0; k < count; k += CG_SUM_U) {
+        double t[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) t[u] = (k + u < count) ? p[(size_t)(k + u) * stride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++)
+            if (k + u < count) s += t[u];
+    }
+    return s;
+}
+
+// Two interleaved ordered sums (same index set, different arrays).
+CG_DEV void cg_ordered_sum2(const CG_AS1 double* p, const CG_AS1 double* q, int count, size_t stride, double& sp,
+                            double& sq)
+{
+    sp = 0.0;
+    sq = 0.0;
+    for (int k = 0; k < count; k += CG_SUM_U) {
+        double a[CG_SUM_U], c[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) {
+            const bool ok = k + u < count;
+            a[u] = ok ? p[(size_t)(k + u) * stride] : 0.0;
+            c[u] = ok ? q[(size_t)(k + u) * stride] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++)
+            if (k + u < count) { sp += a[u]; sq += c[u]; }
+    }
+}
+
+// Ordered LSE merge of count (max, sum) partials, loads batched as above.
+CG_DEV void cg_ordered_lse(const CG_AS1 double* pm, const CG_AS1 double* ps, int count, size_t stride, double& mx,
+                           double& s)
+{
+    mx = -INFINITY;
+    s = 0.0;
+    for (int k = 0; k < count; k += CG_SUM_U) {
+        double a[CG_SUM_U], c[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) {
+            const bool ok = k + u < count;
+            a[u] = ok ? pm[(size_t)(k + u) * stride] : -INFINITY;
+            c[u] = ok ? ps[(size_t)(k + u) * stride] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) cg_lse_merge(mx, s, a[u], c[u]);
+    }
+}
+
 // Order-preserving key of a non-negative double (T >= 0).
 CG_DEV unsigned long long cg_key(double v) { return (unsigned long long)__double_as_longlong(v); }

@@ -146,14 +146,26 @@ CG_DEV bool cg_decode_wave(const CgInst* insts, const int* wave_inst, const doub
 }
 
 // Build row s2 of the lower Schur complement, one wave per row:
-// S[s2][s2] = D_small[s2]; then for each large index l adjacent to s2
-// (ascending) the lanes cover the small indices s <= s2 adjacent to l:
-// S[s2][s] -= w(s2,l) w(l,s) / D_large[l].  Each entry accumulates its terms
-// in ascending l; a fence between l steps orders the lanes' updates.
+// S[s2][s] = D_small[s2] delta - sum_{l adjacent to s2} w(s2,l) w(l,s) / D_large[l]
+// for s <= s2.  Rows of up to CG_SCHUR_LDS entries accumulate in LDS: the wave
+// takes the adjacent l in chunks of 8 (one group of 8 lanes per l, so the
+// gathers of 8 l's are in flight together) and applies the 8 groups' updates
+// one group after another, so every entry receives its terms in a fixed
+// order; the finished row is written once.  Longer rows update S in place, l
+// by l in ascending order with a fence between l steps.
+#define CG_SCHUR_LDS 1024
+CG_DEV void cg_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict__ insts,
                                                        const int* __restrict__ wave_inst,
                                                        const double* __restrict__ ctl)
 {
+    __shared__ double rowbuf[CG_WAVES][CG_SCHUR_LDS];
     int b, s2;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, s2)) return;
     const CgInstD& I = cg_inst(insts, b);
@@ -173,17 +185,49 @@ __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict_
         Ds = I.Da; Dl = I.Db;
     }
     auto Srow = I.S + (size_t)s2 * P;
-    if (lane == 0) Srow[s2] = Ds[s2];
-    for (int p = sp[s2]; p < sp[s2 + 1]; p++) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        const int l = sidx[p];
-        const double ws = sval[p];
-        const double dl = Dl[l];
-        for (int q = lp[l] + lane; q < lp[l + 1]; q += 64) {
-            const int s = lidx[q];
-            if (s <= s2) Srow[s] -= ws * lval[q] / dl;
+    const int p0 = sp[s2], p1 = sp[s2 + 1];
+    if (P > CG_SCHUR_LDS) {
+        if (lane == 0) Srow[s2] = Ds[s2];
+        for (int p = p0; p < p1; p++) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            const int l = sidx[p];
+            const double ws = sval[p];
+            const double dl = Dl[l];
+            for (int q = lp[l] + lane; q < lp[l + 1]; q += 64) {
+                const int s = lidx[q];
+                if (s <= s2) Srow[s] -= ws * lval[q] / dl;
+            }
+        }
+        return;
+    }
+    double* R = rowbuf[threadIdx.x >> 6];
+    const double dss = Ds[s2];
+    for (int s = lane; s <= s2; s += 64) R[s] = (s == s2) ? dss : 0.0;
+    cg_wave_sync();
+    const int g = lane >> 3, t = lane & 7;
+    for (int pb = p0; pb < p1; pb += 8) {
+        const int p = pb + g;
+        const bool okp = p < p1;
+        const int l = okp ? sidx[p] : 0;
+        const double ws = okp ? sval[p] : 0.0;
+        const double dl = okp ? Dl[l] : 1.0;
+        const int q0 = okp ? lp[l] : 0;
+        const int q1 = okp ? lp[l + 1] : 0;
+        int len = q1 - q0;
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) len = max(len, __shfl_xor(len, o));
+        for (int qb = 0; qb < len; qb += 8) {
+            const int q = q0 + qb + t;
+            const bool okq = q < q1;
+            const int s = okq ? lidx[q] : 0;
+            const double v = okq ? ws * lval[q] / dl : 0.0;
+            for (int gg = 0; gg < 8; gg++) {
+                if (g == gg && okq && s <= s2) R[s] -= v;
+                cg_wave_sync();
+            }
         }
     }
+    for (int s = lane; s <= s2; s += 64) Srow[s] = R[s];
 }
 
 // Schur right-hand sides for [g, y]: r_s = g_s - sum_l w(s,l) g_l / D_l (wave per s).

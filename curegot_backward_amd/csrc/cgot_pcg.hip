@@ -38,10 +38,7 @@ CG_DEV bool cg_inst_reduce_p(const CgInstD& I, int wgi, double (&v)[NV], double 
     if (!is_last) return false;
     __threadfence();
     if (threadIdx.x < NV) {
-        const volatile CG_AS1 double* wp = I.wgpart + (size_t)threadIdx.x * I.nvec;
-        double s = 0.0;
-        for (int q = 0; q < I.nvec; q++) s += wp[q];
-        tot[threadIdx.x] = s;
+        tot[threadIdx.x] = cg_ordered_sum(I.wgpart + (size_t)threadIdx.x * I.nvec, I.nvec, 1);
     }
     if (threadIdx.x == 0) I.counter[slot] = 0u;
     __syncthreads();
@@ -82,20 +79,14 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ i
         if (mu != 0.0) v[2] = 1.0;
     };
     if (q < n) {
-        double a = 0.0, t = 0.0;
-        for (int s = 0; s < I.nstrips; s++) {
-            a += I.rp0[(size_t)q * I.nstrips + s];
-            t += I.rp1[(size_t)q * I.nstrips + s];
-        }
+        double a, t;
+        cg_ordered_sum2(I.rp0 + (size_t)q * I.nstrips, I.rp1 + (size_t)q * I.nstrips, I.nstrips, 1, a, t);
         I.rowsum[q] = t;
         setk(q, a);
     }
     if (q < m) {
-        double a = 0.0, t = 0.0;
-        for (int r = 0; r < I.nrb; r++) {
-            a += I.cp0[(size_t)r * m + q];
-            t += I.cp1[(size_t)r * m + q];
-        }
+        double a, t;
+        cg_ordered_sum2(I.cp0 + q, I.cp1 + q, I.nrb, m, a, t);
         I.colsum[q] = t;
         if (q < m - 1) setk(n + q, a);
     }
@@ -123,8 +114,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ 
     const double beta = I.scal[PCG_BETA];
     double v[1] = {0.0};
     if (q < n) {
-        double a = 0.0;
-        for (int s = 0; s < I.nstrips; s++) a += I.rp0[(size_t)q * I.nstrips + s];
+        const double a = cg_ordered_sum(I.rp0 + (size_t)q * I.nstrips, I.nstrips, 1);
         const double p = I.sv[q] + beta * I.uv[q];
         const double w = I.rowsum[q] * p + a;
         I.uv[q] = p;
@@ -132,8 +122,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ 
         v[0] += p * w;
     }
     if (q < m - 1) {
-        double a = 0.0;
-        for (int r = 0; r < I.nrb; r++) a += I.cp0[(size_t)r * m + q];
+        const double a = cg_ordered_sum(I.cp0 + q, I.nrb, m);
         const int k = n + q;
         const double p = I.sv[k] + beta * I.uv[k];
         const double w = a + I.colsum[q] * p;

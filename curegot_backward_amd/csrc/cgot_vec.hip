@@ -41,10 +41,7 @@ CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&
     __threadfence();
     if (threadIdx.x < NV) {
         const int k = threadIdx.x;
-        const volatile CG_AS1 double* wp = I.wgpart + (size_t)k * I.nvec;
-        double s = 0.0;
-        for (int q = 0; q < I.nvec; q++) s += wp[q];
-        tot[k] = s;
+        tot[k] = cg_ordered_sum(I.wgpart + (size_t)k * I.nvec, I.nvec, 1);
     }
     if (threadIdx.x == 0) I.counter[slot] = 0u;
     __syncthreads();
@@ -73,8 +70,7 @@ __global__ __launch_bounds__(CG_WG) void k_tpass_fin(const CgInst* __restrict__ 
     const auto gam = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // tsum, dot, g2, dg, dg0, g2 over alpha
     if (q < n) {
-        double rs = 0.0;
-        for (int s = 0; s < I.nstrips; s++) rs += I.rp0[(size_t)q * I.nstrips + s];
+        const double rs = cg_ordered_sum(I.rp0 + (size_t)q * I.nstrips, I.nstrips, 1);
         I.rowsum[q] = rs;
         const double g = rs - I.ab[q];
         grad[q] = g;
@@ -87,8 +83,7 @@ __global__ __launch_bounds__(CG_WG) void k_tpass_fin(const CgInst* __restrict__ 
         v[1] = p * I.ab[q];
     }
     if (q < m) {
-        double cs = 0.0;
-        for (int r = 0; r < I.nrb; r++) cs += I.cp0[(size_t)r * m + q];
+        const double cs = cg_ordered_sum(I.cp0 + q, I.nrb, m);
         I.colsum[q] = cs;
         const int k = n + q;
         if (k < N) {
@@ -129,9 +124,8 @@ __global__ __launch_bounds__(CG_WG) void k_rowlse_fin(const CgInst* __restrict__
     const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.n) return;
-    double mx = -INFINITY, s = 0.0;
-    for (int st = 0; st < I.nstrips; st++)
-        cg_lse_merge(mx, s, I.rp0[(size_t)q * I.nstrips + st], I.rp1[(size_t)q * I.nstrips + st]);
+    double mx, s;
+    cg_ordered_lse(I.rp0 + (size_t)q * I.nstrips, I.rp1 + (size_t)q * I.nstrips, I.nstrips, 1, mx, s);
     auto alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     alpha[q] = I.reg * (I.logab[q] - (mx + log(s)));
 }
@@ -147,9 +141,8 @@ __global__ __launch_bounds__(CG_WG) void k_collse_fin(const CgInst* __restrict__
     const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.m) return;
-    double mx = -INFINITY, s = 0.0;
-    for (int r = 0; r < I.nrb; r++)
-        cg_lse_merge(mx, s, I.cp0[(size_t)r * I.m + q], I.cp1[(size_t)r * I.m + q]);
+    double mx, s;
+    cg_ordered_lse(I.cp0 + q, I.cp1 + q, I.nrb, I.m, mx, s);
     auto beta = ((mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma) + I.n;
     beta[q] = I.reg * (I.logab[I.n + q] - (mx + log(s)));
 }
