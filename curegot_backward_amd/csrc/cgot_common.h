@@ -14,7 +14,7 @@
 
 #define CG_WG 256          // threads per workgroup (4 wave64)
 #define CG_WAVES 4
-#define CG_CMAX 8          // columns per lane in an M-pass strip
+#define CG_CMAX 4          // columns per lane in an M-pass strip
 #define CG_RPW 8           // rows per wave in an M-pass tile
 #define CG_RB (CG_WAVES * CG_RPW)   // rows per M-pass tile
 #define CG_NB 64           // dense Cholesky block size

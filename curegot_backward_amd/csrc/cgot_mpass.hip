@@ -39,11 +39,11 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
 // T pass: T_ij = exp((alpha_i + beta_j - M_ij)/reg); row and column partial sums.
 // Reference: T_fused_kernel / line_search_fused_kernel2,
 // src/sinkhorn_splr_kernel.cu:214-338 (float atomics there; fixed-order here).
-// Each wave takes its CG_RPW rows in groups of 4 and issues the loads of a
-// whole group before the exponentials, so 4*C loads per lane are in flight.
+// Each wave takes its CG_RPW rows in groups of R and issues the loads of a
+// whole group before the exponentials, so R*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
 template <int C>
-CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st)
+CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, double (*lds)[CG_CMAX * 64])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
@@ -59,11 +59,13 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st)
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
     const double ir = I.inv_reg;
+    // rows per load group: 4*C loads per lane in flight at C <= 4, 2*C at C = 8
+    constexpr int R = (C >= 8) ? 2 : 4;
 #pragma unroll
-    for (int r0 = 0; r0 < CG_RPW; r0 += 4) {
-        double mv[4][C], ai[4];
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        double mv[R][C], ai[R];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = cg_point(I, mode, t, i);
             const auto Mi = I.M + (size_t)i * m + s0 + lane;
@@ -71,7 +73,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st)
             for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < R; r++) {
             const int i = i0 + r0 + r;
             double rs = 0.0;
 #pragma unroll
@@ -84,7 +86,6 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st)
             if (lane == 0 && i < n) I.rp0[(size_t)i * I.nstrips + st] = rs;
         }
     }
-    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
 #pragma unroll
     for (int k = 0; k < C; k++) lds[w][lane + 64 * k] = cs[k];
     __syncthreads();
@@ -102,11 +103,11 @@ __global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ inst
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
     switch (I.c) {
-        case 1: cg_tpass_tile<1>(I, mode, t, rb, st); break;
-        case 2: cg_tpass_tile<2>(I, mode, t, rb, st); break;
-        case 4: cg_tpass_tile<4>(I, mode, t, rb, st); break;
-        default: cg_tpass_tile<8>(I, mode, t, rb, st); break;
+        case 1: cg_tpass_tile<1>(I, mode, t, rb, st, lds); break;
+        case 2: cg_tpass_tile<2>(I, mode, t, rb, st, lds); break;
+        default: cg_tpass_tile<CG_CMAX>(I, mode, t, rb, st, lds); break;
     }
 }
 
