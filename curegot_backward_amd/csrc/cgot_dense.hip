@@ -388,6 +388,7 @@ CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     __shared__ int bad;
+    __shared__ double cb[2][16];
     if (tid == 0) bad = 0;
     if (nb < CG_NB) {
         for (int e = tid; e < CG_NB * CG_NB; e += CG_WG) {
@@ -407,16 +408,38 @@ CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int
             for (int k = 0; k < 16; k++) a[k] = act ? L[b0 + lane][b0 + k] : 0.0;
             bool badp = false;
             double myd = 0.0;
+            // The pivot chain runs through readlane: the next pivot
+            // a_{c+1}[c+1] - l_{c+1,c}^2 uses only lane c+1's own values.  The
+            // other column entries l_kc reach the lanes through LDS (cb,
+            // double-buffered by column parity), off the pivot chain.
+            // Pivot chain: pivot c -> l_rc -> l_{c+1,c} (readlane) -> update of
+            // column c+1 -> pivot c+1 (readlane) -> its square root.  The
+            // other entries l_kc, k >= c+2, reach the lanes through LDS (cb,
+            // double-buffered by column parity); they are first needed one
+            // column later, so their latency overlaps the next pivot step.
+            double piv = cg_readlane_d(a[0], 0);
+            badp = !(piv > 0.0);
+            double sq, rs;
+            cg_sqrt_rsqrt(piv, sq, rs);
 #pragma unroll
             for (int c = 0; c < 16; c++) {
-                const double piv = cg_readlane_d(a[c], c);
-                badp = badp || !(piv > 0.0);
-                double sq, rs;
-                cg_sqrt_rsqrt(piv, sq, rs);
                 a[c] = (lane == c) ? sq : a[c] * rs;
                 myd = (lane == c) ? rs : myd;
+                if (c < 15) {
+                    a[c + 1] = fma(-a[c], cg_readlane_d(a[c], c + 1), a[c + 1]);
+                    const double pnext = cg_readlane_d(a[c + 1], c + 1);
+                    double lk[16];
+                    if (c < 14) {
+                        if (lane < 16) cb[c & 1][lane] = a[c];
+                        cg_wave_sync();
 #pragma unroll
-                for (int k = c + 1; k < 16; k++) a[k] = fma(-a[c], cg_readlane_d(a[c], k), a[k]);
+                        for (int k = c + 2; k < 16; k++) lk[k] = cb[c & 1][k];
+                    }
+                    badp = badp || !(pnext > 0.0);
+                    cg_sqrt_rsqrt(pnext, sq, rs);
+#pragma unroll
+                    for (int k = c + 2; k < 16; k++) a[k] = fma(-a[c], lk[k], a[k]);
+                }
             }
             if (act) {
 #pragma unroll
