@@ -69,11 +69,19 @@ def main():
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
+    ap.add_argument("--serve", action="store_true", help="measure the serving loop on the mixed-size stream")
+    ap.add_argument("--serve-n", type=int, default=300, help="requests replayed from the serving trace")
+    ap.add_argument("--rate", type=float, default=2000.0, help="arrival rate of the serving trace (req/s)")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--max-batch", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (instances are independent; a multi-GPU "
               "run would shard the batch by HBM footprint). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        serve_main(args)
+        return
 
     import curegot_backward_amd as cg
     from curegot_backward_amd import _lib
@@ -180,6 +188,92 @@ def main():
             f"p99 {np.percentile(lat, 99):.1f} ms")
 
     result["cpu_baseline"] = cpu_baseline(probs, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+
+
+SERVE_METRIC = "served SPLR-OT solves/s + p50/p99 latency vs sequential, mixed-size Poisson stream, FP64 tol 1e-8"
+
+
+def serve_main(args):
+    """Replay the head of the serving trace (Poisson arrivals at args.rate
+    req/s, n, m log-uniform in [64, 8192], d in {4,16,64}, Dirichlet(0.5..2)
+    marginals, eta in {0.01,0.05,0.1}, 30% with an upstream gradient G) through
+    the continuous-batching server, then solve the same requests one at a time."""
+    import curegot_backward_amd as cg
+    from curegot_backward_amd.serving import OTServer
+    from curegot_backward_amd.workloads import serving_problem, serving_trace
+
+    dev = torch.device("cuda", 0)
+    trace = serving_trace(rate=args.rate, seconds=1.5 * args.serve_n / args.rate, seed=args.seed)[: args.serve_n]
+    log(f"serving sample: {len(trace)} requests over {trace[-1]['t']:.3f} s of arrivals")
+    reqs = []
+    g = torch.Generator().manual_seed(args.seed + 7)
+    for r in trace:
+        M, a, b, eta = serving_problem(r, device=dev)
+        G = torch.randn(M.shape, dtype=torch.float64, generator=g).to(dev) if r["backward"] else None
+        reqs.append((M, a, b, eta, G))
+    torch.cuda.synchronize()
+    cells = sum(q[0].numel() for q in reqs)
+    log(f"cells {cells / 1e6:.1f}M, {sum(1 for q in reqs if q[4] is not None)} with backward")
+
+    with OTServer(slots=args.slots, max_batch=args.max_batch, tol=1e-8) as srv:
+        for q in reqs[: min(4, len(reqs))]:           # warm-up (allocator, kernels)
+            srv.submit(*q).result()
+    with OTServer(slots=args.slots, max_batch=args.max_batch, tol=1e-8) as srv:
+        t0 = time.perf_counter()
+        futs = []
+        for r, q in zip(trace, reqs):
+            dt = r["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(srv.submit(*q))
+        outs = [f.result() for f in futs]
+        el = time.perf_counter() - t0
+        st = srv.stats()
+        for (k, c, nb, a, b) in srv.batch_log():
+            log(f"  batch slot {k} class {c} requests {nb:3d}  {1000 * a:8.1f} -> {1000 * b:8.1f} ms")
+    conv = sum(1 for o in outs if o["converged"])
+    log(f"served: {len(outs)} in {el:.3f} s, batches {st['batches']}, p50 {st['p50_ms']:.1f} ms, "
+        f"p99 {st['p99_ms']:.1f} ms, converged {conv}")
+
+    lat = []
+    t1 = time.perf_counter()
+    for (M, a, b, eta, G) in reqs:
+        ts = time.perf_counter()
+        o = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 1000, allow_invalid=True)
+        if G is not None and o["context"] is not None:
+            cg.solve_backward(o["plan"], G, o["context"], eta)
+        torch.cuda.synchronize()
+        lat.append(1000 * (time.perf_counter() - ts))
+    seq_el = time.perf_counter() - t1
+    order = np.argsort(lat)[::-1][:8]
+    log("slowest solo: " + ", ".join(f"{tuple(reqs[i][0].shape)} {lat[i]:.1f} ms" for i in order))
+    value = len(outs) / el
+    seq = len(reqs) / seq_el
+    log(f"sequential: {seq:.2f} solves/s")
+    result = {
+        "metric": SERVE_METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"head of the serving trace: {len(reqs)} requests, Poisson {args.rate:g} req/s, "
+                        "n,m log-uniform in [64,8192], d in {4,16,64}, Dirichlet(0.5..2), eta in "
+                        "{0.01,0.05,0.1}, 30% forward+backward (G ~ N(0,1)), tol 1e-8",
+            "requests": len(reqs), "cells": cells, "slots": args.slots, "max_batch": args.max_batch,
+            "seed": args.seed, "converged": conv,
+        },
+        "served_latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "max": st["max_ms"],
+                              "queue_p50": st["queue_p50_ms"], "queue_p99": st["queue_p99_ms"]},
+        "batches": st["batches"],
+        "sequential": {"solves_per_s": seq, "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99)), "total_s": seq_el},
+        "speedup_vs_sequential": value / seq,
+    }
     print(json.dumps(result), flush=True)
 
 

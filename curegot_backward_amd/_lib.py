@@ -136,8 +136,9 @@ PROFILE_CLASSES = {"tpass": 0, "row_lse": 1, "col_lse": 2, "qv": 3, "chol_update
 
 
 def set_max_groups(groups: int):
-    """Bound the number of concurrent size classes a batched SPLR solve is split
-    into (1..4; 1 = one lockstep group on the caller's stream)."""
+    """Bound the number of concurrent size classes a batched SPLR solve issued
+    from the calling thread is split into (1..4; 1 = one lockstep group on the
+    caller's stream)."""
     check(load().cgot_set_max_groups(int(groups)), "set_max_groups")
 
 

@@ -1108,7 +1108,10 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         }
         chunk = std::min(chunk * 2, 32);
     }
-    hipLaunchKernelGGL(k_grad_assemble, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc,
+    // a fresh copy of the all-active control: the slot of dc may have been
+    // reused by the PCG loop's uploads (the ring holds kCtlSlots blocks)
+    const double* dfin = bt.ctl(c);
+    hipLaunchKernelGGL(k_grad_assemble, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dfin,
                        lams, Gs, outs);
     bt.check_launch();
     bt.sync();
@@ -1249,7 +1252,7 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
 // grouping.
 // ---------------------------------------------------------------------------
 constexpr int kMaxGroups = 4;
-int g_max_groups = kMaxGroups;   // cgot_set_max_groups
+thread_local int g_max_groups = kMaxGroups;   // cgot_set_max_groups (per calling thread)
 
 struct GroupPlan {
     std::vector<std::vector<int>> idx;                 // instance indices per group

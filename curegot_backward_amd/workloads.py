@@ -31,6 +31,16 @@ def sqdist(X, Y, mean=False):
     return D.mean(-1) if mean else D.sum(-1)
 
 
+def sqdist_rows(X, Y, max_elems=1 << 26):
+    """sqdist(X, Y) formed a block of rows at a time (bounded temporary)."""
+    n, m, d = X.shape[0], Y.shape[0], X.shape[1]
+    step = max(1, max_elems // max(1, m * d))
+    out = torch.empty(n, m, dtype=X.dtype, device=X.device)
+    for i in range(0, n, step):
+        out[i:i + step] = sqdist(X[i:i + step], Y)
+    return out
+
+
 def readme_problem(seed=0, n=64, m=80, d=8):
     g = _gen(seed)
     X = torch.rand(n, d, generator=g, dtype=f64)
@@ -130,13 +140,20 @@ def serving_trace(rate=2000.0, seconds=60.0, seed=0, lo=64, hi=8192, backward_fr
     return reqs
 
 
-def serving_problem(req):
+def serving_problem(req, device=None):
+    """(M, a, b, eta) of one serving request; M is formed on `device` (CPU by
+    default) from the CPU-drawn points."""
     g = _gen(req["seed"])
     n, m, d = req["n"], req["m"], req["d"]
     X = torch.randn(n, d, generator=g, dtype=f64)
     Y = torch.randn(m, d, generator=g, dtype=f64)
-    M = sqdist(X, Y)
+    if device is not None:
+        X, Y = X.to(device), Y.to(device)
+    M = sqdist_rows(X, Y)
     M = M / M.max()
     a = dirichlet(n, req["alpha"], g).clamp_min(1e-12)
     b = dirichlet(m, req["alpha"], g).clamp_min(1e-12)
-    return M, a / a.sum(), b / b.sum(), req["eta"]
+    a, b = a / a.sum(), b / b.sum()
+    if device is not None:
+        a, b = a.to(device), b.to(device)
+    return M, a, b, req["eta"]

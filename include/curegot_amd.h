@@ -77,8 +77,8 @@ size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
  * classes that run concurrently (each on its own internal HIP stream and host
  * thread, ordered after `stream` on entry and before its later work on exit).
  * Results are per instance and independent of the split.  cgot_set_max_groups
- * (1..4, default 4) bounds the number of classes; 1 runs the whole batch in one
- * lockstep group on `stream`.  No counterpart in the reference, whose
+ * (1..4, default 4) bounds the number of classes for the solves issued from the
+ * calling thread; 1 runs the whole batch in one lockstep group on `stream`.  No counterpart in the reference, whose
  * splr_ot_batch loops over instances (curegot/autograd.py:340-367). */
 int cgot_set_max_groups(int32_t groups);
 int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,

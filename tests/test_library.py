@@ -40,3 +40,14 @@ def test_workspace_size_is_host_only():
     probs[0].n = 0
     assert L.cgot_splr_workspace_size(len(sizes), probs) == 0
     assert b"n and m" in L.cgot_last_error()
+
+
+def test_serving_footprint_host_only():
+    """The serving admission estimate uses the host-only workspace query."""
+    from curegot_backward_amd.serving import splr_footprint
+    small = splr_footprint(100, 120)
+    big = splr_footprint(1000, 1200)
+    assert 0 < small < big
+    assert splr_footprint(100, 120, backward=True) > small
+    # dense Schur block dominates at scale: at least P^2 doubles
+    assert big >= 8 * 999 * 999
