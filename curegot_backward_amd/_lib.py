@@ -34,6 +34,7 @@ class SplrInfo(ctypes.Structure):
         ("ys", ctypes.c_double), ("f", ctypes.c_double), ("gnorm", ctypes.c_double),
         ("K", ctypes.c_int64),
         ("inv_diag", _dp), ("s", _dp), ("y", _dp), ("u", _dp), ("gamma", _dp),
+        ("factor", _dp),
     ]
 
 
@@ -45,6 +46,7 @@ class AdjointProblem(ctypes.Structure):
         ("inv_diag", _dp),
         ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
         ("lam", _dp), ("grad", _dp),
+        ("factor", _dp),
     ]
 
 
@@ -66,6 +68,7 @@ class BcdProblem(ctypes.Structure):
 EXPORTS = (
     "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
+    "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply",
     "cgot_bcd_workspace_size", "cgot_bcd_solve",
     "cgot_lse_workspace_size", "cgot_lse_update",
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
@@ -98,6 +101,10 @@ def load():
     L.cgot_adjoint_solve.argtypes = [i32, ctypes.POINTER(AdjointProblem), vp, sz, vp,
                                      ctypes.POINTER(AdjointInfo)]
     L.cgot_adjoint_solve.restype = ctypes.c_int
+    L.cgot_splr_precond_workspace_size.argtypes = [vp, i32, i32]
+    L.cgot_splr_precond_workspace_size.restype = sz
+    L.cgot_splr_precond_apply.argtypes = [vp, i32, i32, vp, vp, vp, sz, vp]
+    L.cgot_splr_precond_apply.restype = ctypes.c_int
     L.cgot_bcd_workspace_size.argtypes = [i32, ctypes.POINTER(BcdProblem)]
     L.cgot_bcd_workspace_size.restype = sz
     L.cgot_bcd_solve.argtypes = [i32, ctypes.POINTER(BcdProblem), vp, sz, vp, ctypes.POINTER(i32)]

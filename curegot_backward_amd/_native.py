@@ -71,6 +71,7 @@ class SPLRBackwardContext:
         self._s = self._view(info.s, N)
         self._y = self._view(info.y, N)
         self._u = self._view(info.u, N)
+        self._factor = int(info.factor) if info.factor else None
 
     def _view(self, ptr, count) -> torch.Tensor:
         off = int(ptr) - self._ws.data_ptr()
@@ -142,6 +143,34 @@ class SPLRBackwardContext:
 
     def inv_diag_ptr(self) -> int:
         return self._inv_diag.data_ptr()
+
+    @property
+    def has_factor(self) -> bool:
+        """True when the forward's last Newton factor is stored (niter > 0)."""
+        return self._factor is not None
+
+    def factor_ptr(self) -> int:
+        if self._factor is None:
+            raise RuntimeError("SPLRBackwardContext: no stored factor (the forward took no step)")
+        return self._factor
+
+    def apply_preconditioner(self, r: torch.Tensor) -> torch.Tensor:
+        """z = B^{-1} r with B = A_k + the rank-2 BFGS term of the last forward
+        iteration, A_k^{-1} by the stored Schur factor (reference
+        backward_context.cu:543-581)."""
+        self._check_r(r)
+        f = self.factor_ptr()
+        rr = r.to(torch.float64).contiguous()
+        z = torch.empty_like(rr)
+        L = _lib.load()
+        dev = self._M.device
+        with torch.cuda.device(dev):
+            nbytes = L.cgot_splr_precond_workspace_size(f, self._n, self._m)
+            ws = _workspace(nbytes, dev)
+            rc = L.cgot_splr_precond_apply(f, self._n, self._m, rr.data_ptr(), z.data_ptr(), ws.data_ptr(),
+                                           nbytes, _stream(dev))
+            _lib.check(rc, "apply_preconditioner")
+        return z
 
 
 def _kwargs_density(kwargs, n, m):
@@ -322,13 +351,15 @@ def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
 
 
 def adjoint_batch(items, tol, max_iter):
-    """Batched implicit backward.  items: list of (M, dual, G, reg, inv_diag or None).
-    Returns (lambdas, grads, infos)."""
+    """Batched implicit backward.  items: list of (M, dual, G, reg, inv_diag or
+    None[, factor pointer or None]).  Returns (lambdas, grads, infos)."""
     B = len(items)
     device = items[0][0].device
     probs = (_lib.AdjointProblem * B)()
     lams, grads, keep = [], [], []
-    for i, (M, dual, G, reg, inv) in enumerate(items):
+    for i, item in enumerate(items):
+        M, dual, G, reg, inv = item[:5]
+        factor = item[5] if len(item) > 5 else None
         n, m = M.shape
         Gd = G.to(device=device, dtype=torch.float64).contiguous()
         lam = torch.empty(n + m - 1, dtype=torch.float64, device=device)
@@ -337,6 +368,7 @@ def adjoint_batch(items, tol, max_iter):
         p.M, p.dual, p.G = M.data_ptr(), dual.data_ptr(), Gd.data_ptr()
         p.n, p.m, p.reg = n, m, float(reg)
         p.inv_diag = inv.data_ptr() if inv is not None else None
+        p.factor = factor
         p.tol, p.max_iter = float(tol), int(max_iter)
         p.lam, p.grad = lam.data_ptr(), grad.data_ptr()
         keep.append(Gd)

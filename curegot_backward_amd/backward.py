@@ -13,9 +13,9 @@ iteration, T* recomputed from the context's duals; no stored plan) and the
 batch variant solves many instances in lockstep.
 
 Preconditioners: "diag" (default; 1/diag(A_k) of the context matrix, the
-reference's native diagonal) and "none" (plain CG).  "splr" (A_k^{-1} with the
-rank-2 Woodbury term) is accepted by the reference; here it raises
-NotImplementedError.
+reference's native diagonal), "splr" (B_k^{-1} = (A_k + rank-2 BFGS term)^{-1}
+with the forward's stored Schur Cholesky factor of A_k, applied on the device
+inside the PCG loop) and "none" (plain CG).
 """
 from __future__ import annotations
 
@@ -29,18 +29,19 @@ _VALID_PRECONDITIONERS = ("diag", "splr", "none")
 
 
 def select_preconditioner(ctx, preconditioner):
-    """Return the inverse-diagonal tensor for "diag", None for "none".
-    Reference: select_preconditioner, curegot/backward.py:46-69."""
+    """Return (inverse-diagonal tensor or None, factor pointer or None) for the
+    native PCG.  Reference: select_preconditioner, curegot/backward.py:46-69."""
     if preconditioner == "diag":
         if not bool(ctx.has_diagonal_preconditioner):
             raise RuntimeError("SPLRBackwardContext: diagonal preconditioner not set up")
-        return ctx._inv_diag
+        return ctx._inv_diag, None
     if preconditioner == "none":
-        return None
+        return None, None
     if preconditioner == "splr":
-        raise NotImplementedError(
-            "preconditioner='splr' (factorised A_k + rank-2 Woodbury) is not available in the "
-            "MI355X build; use 'diag' (default) or 'none'")
+        if not ctx.has_factor:
+            raise RuntimeError("SPLRBackwardContext: preconditioner 'splr' needs the forward's stored factor "
+                               "(the forward took no step); use 'diag'")
+        return None, ctx.factor_ptr()
     raise ValueError(
         f"invalid preconditioner '{preconditioner}'; "
         f"expected one of: {' | '.join(_VALID_PRECONDITIONERS)}"
@@ -63,8 +64,8 @@ def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=20
     for G, ctx, eta in items:
         if G.shape != ctx.M.shape:
             raise ValueError("G must have the same shape as T")
-        inv = select_preconditioner(ctx, preconditioner)
-        jobs.append((ctx.M, ctx.dual, G, float(eta), inv))
+        inv, factor = select_preconditioner(ctx, preconditioner)
+        jobs.append((ctx.M, ctx.dual, G, float(eta), inv, factor))
     lams, grads, infos = _native.adjoint_batch(jobs, tol, max_iter)
     out = []
     for lam, grad, info, (G, ctx, eta) in zip(lams, grads, infos, items):

@@ -89,6 +89,11 @@
     Q double* rhs; /* [2*P] Schur right-hand sides / solutions */                                \
     Q double* ltmp; /* [2*max(n,m)] Schur solutions */                                           \
     Q double* inv_diag; /* [N] 1/diag(A) of the context matrix */                                \
+    /* backward with the SPLR preconditioner: device address of the forward */ \
+    /* instance's descriptor (0 = none), Schur right-hand side and solution */ \
+    unsigned long long fctx;                                                                     \
+    Q double* prhs; /* [P of the forward instance] */                                            \
+    Q double* pltmp; /* [P] */                                                                   \
     /* outputs */                                                                                \
     Q double* plan; /* [n*m] or null */                                                          \
     Q double* dual; /* [n+m] or null */                                                         

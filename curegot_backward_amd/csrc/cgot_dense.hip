@@ -534,21 +534,31 @@ CG_DEV void cg_diag_block(const CgInstD& I, int kb, double (*L)[CG_NB + 1], doub
 
 // Bounded wait (thread 0) until flags[k] >= target, then a workgroup barrier
 // and an acquire fence so the data published before the flag is visible.  A
-// wait that exceeds the bound flags CS_PIVOT = 2 and returns.
-CG_DEV void cg_wait_flag(const CgInstD& I, int k, int target)
+// wait that exceeds the bound sets scal[CS_PIVOT] = 2 and returns.
+CG_DEV void cg_wait_flag_p(const CG_AS1 int* flags, CG_AS1 double* scal, int k, int target)
 {
     if (threadIdx.x == 0) {
         int spins = 0;
-        while (__hip_atomic_load(&I.flags[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(&flags[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > (1 << 25)) {
-                I.scal[CS_PIVOT] = 2.0;
+                scal[CS_PIVOT] = 2.0;
                 break;
             }
         }
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+CG_DEV void cg_wait_flag(const CgInstD& I, int k, int target) { cg_wait_flag_p(I.flags, I.scal, k, target); }
+
+// Publish block k: every thread's stores are made visible, then flags[k] = value.
+CG_DEV void cg_publish_flag(CG_AS1 int* flags, int k, int value)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[k], value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Block 0: factor/invert the first diagonal block (one workgroup per instance).
@@ -713,40 +723,42 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
             }
 }
 
-// Backward substitution L^T x = y as a dataflow over the block columns: one
-// workgroup per (instance, block kb).  Workgroups are numbered level by level,
-// level t = nblk-1-kb, with the instances of each level in the order `order`
-// (sorted by decreasing block count, so level t holds the first lvl_pref[t+1]
-// - lvl_pref[t] of them): every instance's chain starts at once.  Workgroup kb
-// accumulates L_{j,kb}^T x_j for j = nblk-1 .. kb+1 as each x_j is published
-// (flags[j] == epoch; the tile is loaded before the wait), then forms
-// x_kb = W_kb^T (y_kb - sum) and publishes it.  A workgroup waits only on
-// lower levels, i.e. on workgroups with smaller launch indices, which were
-// dispatched before it and themselves wait only on earlier ones, so every wait
-// ends; the spin is also bounded and flags CS_PIVOT = 2 instead of hanging.
-// y lives in rhs, x goes to ltmp.
-__global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ insts, const int* __restrict__ order,
-                                                     const int* __restrict__ lvl_pref, int nlvl, int epoch)
+// Operands of a blocked triangular solve with the stored factor: the panels
+// of L in S (row-major P x P), the inverses of its diagonal blocks in Winv,
+// right-hand sides y and solutions x (NR of each, stride P), the block flags
+// and the instance scalars (for the wait-bound flag).
+struct CgTriArgs {
+    const CG_AS1 double* S;
+    const CG_AS1 double* Winv;
+    int P;
+    const CG_AS1 double* y;
+    CG_AS1 double* x;
+    CG_AS1 int* flags;
+    CG_AS1 double* scal;
+};
+
+// Backward substitution L^T x = y, block kb of a dataflow launch: accumulate
+// L_{j,kb}^T x_j for j = nblk-1 .. kb+1 as each x_j is published (flags[j] >=
+// target; the tile is loaded before the wait), then x_kb = W_kb^T (y_kb - sum),
+// published with flags[kb] = target.
+template <int NR>
+CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb, int target)
 {
-    const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
-    const int b = order[blockIdx.x - lvl_pref[t]];
-    const CgInstD& I = cg_inst(insts, b);
-    const int P = I.P;
+    const int P = A.P;
     const int nblk = (P + CG_NB - 1) / CG_NB;
-    const int kb = nblk - 1 - t;
     const int c0 = kb * CG_NB;
     const int nb = min(CG_NB, P - c0);
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    const auto S = I.S;
     __shared__ double Ws[CG_NB][CG_NB + 1];
     __shared__ double xs[2][CG_NB];
     __shared__ double part[2][CG_WAVES][CG_NB];
-    const auto Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
-    cg_tile_to_lds(Ws, Wd, CG_NB, CG_NB, CG_NB, false);
-    // y_kb (thread < 128: right-hand side tid/64, entry tid%64)
-    const double yv = (tid < 2 * CG_NB && (tid % CG_NB) < nb) ? I.rhs[(size_t)(tid / CG_NB) * P + c0 + tid % CG_NB]
-                                                               : 0.0;
-    double v0 = 0.0, v1 = 0.0;
+    cg_tile_to_lds(Ws, A.Winv + (size_t)kb * CG_NB * CG_NB, CG_NB, CG_NB, CG_NB, false);
+    // y_kb (thread < NR*64: right-hand side tid/64, entry tid%64)
+    const double yv = (tid < NR * CG_NB && (tid % CG_NB) < nb) ? A.y[(size_t)(tid / CG_NB) * P + c0 + tid % CG_NB]
+                                                                : 0.0;
+    double v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) v[r] = 0.0;
     for (int j = nblk - 1; j > kb; j--) {
         // block kb is full width here (kb < nblk-1); tile rows g, g+4, ...
         const int r0 = j * CG_NB;
@@ -755,26 +767,26 @@ __global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ 
 #pragma unroll
         for (int q = 0; q < CG_NB / CG_WAVES; q++) {
             const int rr = g + CG_WAVES * q;
-            sv[q] = (rr < nbj) ? S[(size_t)(r0 + rr) * P + c0 + c] : 0.0;
+            sv[q] = (rr < nbj) ? A.S[(size_t)(r0 + rr) * P + c0 + c] : 0.0;
         }
-        cg_wait_flag(I, j, epoch);
-        if (tid < 2 * CG_NB) {
+        cg_wait_flag_p(A.flags, A.scal, j, target);
+        if (tid < NR * CG_NB) {
             const int r = tid / CG_NB, q = tid % CG_NB;
-            xs[r][q] = (q < nbj) ? I.ltmp[(size_t)r * P + r0 + q] : 0.0;
+            xs[r][q] = (q < nbj) ? A.x[(size_t)r * P + r0 + q] : 0.0;
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < CG_NB / CG_WAVES; q++) {
             const int rr = g + CG_WAVES * q;
-            v0 = fma(sv[q], xs[0][rr], v0);
-            v1 = fma(sv[q], xs[1][rr], v1);
+#pragma unroll
+            for (int r = 0; r < NR; r++) v[r] = fma(sv[q], xs[r][rr], v[r]);
         }
         __syncthreads();
     }
-    part[0][g][c] = v0;
-    part[1][g][c] = v1;
+#pragma unroll
+    for (int r = 0; r < NR; r++) part[r][g][c] = v[r];
     __syncthreads();
-    if (tid < 2 * CG_NB) {
+    if (tid < NR * CG_NB) {
         const int r = tid / CG_NB, q = tid % CG_NB;
         const double acc = (part[r][0][q] + part[r][1][q]) + (part[r][2][q] + part[r][3][q]);
         xs[r][q] = (q < nb) ? yv - acc : 0.0;
@@ -782,25 +794,202 @@ __global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ 
     __syncthreads();
     {
         // x[q] = sum_{cc >= q} W[cc][q] r[cc]: thread (q, quarter g) sums cc in
-        // [16g, 16g + 16), both right-hand sides; partials reduced through LDS
-        double x0 = 0.0, x1 = 0.0;
+        // [16g, 16g + 16); partials reduced through LDS
+        double xq[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) xq[r] = 0.0;
 #pragma unroll
         for (int u = 0; u < 16; u++) {
             const int cc = 16 * g + u;
             const double w = (cc >= c) ? Ws[cc][c] : 0.0;
-            x0 = fma(w, xs[0][cc], x0);
-            x1 = fma(w, xs[1][cc], x1);
+#pragma unroll
+            for (int r = 0; r < NR; r++) xq[r] = fma(w, xs[r][cc], xq[r]);
         }
-        part[0][g][c] = x0;
-        part[1][g][c] = x1;
+#pragma unroll
+        for (int r = 0; r < NR; r++) part[r][g][c] = xq[r];
     }
     __syncthreads();
-    if (tid < 2 * CG_NB) {
+    if (tid < NR * CG_NB) {
         const int r = tid / CG_NB, q = tid % CG_NB;
         const double x = (part[r][0][q] + part[r][1][q]) + (part[r][2][q] + part[r][3][q]);
-        if (q < nb) I.ltmp[(size_t)r * P + c0 + q] = x;
+        if (q < nb) A.x[(size_t)r * P + c0 + q] = x;
     }
-    __threadfence();
+    cg_publish_flag(A.flags, kb, target);
+}
+
+// Backward substitution L^T x = y of the Newton solve as a dataflow over the
+// block columns: one workgroup per (instance, block kb).  Workgroups are
+// numbered level by level, level t = nblk-1-kb, with the instances of each
+// level in the order `order` (sorted by decreasing block count, so level t
+// holds the first lvl_pref[t+1] - lvl_pref[t] of them): every instance's chain
+// starts at once.  A workgroup waits only on lower levels, i.e. on workgroups
+// with smaller launch indices, which were dispatched before it and themselves
+// wait only on earlier ones, so every wait ends; the spin is also bounded and
+// flags CS_PIVOT = 2 instead of hanging.  y lives in rhs, x goes to ltmp.
+__global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ insts, const int* __restrict__ order,
+                                                     const int* __restrict__ lvl_pref, int nlvl, int epoch)
+{
+    const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
+    const int b = order[blockIdx.x - lvl_pref[t]];
+    const CgInstD& I = cg_inst(insts, b);
+    const int nblk = (I.P + CG_NB - 1) / CG_NB;
+    const CgTriArgs A{I.S, I.Winv, I.P, I.rhs, I.ltmp, I.flags, I.scal};
+    cg_bwd_block<2>(A, nblk - 1 - t, epoch);
+}
+
+// ---------------------------------------------------------------------------
+// SPLR preconditioner of the implicit backward: q = A^{-1} r with the forward
+// instance's stored Schur factor (A = the sparsified Hessian of the last
+// forward iteration; its W, D and factor are left in the forward workspace).
+// An adjoint instance with fctx != 0 points at that forward descriptor; these
+// kernels act only while its PCG is running.  Reference: the cuDSS solve with
+// the adopted factor in SPLRBackwardContext::apply_preconditioner,
+// src/backward_context.cu:543-581.
+// ---------------------------------------------------------------------------
+CG_DEV const CgInstD& cg_fwd_of(const CgInstD& I) { return *reinterpret_cast<const CgInstD*>(I.fctx); }
+
+CG_DEV bool cg_pc_live(const CgInstD& I) { return I.fctx != 0 && I.scal[PCG_STATUS] == PCG_RUNNING; }
+
+// Schur right-hand side of r (the PCG residual, N layout), wave per small index.
+__global__ __launch_bounds__(CG_WG) void k_pc_rhs(const CgInst* __restrict__ insts,
+                                                  const int* __restrict__ wave_inst,
+                                                  const double* __restrict__ ctl)
+{
+    int b, s;
+    if (!cg_decode_wave(insts, wave_inst, ctl, b, s)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    if (!cg_pc_live(I)) return;
+    const CgInstD& F = cg_fwd_of(I);
+    const int P = F.P, n = F.n;
+    if (s >= P) return;
+    const int lane = threadIdx.x & 63;
+    const auto r = I.yv;
+    double acc = 0.0;
+    if (F.beta_side) {
+        for (int p = F.w_colptr[s] + lane; p < F.w_colptr[s + 1]; p += 64) {
+            const int i = F.w_rowi[p];
+            acc += F.w_cval[p] * (r[i] / F.Da[i]);
+        }
+    } else {
+        for (int p = F.w_rowptr[s] + lane; p < F.w_rowptr[s + 1]; p += 64) {
+            const int j = F.w_colj[p];
+            acc += F.w_rval[p] * (r[n + j] / F.Db[j]);
+        }
+    }
+    acc = cg_wave_sum(acc);
+    if (lane == 0) I.prhs[s] = r[F.beta_side ? n + s : s] - acc;
+}
+
+// Forward substitution L y = rhs, block kb of a dataflow launch (y replaces rhs
+// block by block): accumulate L_{kb,j} y_j for j = 0 .. kb-1 as each y_j is
+// published (flags[j] >= target), then y_kb = W_kb (rhs_kb - sum).
+CG_DEV void cg_fwd_block(const CgTriArgs& A, CG_AS1 double* y, int kb, int target)
+{
+    const int P = A.P;
+    const int c0 = kb * CG_NB;
+    const int nb = min(CG_NB, P - c0);
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6, lane = tid & 63;
+    __shared__ double Ws[CG_NB][CG_NB + 1];
+    __shared__ double xs[CG_NB];
+    __shared__ double red[CG_NB];
+    __shared__ double part[CG_WAVES][CG_NB];
+    cg_tile_to_lds(Ws, A.Winv + (size_t)kb * CG_NB * CG_NB, CG_NB, CG_NB, CG_NB, false);
+    const double rv = (tid < nb) ? y[c0 + tid] : 0.0;
+    // thread (column c, row group g) accumulates rows g, g+4, ... over column c
+    double acc[CG_NB / CG_WAVES];
+#pragma unroll
+    for (int q = 0; q < CG_NB / CG_WAVES; q++) acc[q] = 0.0;
+    for (int j = 0; j < kb; j++) {
+        double sv[CG_NB / CG_WAVES];
+#pragma unroll
+        for (int q = 0; q < CG_NB / CG_WAVES; q++) {
+            const int rr = g + CG_WAVES * q;
+            sv[q] = (rr < nb) ? A.S[(size_t)(c0 + rr) * P + j * CG_NB + c] : 0.0;   // block j is full
+        }
+        cg_wait_flag_p(A.flags, A.scal, j, target);
+        if (tid < CG_NB) xs[tid] = y[j * CG_NB + tid];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CG_NB / CG_WAVES; q++) acc[q] = fma(sv[q], xs[c], acc[q]);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < CG_NB / CG_WAVES; q++) {
+        const double t = cg_wave_sum(acc[q]);
+        if (lane == 0) red[g + CG_WAVES * q] = t;
+    }
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&I.flags[kb], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < CG_NB) xs[tid] = (tid < nb) ? rv - red[tid] : 0.0;
+    __syncthreads();
+    {
+        // y[q] = sum_{cc <= q} W[q][cc] r[cc]: thread (q, quarter g) sums cc in [16g, 16g + 16)
+        double yq = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int cc = 16 * g + u;
+            yq = fma((cc <= c) ? Ws[c][cc] : 0.0, xs[cc], yq);
+        }
+        part[g][c] = yq;
+    }
+    __syncthreads();
+    if (tid < nb) y[c0 + tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+    cg_publish_flag(A.flags, kb, target);
+}
+
+// Dataflow launches of the two triangular solves (level tables as for
+// k_trsv_flow; level t is block t forward, block nblk-1-t backward).
+__global__ __launch_bounds__(CG_WG) void k_pc_fwd(const CgInst* __restrict__ insts, const int* __restrict__ order,
+                                                  const int* __restrict__ lvl_pref, int nlvl, int target)
+{
+    const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
+    const CgInstD& I = cg_inst(insts, order[blockIdx.x - lvl_pref[t]]);
+    if (!cg_pc_live(I)) return;
+    const CgInstD& F = cg_fwd_of(I);
+    const CgTriArgs A{F.S, F.Winv, F.P, I.prhs, I.pltmp, I.flags, I.scal};
+    cg_fwd_block(A, I.prhs, t, target);
+}
+
+__global__ __launch_bounds__(CG_WG) void k_pc_bwd(const CgInst* __restrict__ insts, const int* __restrict__ order,
+                                                  const int* __restrict__ lvl_pref, int nlvl, int target)
+{
+    const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
+    const CgInstD& I = cg_inst(insts, order[blockIdx.x - lvl_pref[t]]);
+    if (!cg_pc_live(I)) return;
+    const CgInstD& F = cg_fwd_of(I);
+    const int nblk = (F.P + CG_NB - 1) / CG_NB;
+    const CgTriArgs A{F.S, F.Winv, F.P, I.prhs, I.pltmp, I.flags, I.scal};
+    cg_bwd_block<1>(A, nblk - 1 - t, target);
+}
+
+// q = A^{-1} r in the N layout into z (sv): the small side is the Schur
+// solution, the large side is back-substituted (wave per entry).
+__global__ __launch_bounds__(CG_WG) void k_pc_back(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ wave_inst,
+                                                   const double* __restrict__ ctl)
+{
+    int b, q;
+    if (!cg_decode_wave(insts, wave_inst, ctl, b, q)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    if (!cg_pc_live(I)) return;
+    const CgInstD& F = cg_fwd_of(I);
+    const int n = F.n;
+    if (q >= F.N) return;
+    const int lane = threadIdx.x & 63;
+    const auto x = I.pltmp;
+    const bool large = F.beta_side ? (q < n) : (q >= n);
+    if (!large) {
+        if (lane == 0) I.sv[q] = x[F.beta_side ? q - n : q];
+        return;
+    }
+    double acc = 0.0, d;
+    if (F.beta_side) {
+        for (int p = F.w_rowptr[q] + lane; p < F.w_rowptr[q + 1]; p += 64) acc += F.w_rval[p] * x[F.w_colj[p]];
+        d = F.Da[q];
+    } else {
+        const int j = q - n;
+        for (int p = F.w_colptr[j] + lane; p < F.w_colptr[j + 1]; p += 64) acc += F.w_cval[p] * x[F.w_rowi[p]];
+        d = F.Db[j];
+    }
+    acc = cg_wave_sum(acc);
+    if (lane == 0) I.sv[q] = (I.yv[q] - acc) / d;
 }

@@ -1007,6 +1007,7 @@ struct SplrSolver {
             o.y = I.yv;
             o.u = I.uv;
             o.gamma = I.gamma;
+            o.factor = (H.niter > 0) ? (const void*)(bt.d_insts + b) : nullptr;
         }
     }
 };
@@ -1028,6 +1029,7 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
 {
     const int B = bt.B;
     bt.h.assign(B, CgInst{});
+    int nblk_max = 0;
     for (int b = 0; b < B; b++) {
         const auto& p = probs[b];
         CgInst& I = bt.h[b];
@@ -1036,16 +1038,33 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
         I.gamma = const_cast<double*>(p.dual);
         I.dir = p.lambda;
         I.Da = const_cast<double*>(p.inv_diag);
+        I.fctx = (unsigned long long)(uintptr_t)p.factor;
+        I.P = 0;
+        if (p.factor) {
+            // the forward descriptor: Schur size and orientation for the launch geometry
+            CgInst F;
+            CG_HIP(hipMemcpy(&F, p.factor, sizeof(CgInst), hipMemcpyDeviceToHost));
+            if (F.n != p.n || F.m != p.m) throw std::invalid_argument("factor does not match the problem shape");
+            I.P = F.P;
+            I.beta_side = F.beta_side;
+            nblk_max = std::max(nblk_max, cdiv(F.P, CG_NB));
+        }
     }
     bt.assign_maps();
     Bump bp(base);
-    bt.layout_common(bp, 0);
-    bt.zero_base = bp.take<unsigned int>((size_t)B * 4);
-    bt.zero_bytes = (size_t)B * 4 * sizeof(unsigned int);
+    bt.layout_common(bp, nblk_max);
+    const size_t zstride = 4 + (size_t)nblk_max;
+    bt.zero_base = bp.take<unsigned int>((size_t)B * zstride);
+    bt.zero_bytes = (size_t)B * zstride * sizeof(unsigned int);
     for (int b = 0; b < B; b++) {
         CgInst& I = bt.h[b];
         const int n = I.n, m = I.m, N = I.N;
-        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * 4 : nullptr;
+        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * zstride : nullptr;
+        I.flags = bt.zero_base ? (int*)(I.counter + 4) : nullptr;
+        if (I.fctx) {
+            I.prhs = bp.take<double>(I.P);
+            I.pltmp = bp.take<double>(I.P);
+        }
         I.yv = bp.take<double>(N);
         I.sv = bp.take<double>(N);
         I.uv = bp.take<double>(N);
@@ -1060,6 +1079,57 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
     }
     return bp.off + 256;
 }
+
+// SPLR preconditioner z = B^{-1} r for the adjoint instances with a forward
+// factor (fctx != 0): level tables of their triangular solves (instances by
+// decreasing block count; level t holds those with more than t blocks) and
+// the launch sequence of one application.
+struct PcSolver {
+    Batch& bt;
+    std::vector<int> ord, lvl;
+    int nlvl = 0;
+    int* d_ord = nullptr;
+    int* d_lvl = nullptr;
+    int epoch = 0;
+
+    explicit PcSolver(Batch& b) : bt(b)
+    {
+        for (int i = 0; i < bt.B; i++)
+            if (bt.h[i].fctx) ord.push_back(i);
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return bt.h[x].P > bt.h[y].P; });
+        for (int i : ord) nlvl = std::max(nlvl, cdiv(bt.h[i].P, CG_NB));
+        lvl.assign(nlvl + 1, 0);
+        for (int t = 0; t < nlvl; t++) {
+            int cnt = 0;
+            while (cnt < (int)ord.size() && cdiv(bt.h[ord[cnt]].P, CG_NB) > t) cnt++;
+            lvl[t + 1] = lvl[t] + cnt;
+        }
+        d_ord = bt.d_pref;
+        d_lvl = bt.d_pref + bt.B;
+        if (!ord.empty()) {
+            CG_HIP(hipMemcpyAsync(d_ord, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, bt.stream));
+            CG_HIP(hipMemcpyAsync(d_lvl, lvl.data(), sizeof(int) * lvl.size(), hipMemcpyHostToDevice, bt.stream));
+        }
+    }
+
+    // r = yv -> z = sv for the instances still running; then r.z and beta
+    // (init: beta = 0).  Flags of one application: forward blocks 2e-1,
+    // backward blocks 2e.
+    void apply(const double* ctlp, int init)
+    {
+        if (ord.empty()) return;
+        epoch++;
+        bt.rows0(k_pc_rhs, ctlp);
+        hipLaunchKernelGGL(k_pc_fwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
+                           2 * epoch - 1);
+        hipLaunchKernelGGL(k_pc_bwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
+                           2 * epoch);
+        bt.rows0(k_pc_back, ctlp);
+        bt.vec0(k_pc_wdots, ctlp);
+        bt.vec0(k_pc_wapply, ctlp);
+        hipLaunchKernelGGL(k_pcg_rz, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, ctlp, init);
+    }
+};
 
 void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info* info)
 {
@@ -1076,9 +1146,12 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
     const double* const* Gs = (const double* const*)bt.d_ptrs;
     const double* const* lams = (const double* const*)(bt.d_ptrs + B);
     double* const* outs = (double* const*)(bt.d_ptrs + 2 * B);
+    PcSolver pc(bt);
+    auto precondition = [&](const double* ctlp, int init) { pc.apply(ctlp, init); };
     hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc,
                        (const double* const*)nullptr, Gs, (int)CG_QV_RHS);
     bt.vec0(k_pcg_init, dc);
+    precondition(dc, 1);
     bt.check_launch();
     bt.read_scal();
     std::vector<char> running(B, 0);
@@ -1098,6 +1171,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
                                (const double* const*)nullptr, (const double* const*)nullptr, (int)CG_QV_PCG);
             bt.vec0(k_pcg_q_fin, d2);
             bt.vec0(k_pcg_update, d2);
+            precondition(d2, 0);
         }
         bt.check_launch();
         bt.read_scal();
@@ -1115,6 +1189,9 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
                        lams, Gs, outs);
     bt.check_launch();
     bt.sync();
+    for (int b = 0; b < B; b++)
+        if (bt.h[b].fctx && bt.sc(b, CS_PIVOT) >= 2.0)
+            throw std::runtime_error("internal: a block-flag wait of the preconditioner exceeded its bound");
     for (int b = 0; b < B; b++) {
         const double st = bt.sc(b, PCG_STATUS);
         info[b].status = (int)st;
@@ -1531,6 +1608,55 @@ int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* works
     }
     bt.upload_maps(ptrs);
     adjoint_run(bt, probs, info);
+    return 0;
+    CG_API_END
+}
+
+size_t cgot_splr_precond_workspace_size(const void* factor, int32_t n, int32_t m)
+{
+    try {
+        if (!factor) throw std::invalid_argument("factor must be non-null");
+        Batch bt;
+        bt.B = 1;
+        cgot_adjoint_problem p{};
+        p.n = n; p.m = m; p.reg = 1.0; p.factor = factor;
+        return adjoint_layout(bt, &p, nullptr);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                            void* workspace, size_t workspace_bytes, void* stream)
+{
+    CG_API_BEGIN
+    if (!factor || !r || !z) throw std::invalid_argument("factor, r and z must be non-null");
+    if (n < 1 || m < 2) throw std::invalid_argument("n >= 1 and m >= 2 required");
+    Batch bt;
+    bt.B = 1;
+    bt.stream = (hipStream_t)stream;
+    cgot_adjoint_problem p{};
+    p.n = n; p.m = m; p.reg = 1.0; p.factor = factor;
+    const size_t need = adjoint_layout(bt, &p, nullptr);
+    if (workspace == nullptr || workspace_bytes < need)
+        throw std::invalid_argument("workspace too small: need " + std::to_string(need) + " bytes");
+    adjoint_layout(bt, &p, workspace);
+    bt.upload_maps(std::vector<const double*>(3, nullptr));
+    CG_HIP(hipMemsetAsync(bt.zero_base, 0, bt.zero_bytes, bt.stream));
+    CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * CG_NSCAL, bt.stream));   // PCG status: running
+    const CgInst& I = bt.h[0];
+    const size_t N = (size_t)n + m - 1;
+    CG_HIP(hipMemcpyAsync(I.yv, r, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+    auto c = ctl_vec(1);
+    c[CTL_ACTIVE] = 1.0;
+    PcSolver pc(bt);
+    pc.apply(bt.ctl(c), 1);
+    CG_HIP(hipMemcpyAsync(z, I.sv, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+    bt.check_launch();
+    bt.read_scal();
+    if (bt.sc(0, CS_PIVOT) >= 2.0)
+        throw std::runtime_error("internal: a block-flag wait of the preconditioner exceeded its bound");
     return 0;
     CG_API_END
 }

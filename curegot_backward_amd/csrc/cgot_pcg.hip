@@ -72,7 +72,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ i
         I.yv[k] = mu;
         I.dir[k] = 0.0;
         I.uv[k] = 0.0;
-        const double z = (I.Da != nullptr) ? I.Da[k] * mu : mu;
+        const double z = (I.Da != nullptr) ? I.Da[k] * mu : mu;   // SPLR preconditioner: k_pc_* then k_pcg_rz
         I.sv[k] = z;
         v[0] += mu * mu;
         v[1] += mu * z;
@@ -155,10 +155,12 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_update(const CgInst* __restrict__
         I.dir[k] += alpha * I.uv[k];
         const double r = I.yv[k] - alpha * I.grad[k];
         I.yv[k] = r;
-        const double z = (I.Da != nullptr) ? I.Da[k] * r : r;
-        I.sv[k] = z;
         v[0] = r * r;
-        v[1] = r * z;
+        if (I.fctx == 0) {   // with the SPLR preconditioner z and r.z come later (k_pc_*, k_pcg_rz)
+            const double z = (I.Da != nullptr) ? I.Da[k] * r : r;
+            I.sv[k] = z;
+            v[1] = r * z;
+        }
     }
     double tot[2];
     if (cg_inst_reduce_p<2>(I, wgi, v, tot, 2) && threadIdx.x == 0) {
@@ -169,9 +171,89 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_update(const CgInst* __restrict__
         if (sqrt(tot[0]) < tol * I.scal[PCG_BNORM]) {
             I.scal[PCG_STATUS] = PCG_CONVERGED;
         } else {
-            I.scal[PCG_BETA] = tot[1] / I.scal[PCG_RZ];
-            I.scal[PCG_RZ] = tot[1];
+            if (I.fctx == 0) {
+                I.scal[PCG_BETA] = tot[1] / I.scal[PCG_RZ];
+                I.scal[PCG_RZ] = tot[1];
+            }
             if (it >= maxit) I.scal[PCG_STATUS] = PCG_MAXIT;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SPLR preconditioner, low-rank part and the PCG scalars that follow it.
+// With q = A^{-1} r in z (from k_pc_back), u = A^{-1} y and the secant pair
+// (s, y) of the forward's last iteration:
+//   B^{-1} r = q - (s^T r / ys) u + [(1/reg + y^T u / ys)(s^T r / ys) - y^T q / ys] s
+// Reference: SPLRBackwardContext::apply_preconditioner, src/backward_context.cu:543-581,
+// with launch_low_rank_search_direc, src/sinkhorn_splr_kernel_low_rank.cu:296-390.
+// ---------------------------------------------------------------------------
+CG_DEV const CgInstD& cg_pc_fwd(const CgInstD& I) { return *reinterpret_cast<const CgInstD*>(I.fctx); }
+
+CG_DEV bool cg_pc_decode(const CgInst* insts, const int* wg_inst, const double* ctl, int& b, int& wgi)
+{
+    if (!cg_pcg_decode(insts, wg_inst, ctl, b, wgi)) return false;
+    return insts[b].fctx != 0;
+}
+
+__global__ __launch_bounds__(CG_WG) void k_pc_wdots(const CgInst* __restrict__ insts,
+                                                    const int* __restrict__ wg_inst,
+                                                    const double* __restrict__ ctl)
+{
+    int b, wgi;
+    if (!cg_pc_decode(insts, wg_inst, ctl, b, wgi)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const CgInstD& F = cg_pc_fwd(I);
+    if (F.scal[CS_LOWRANK] == 0.0) return;   // uniform per instance
+    const int k = wgi * CG_WG + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (k < I.N) {
+        v[0] = F.sv[k] * I.yv[k];
+        v[1] = F.yv[k] * F.uv[k];
+        v[2] = F.yv[k] * I.sv[k];
+    }
+    double tot[3];
+    if (cg_inst_reduce_p<3>(I, wgi, v, tot, 3) && threadIdx.x == 0) {
+        I.scal[CS_SG] = tot[0];
+        I.scal[CS_YAY] = tot[1];
+        I.scal[CS_YAG] = tot[2];
+    }
+}
+
+__global__ __launch_bounds__(CG_WG) void k_pc_wapply(const CgInst* __restrict__ insts,
+                                                     const int* __restrict__ wg_inst,
+                                                     const double* __restrict__ ctl)
+{
+    int b, wgi;
+    if (!cg_pc_decode(insts, wg_inst, ctl, b, wgi)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const CgInstD& F = cg_pc_fwd(I);
+    if (F.scal[CS_LOWRANK] == 0.0) return;
+    const int k = wgi * CG_WG + threadIdx.x;
+    if (k >= I.N) return;
+    const double ys = F.scal[CS_YS];
+    const double srys = I.scal[CS_SG] / ys;
+    const double t1 = (1.0 / F.reg + I.scal[CS_YAY] / ys) * srys - I.scal[CS_YAG] / ys;
+    I.sv[k] += t1 * F.sv[k] - srys * F.uv[k];
+}
+
+// r.z after the SPLR preconditioner; init: rz, beta = 0; else beta = rz_new / rz.
+__global__ __launch_bounds__(CG_WG) void k_pcg_rz(const CgInst* __restrict__ insts,
+                                                  const int* __restrict__ wg_inst,
+                                                  const double* __restrict__ ctl, int init)
+{
+    int b, wgi;
+    if (!cg_pc_decode(insts, wg_inst, ctl, b, wgi)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const int k = wgi * CG_WG + threadIdx.x;
+    double v[1] = {(k < I.N) ? I.yv[k] * I.sv[k] : 0.0};
+    double tot[1];
+    if (cg_inst_reduce_p<1>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
+        if (init) {
+            I.scal[PCG_BETA] = 0.0;
+        } else {
+            I.scal[PCG_BETA] = tot[0] / I.scal[PCG_RZ];
+        }
+        I.scal[PCG_RZ] = tot[0];
     }
 }

@@ -70,6 +70,9 @@ typedef struct cgot_splr_info {
     const double* y;        /* [n+m-1] secant y of the last step */
     const double* u;        /* [n+m-1] A^{-1} y (valid if low_rank) */
     const double* gamma;    /* [n+m] final duals */
+    const void* factor;     /* the instance's stored Newton factor (A of the last iteration,
+                               its Schur Cholesky factor, s, y, u), for the SPLR preconditioner
+                               of cgot_adjoint_solve; NULL when niter == 0 */
 } cgot_splr_info;
 
 size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
@@ -98,6 +101,11 @@ typedef struct cgot_adjoint_problem {
     int32_t max_iter;
     double* lambda;         /* [n+m-1] output */
     double* grad;           /* [n*m] output dL/dM */
+    const void* factor;     /* cgot_splr_info.factor of the forward solve (its workspace alive):
+                               SPLR preconditioner B^{-1} r, B = A + rank-2 BFGS term
+                               (SPLRBackwardContext::apply_preconditioner,
+                               src/backward_context.cu:543-581); takes precedence over inv_diag;
+                               NULL for none */
 } cgot_adjoint_problem;
 
 typedef struct cgot_adjoint_info {
@@ -111,6 +119,15 @@ int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* works
                        size_t workspace_bytes, void* stream, cgot_adjoint_info* info);
 
 /* Sinkhorn/BCD solve (legacy sinkhorn_bcd).  dual is not gauge-fixed. */
+/* SPLR preconditioner of one converged forward solve applied to a vector:
+ * z = B^{-1} r with B = A + rank-2 BFGS term of the last iteration, A^{-1} by
+ * the stored Schur factor (factor = cgot_splr_info.factor, its workspace
+ * alive; r, z: [n+m-1] device vectors).  Replaces
+ * SPLRBackwardContext::apply_preconditioner, src/backward_context.cu:543-581. */
+size_t cgot_splr_precond_workspace_size(const void* factor, int32_t n, int32_t m);
+int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
 typedef struct cgot_bcd_problem {
     const double* M; const double* a; const double* b;
     int32_t n, m;
