@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "batched SPLR-OT solves/s + p50/p99 latency vs sequential, FP64 tol 1e-8"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
 
 
 def log(*a):
@@ -70,6 +71,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
     ap.add_argument("--serve", action="store_true", help="measure the serving loop on the mixed-size stream")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB of one device-to-device copy issued first (counter calibration under rocprofv3)")
     ap.add_argument("--serve-n", type=int, default=300, help="requests replayed from the serving trace")
     ap.add_argument("--rate", type=float, default=2000.0, help="arrival rate of the serving trace (req/s)")
     ap.add_argument("--slots", type=int, default=4)
@@ -88,6 +91,14 @@ def main():
     from curegot_backward_amd.workloads import ragged_batch
 
     dev = torch.device("cuda", 0)
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe * (1 << 17), dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     log("generating workload")
     probs = ragged_batch(B=args.batch, seed=args.seed)
     Ms = [p[0].to(dev) for p in probs]
@@ -132,8 +143,37 @@ def main():
     for k, (ms, work, n) in prof.items():
         if n:
             log(f"  {k:12s} launches {n:6d}  total {ms:9.2f} ms  avg {1000 * ms / n:8.2f} us  work {work:.3e}")
-    ms_t, bytes_t, n_t = prof["tpass"]
-    achieved = (bytes_t / 1e9) / (ms_t / 1e3) if ms_t > 0 else 0.0
+    def hbm_roof(name, kernel):
+        ms, byts, n = prof[name]
+        ach = (byts / 1e9) / (ms / 1e3) if ms > 0 else 0.0
+        return {"kernel": kernel, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": 1000 * ms / max(n, 1),
+                "launches": n, "bytes_per_launch": byts / max(n, 1)}
+
+    roof_tpass = hbm_roof("tpass", "k_tpass (T pass / line-search trial over M)")
+    roof_row = hbm_roof("row_lse", "k_row_lse (Sinkhorn candidate alpha half-sweep)")
+    roof_col = hbm_roof("col_lse", "k_col_lse (Sinkhorn candidate beta half-sweep)")
+    ms_u, fl_u, n_u = prof["chol_update"]
+    ach_u = (fl_u / 1e12) / (ms_u / 1e3) if ms_u > 0 else 0.0
+    roof_chol = {"kernel": "k_chol_update (trailing update, diagonal-block factor, next panel of the Schur Cholesky)",
+                 "bound": "mfma", "achieved": ach_u, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                 "frac": ach_u / FP64_MFMA_PEAK_TFS, "traffic": None, "avg_launch_us": 1000 * ms_u / max(n_u, 1),
+                 "launches": n_u, "flop_per_launch": fl_u / max(n_u, 1)}
+    # HBM bytes per launch measured with rocprofv3 counter passes over this
+    # workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size);
+    # null when the measurement file is absent
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic_splr_ragged_b128.json")) as f:
+            measured = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        measured = {}
+    for roof, kname in ((roof_tpass, "k_tpass"), (roof_chol, "k_chol_update"), (roof_row, "k_row_lse"),
+                        (roof_col, "k_col_lse")):
+        if kname in measured and args.batch == 128 and args.seed == 0:
+            roof["traffic"] = measured[kname]["traffic_bytes_per_launch"]
+            roof["traffic_source"] = "profiles/pmc_traffic_splr_ragged_b128.md"
+    # the dominant kernel by device time in the profiled solve
+    dominant = roof_chol if ms_u >= prof["tpass"][0] else roof_tpass
 
     value = len(Ms) * args.steps / el
     batch_ms = 1000 * el / args.steps
@@ -156,16 +196,11 @@ def main():
             "B": len(Ms), "tol": 1e-8, "seed": args.seed, "cells": cells, "groups": args.groups,
             "converged": conv, "niter_mean": float(np.mean(niters)), "niter_max": int(max(niters)),
         },
-        "roofline": {
-            "kernel": "k_tpass (T pass / line-search trial, M pass)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "avg_launch_us": 1000 * ms_t / max(n_t, 1),
-            "launches": n_t,
+        "roofline": dominant,
+        "roofline_hbm": {
+            "tpass": roof_tpass,
+            "row_lse": roof_row,
+            "col_lse": roof_col,
         },
         "batched_latency_ms": {"p50": batch_ms, "p99": batch_ms},
     }
