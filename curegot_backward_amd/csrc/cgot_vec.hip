@@ -29,16 +29,13 @@ CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) I.wgpart[(size_t)k * I.nvec + wgi] = v[k];
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // release of this workgroup's partials and, in the last workgroup,
+        // acquire of everyone's (one acq_rel read-modify-write by one lane)
+        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (prev == (unsigned int)(I.nvec - 1));
     }
     __syncthreads();
     if (!is_last) return false;
-    __threadfence();
     if (threadIdx.x < NV) {
         const int k = threadIdx.x;
         tot[k] = cg_ordered_sum(I.wgpart + (size_t)k * I.nvec, I.nvec, 1);
