@@ -18,6 +18,7 @@
 #define CG_RPW 8           // rows per wave in an M-pass tile
 #define CG_RB (CG_WAVES * CG_RPW)   // rows per M-pass tile
 #define CG_NB 64           // dense Cholesky block size
+#define CG_SCHUR_LDS 1024  // longest Schur row accumulated in LDS (longer rows update in place)
 
 #define CG_DEV __device__ __forceinline__
 #define CG_AS1 __attribute__((address_space(1)))   // global memory
@@ -104,6 +105,10 @@ struct CgInstD {
     CG_INST_FIELDS(CG_AS1)
 };
 static_assert(sizeof(CgInst) == sizeof(CgInstD), "descriptor views differ");
+
+// A launch map holds one {instance, workgroup index within the instance}
+// pair per workgroup of the launch (the maps may cover a subset of the batch).
+CG_DEV int2 cg_map_entry(const int* map) { return reinterpret_cast<const int2*>(map)[blockIdx.x]; }
 
 CG_DEV const CgInstD& cg_inst(const CgInst* insts, int b)
 {

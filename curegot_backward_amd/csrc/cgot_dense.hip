@@ -139,9 +139,10 @@ __global__ __launch_bounds__(CG_WG) void k_chol_fill(const CgInst* __restrict__ 
 // Wave map: blockIdx.x -> (instance, 4 consecutive indices), one wave each.
 CG_DEV bool cg_decode_wave(const CgInst* insts, const int* wave_inst, const double* ctl, int& b, int& q)
 {
-    b = wave_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wave_inst);
+    b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
-    q = (blockIdx.x - insts[b].row0) * CG_WAVES + (threadIdx.x >> 6);
+    q = e.y * CG_WAVES + (threadIdx.x >> 6);
     return true;
 }
 
@@ -153,7 +154,6 @@ CG_DEV bool cg_decode_wave(const CgInst* insts, const int* wave_inst, const doub
 // one group after another, so every entry receives its terms in a fixed
 // order; the finished row is written once.  Longer rows update S in place, l
 // by l in ascending order with a fence between l steps.
-#define CG_SCHUR_LDS 1024
 CG_DEV void cg_wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict_
                                                        const int* __restrict__ wave_inst,
                                                        const double* __restrict__ ctl)
 {
-    __shared__ double rowbuf[CG_WAVES][CG_SCHUR_LDS];
+    extern __shared__ double rowbuf[];   // CG_WAVES rows of the largest P <= CG_SCHUR_LDS in the launch
     int b, s2;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, s2)) return;
     const CgInstD& I = cg_inst(insts, b);
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict_
         }
         return;
     }
-    double* R = rowbuf[threadIdx.x >> 6];
+    double* R = rowbuf + (size_t)(threadIdx.x >> 6) * P;
     const double dss = Ds[s2];
     for (int s = lane; s <= s2; s += 64) R[s] = (s == s2) ? dss : 0.0;
     cg_wave_sync();
@@ -612,19 +612,39 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
 // Trailing update of tile (TI, TJ), kb < TJ <= TI: C -= A B^T with A, B the
 // panel rows of blocks TI and TJ (C is loaded before the product).  Diagonal
 // tiles also apply the forward-substitution update rhs_TI -= A y_kb.  The tile
-// (kb+1, kb+1), the first workgroup of the instance's range, is final after
-// this update: it factors and inverts itself, publishes W_{kb+1}
-// (flags[kb+1] = fepoch) and solves its rhs block.  The tiles (TI, kb+1) below
-// it are the next panel: after their update they wait for W_{kb+1} and store
-// X = C W_{kb+1}^T in place (they wait only on the first workgroup of their
-// range, which waits on nothing).
+// (kb+1, kb+1) is final after this update: it factors and inverts itself,
+// publishes W_{kb+1} (flags[kb+1] = fepoch) and solves its rhs block.  The
+// tiles (TI, kb+1) below it are the next panel: after their update they wait
+// for W_{kb+1} and store X = C W_{kb+1}^T in place.
+// The launch holds three segments, each with its instance prefix table
+// (pref, pref + seg_stride, pref + 2 seg_stride): the diagonal tiles of all
+// instances, so that every factor starts at once; the generic tiles, which
+// wait on nothing; and the next-panel tiles last, which wait only on a
+// diagonal tile of the first segment (dispatched before them).
 __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict__ insts, int B,
-                                                       const int* __restrict__ pref, int kb, int fepoch)
+                                                       const int* __restrict__ pref, int seg_stride, int kb,
+                                                       int fepoch)
 {
-    const int b = cg_find_inst(pref, B, blockIdx.x);
+    const int* prefG = pref + seg_stride;
+    const int* prefP = pref + 2 * seg_stride;
+    const int nD = pref[B], nG = prefG[B];
+    int b, ti, tj;
+    if ((int)blockIdx.x < nD) {
+        b = cg_find_inst(pref, B, blockIdx.x);
+        ti = tj = 0;
+    } else if ((int)blockIdx.x < nD + nG) {
+        const int w = blockIdx.x - nD;
+        b = cg_find_inst(prefG, B, w);
+        cg_tri_index(w - prefG[b], ti, tj);
+        ti++;
+        tj++;
+    } else {
+        const int w = blockIdx.x - nD - nG;
+        b = cg_find_inst(prefP, B, w);
+        ti = 1 + w - prefP[b];
+        tj = 0;
+    }
     const CgInstD& I = cg_inst(insts, b);
-    int ti, tj;
-    cg_tri_index(blockIdx.x - pref[b], ti, tj);
     const int TI = kb + 1 + ti, TJ = kb + 1 + tj;
     const int P = I.P;
     const int c0 = kb * CG_NB;

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <array>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -53,17 +54,21 @@ struct Pinned {
     double* get(size_t n)
     {
         if (n > cap) {
-            // the old buffer may still be read by an in-flight copy: keep it
+            // the old buffer may still be read by an in-flight copy: keep it.
+            // Grow geometrically so that a thread reaches its working size in
+            // a few allocations (pinning host memory is slow).
+            const size_t c = std::max<size_t>({n, 2 * cap, 65536});
             double* q = nullptr;
-            CG_HIP(hipHostMalloc((void**)&q, std::max<size_t>(n, 1024) * sizeof(double), hipHostMallocDefault));
+            CG_HIP(hipHostMalloc((void**)&q, c * sizeof(double), hipHostMallocDefault));
             p = q;
-            cap = std::max<size_t>(n, 1024);
+            cap = c;
         }
         return p;
     }
 };
 thread_local Pinned g_ctl_host;
 thread_local Pinned g_scal_host;
+thread_local Pinned g_map_host;
 
 // ---------------------------------------------------------------------------
 // Workspace layout: a bump allocator over the caller's bytes.
@@ -169,10 +174,17 @@ struct Batch {
     int B = 0;
     std::vector<CgInst> h;     // host copy of descriptors
     CgInst* d_insts = nullptr;
+    // Launch maps, one {instance, workgroup within the instance} pair per
+    // workgroup: the current maps (and their workgroup counts) are either the
+    // full maps of the batch or the packed maps of the instances still live.
     int* d_tile = nullptr;
     int* d_vec = nullptr;
     int* d_row = nullptr;
     int ntiles = 0, nvec = 0, nrow = 0;
+    int* d_full = nullptr;     // tiles | vecs | rows of every instance
+    int* d_live = nullptr;     // the same layout for the packed live set
+    int ntiles_full = 0, nvec_full = 0, nrow_full = 0;
+    long long packed_wgs = 0;  // workgroups of the current maps
     double* d_ctl = nullptr;   // kCtlSlots * B * CG_NCTL
     double* d_scal = nullptr;  // B * CG_NSCAL
     int* d_pref = nullptr;     // dense prefix tables
@@ -189,12 +201,12 @@ struct Batch {
     void layout_common(Bump& bp, int nblk_max)
     {
         d_insts = bp.take<CgInst>(B);
-        d_tile = bp.take<int>(ntiles);
-        d_vec = bp.take<int>(nvec);
-        d_row = bp.take<int>(nrow);
+        const size_t nmap = 2 * ((size_t)ntiles + nvec + nrow);
+        d_full = bp.take<int>(std::max<size_t>(nmap, 2));
+        d_live = bp.take<int>(std::max<size_t>(nmap, 2));
         d_ctl = bp.take<double>((size_t)kCtlSlots * B * CG_NCTL);
         d_scal = bp.take<double>((size_t)B * CG_NSCAL);
-        pref_cap = (3 * nblk_max + 3) * (B + 1) + nblk_max + 1;
+        pref_cap = (4 * nblk_max + 3) * (B + 1) + nblk_max + 1;
         d_pref = bp.take<int>(pref_cap);
         d_ptrs = bp.take<const double*>(3 * (size_t)B);
     }
@@ -207,22 +219,66 @@ struct Batch {
             I.vec0 = nvec; nvec += I.nvec;
             I.row0 = nrow; nrow += I.nrow;
         }
+        ntiles_full = ntiles; nvec_full = nvec; nrow_full = nrow;
+    }
+
+    // Write the maps of the instances with keep[b] into dst (device) through
+    // the pinned staging buffer; returns their workgroup counts.
+    void write_maps(int* dst, const std::vector<char>& keep, int& nt, int& nv, int& nr)
+    {
+        const size_t cap = 2 * ((size_t)ntiles_full + nvec_full + nrow_full);
+        int* hm = (int*)g_map_host.get((cap + 1) / 2);
+        nt = nv = nr = 0;
+        for (int b = 0; b < B; b++) if (keep[b]) { nt += h[b].ntiles; nv += h[b].nvec; nr += h[b].nrow; }
+        int* pt = hm;
+        int* pv = hm + 2 * (size_t)nt;
+        int* pr = pv + 2 * (size_t)nv;
+        for (int b = 0; b < B; b++) {
+            if (!keep[b]) continue;
+            const CgInst& I = h[b];
+            for (int t = 0; t < I.ntiles; t++) { *pt++ = b; *pt++ = t; }
+            for (int t = 0; t < I.nvec; t++) { *pv++ = b; *pv++ = t; }
+            for (int t = 0; t < I.nrow; t++) { *pr++ = b; *pr++ = t; }
+        }
+        const size_t used = 2 * ((size_t)nt + nv + nr);
+        if (used) CG_HIP(hipMemcpyAsync(dst, hm, sizeof(int) * used, hipMemcpyHostToDevice, stream));
+    }
+
+    void set_maps(int* base, int nt, int nv, int nr)
+    {
+        d_tile = base;
+        d_vec = base + 2 * (size_t)nt;
+        d_row = d_vec + 2 * (size_t)nv;
+        ntiles = nt; nvec = nv; nrow = nr;
+        packed_wgs = (long long)nt + nv + nr;
+    }
+
+    // Launch over every instance again (after a loop that packed the maps).
+    void use_full() { set_maps(d_full, ntiles_full, nvec_full, nrow_full); }
+
+    // Pack the maps to the instances with live[b] once they hold at most 7/8
+    // of the current workgroups: the M passes and the per-row and per-vector
+    // kernels then dispatch no workgroups for retired instances.  Call only
+    // after a stream synchronisation since the last call (the pinned staging
+    // buffer is reused) and launch nothing for instances outside `live` until
+    // the next call or use_full().
+    void pack_live(const std::vector<char>& live)
+    {
+        long long w = 0;
+        for (int b = 0; b < B; b++) if (live[b]) w += (long long)h[b].ntiles + h[b].nvec + h[b].nrow;
+        if (8 * w > 7 * packed_wgs) return;
+        int nt, nv, nr;
+        write_maps(d_live, live, nt, nv, nr);
+        set_maps(d_live, nt, nv, nr);
     }
 
     void upload_maps(const std::vector<const double*>& ptrs)
     {
-        std::vector<int> tile(ntiles), vec(nvec), row(nrow);
-        for (int b = 0; b < B; b++) {
-            const CgInst& I = h[b];
-            for (int t = 0; t < I.ntiles; t++) tile[I.tile0 + t] = b;
-            for (int t = 0; t < I.nvec; t++) vec[I.vec0 + t] = b;
-            for (int t = 0; t < I.nrow; t++) row[I.row0 + t] = b;
-            h[b].scal = d_scal + (size_t)b * CG_NSCAL;
-        }
+        for (int b = 0; b < B; b++) h[b].scal = d_scal + (size_t)b * CG_NSCAL;
         CG_HIP(hipMemcpyAsync(d_insts, h.data(), sizeof(CgInst) * B, hipMemcpyHostToDevice, stream));
-        CG_HIP(hipMemcpyAsync(d_tile, tile.data(), sizeof(int) * std::max(ntiles, 1), hipMemcpyHostToDevice, stream));
-        CG_HIP(hipMemcpyAsync(d_vec, vec.data(), sizeof(int) * std::max(nvec, 1), hipMemcpyHostToDevice, stream));
-        CG_HIP(hipMemcpyAsync(d_row, row.data(), sizeof(int) * std::max(nrow, 1), hipMemcpyHostToDevice, stream));
+        int nt, nv, nr;
+        write_maps(d_full, std::vector<char>(B, 1), nt, nv, nr);
+        set_maps(d_full, nt, nv, nr);
         if (!ptrs.empty())
             CG_HIP(hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(void*) * ptrs.size(), hipMemcpyHostToDevice, stream));
         // pageable sources: make sure the copies are done before the vectors die
@@ -296,9 +352,9 @@ struct Batch {
     {
         if (ntiles) hipLaunchKernelGGL(k, dim3(ntiles), dim3(CG_WG), 0, stream, d_insts, d_tile, c);
     }
-    void rows0(void (*k)(const CgInst*, const int*, const double*), const double* c)
+    void rows0(void (*k)(const CgInst*, const int*, const double*), const double* c, size_t lds = 0)
     {
-        if (nrow) hipLaunchKernelGGL(k, dim3(nrow), dim3(CG_WG), 0, stream, d_insts, d_row, c);
+        if (nrow) hipLaunchKernelGGL(k, dim3(nrow), dim3(CG_WG), lds, stream, d_insts, d_row, c);
     }
     void per_inst(void (*k)(const CgInst*, const double*, int), const double* c, int arg)
     {
@@ -687,10 +743,12 @@ struct SplrSolver {
         int nblk_max = 0;
         for (int b = 0; b < B; b++)
             if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
-        // dense prefix tables: fill | panel[kb] | update[kb] | (unused)[kb] | diag0,
-        // then the backward-substitution order (instances by decreasing block
-        // count) and its level prefix (level t: instances with more than t blocks)
-        const size_t lvl_off = (size_t)(3 * nblk_max + 3) * (B + 1);
+        // dense prefix tables: fill | panel[kb] | update launch kb in three
+        // segments: diagonal tile[kb] | generic tiles[kb] | next-panel tiles[kb]
+        // | diag0, then the backward-substitution order (instances by
+        // decreasing block count) and its level prefix (level t: instances with
+        // more than t blocks)
+        const size_t lvl_off = (size_t)(4 * nblk_max + 3) * (B + 1);
         std::vector<int> pref(lvl_off + nblk_max + 1, 0);
         auto tab = [&](int t) { return pref.data() + (size_t)t * (B + 1); };
         for (int b = 0; b < B; b++) {
@@ -699,16 +757,17 @@ struct SplrSolver {
             for (int kb = 0; kb < nblk_max; kb++) {
                 const int T = nb - kb - 1;
                 tab(1 + kb)[b + 1] = (kb < nb) ? (nb - kb - 1) : 0;
-                tab(1 + nblk_max + kb)[b + 1] = (T > 0) ? T * (T + 1) / 2 : 0;
-                tab(1 + 2 * nblk_max + kb)[b + 1] = (kb < nb) ? (kb + 1) : 0;
+                tab(1 + nblk_max + kb)[b + 1] = (T > 0) ? 1 : 0;
+                tab(1 + 2 * nblk_max + kb)[b + 1] = (T > 0) ? T * (T - 1) / 2 : 0;
+                tab(1 + 3 * nblk_max + kb)[b + 1] = (T > 0) ? T - 1 : 0;
             }
-            tab(1 + 3 * nblk_max)[b + 1] = (nb > 0) ? 1 : 0;
+            tab(1 + 4 * nblk_max)[b + 1] = (nb > 0) ? 1 : 0;
         }
-        for (int t = 0; t < 3 * nblk_max + 2; t++)
+        for (int t = 0; t < 4 * nblk_max + 2; t++)
             for (int b = 0; b < B; b++) tab(t)[b + 1] += tab(t)[b];
         int nact = 0;
         {
-            int* ord = tab(2 + 3 * nblk_max);
+            int* ord = tab(2 + 4 * nblk_max);
             for (int b = 0; b < B; b++)
                 if (on[b] && hi[b].nblk > 0) ord[nact++] = b;
             std::stable_sort(ord, ord + nact, [&](int x, int y) { return hi[x].nblk > hi[y].nblk; });
@@ -734,15 +793,19 @@ struct SplrSolver {
         auto total = [&](int t) { return tab(t)[B]; };
         if (nblk_max > 0 && total(0) > 0)
             hipLaunchKernelGGL(k_chol_fill, dim3(total(0)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(0));
-        bt.rows0(k_schur_build, dc);
+        // LDS rows of the Schur build: CG_WAVES rows of the largest P that fits
+        int pcap = 1;
+        for (int b = 0; b < B; b++)
+            if (on[b] && bt.h[b].P <= CG_SCHUR_LDS) pcap = std::max(pcap, bt.h[b].P);
+        bt.rows0(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
         bt.rows0(k_schur_rhs, dc);
         // block flags: factor of diagonal block k published = 2*epoch-1, solution
         // block k published = 2*epoch (flags start at 0 for the solve)
         epoch++;
         const int fepoch = 2 * epoch - 1;
-        if (nblk_max > 0 && total(1 + 3 * nblk_max) > 0)
-            hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 3 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                               dtab(1 + 3 * nblk_max), fepoch);
+        if (nblk_max > 0 && total(1 + 4 * nblk_max) > 0)
+            hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 4 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
+                               dtab(1 + 4 * nblk_max), fepoch);
         for (int kb = 0; kb < nblk_max; kb++) {
             // panel of block 0 on its own; later panels are formed by the
             // previous step's update launch
@@ -751,7 +814,10 @@ struct SplrSolver {
                 ProfScope ps(bt.stream, PROF_CHOL_PANEL, 0.0);
                 hipLaunchKernelGGL(k_chol_panel, dim3(tp), dim3(CG_WG), 0, bt.stream, bt.d_insts, B, dtab(1 + kb), kb);
             }
-            const int tu = total(1 + nblk_max + kb);
+            // diagonal tiles first (their factors start at once), then the
+            // generic tiles, then the next-panel tiles (they wait for the
+            // factor of their instance's diagonal tile)
+            const int tu = total(1 + nblk_max + kb) + total(1 + 2 * nblk_max + kb) + total(1 + 3 * nblk_max + kb);
             if (tu > 0) {
                 // algorithmic flops: 2*64^3 per full lower tile (diagonal tiles count half)
                 double fl = 0.0;
@@ -762,14 +828,14 @@ struct SplrSolver {
                 }
                 ProfScope ps(bt.stream, PROF_CHOL_UPDATE, fl);
                 hipLaunchKernelGGL(k_chol_update, dim3(tu), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                                   dtab(1 + nblk_max + kb), kb, fepoch);
+                                   dtab(1 + nblk_max + kb), nblk_max * (B + 1), kb, fepoch);
             }
         }
         const int ntrsv = (nblk_max > 0) ? pref[lvl_off + nblk_max] : 0;
         if (ntrsv > 0) {
             ProfScope ps(bt.stream, PROF_TRSV, 0.0);
             hipLaunchKernelGGL(k_trsv_flow, dim3(ntrsv), dim3(CG_WG), 0, bt.stream, bt.d_insts,
-                               dtab(2 + 3 * nblk_max), dtab0 + lvl_off, nblk_max, 2 * epoch);
+                               dtab(2 + 4 * nblk_max), dtab0 + lvl_off, nblk_max, 2 * epoch);
         }
         bt.rows0(k_schur_back, dc);
     }
@@ -844,6 +910,7 @@ struct SplrSolver {
                 nact++;
             }
             if (nact == 0) break;
+            bt.pack_live(act);
             auto c = mask(act);
             for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ALLOW_LR] = iter > 0 ? 1.0 : 0.0;
             const double* dc = bt.ctl(c);
@@ -981,6 +1048,7 @@ struct SplrSolver {
         }
 
         // outputs and context
+        bt.use_full();
         const double* dall2 = bt.ctl(mask(all));
         bt.vec0(k_output_dual, dall2);
         bt.tiles0(k_plan, dall2);
@@ -1162,6 +1230,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
     }
     int chunk = 4;
     while (nrun > 0) {
+        bt.pack_live(running);
         auto c2 = c;
         for (int b = 0; b < B; b++) c2[(size_t)b * CG_NCTL + CTL_ACTIVE] = running[b] ? 1.0 : 0.0;
         const double* d2 = bt.ctl(c2);
@@ -1184,6 +1253,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
     }
     // a fresh copy of the all-active control: the slot of dc may have been
     // reused by the PCG loop's uploads (the ring holds kCtlSlots blocks)
+    bt.use_full();
     const double* dfin = bt.ctl(c);
     hipLaunchKernelGGL(k_grad_assemble, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dfin,
                        lams, Gs, outs);
@@ -1266,6 +1336,7 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
             na += act[b];
         }
         if (na == 0) break;
+        bt.pack_live(act);
         const double* dc = mk(act);
         bt.vec(k_vcopy, dc, 0);
         row_lse(bt, dc, CG_PT_GAMMA);
@@ -1291,6 +1362,7 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
             }
         }
     }
+    bt.use_full();
     const double* da = mk(all);
     bt.vec0(k_output_dual, da);
     bt.tiles0(k_plan, da);
@@ -1424,16 +1496,20 @@ private:
     int workers_ = 0, idle_ = 0;
 };
 
-// Stream of the calling worker thread on `device` (created once, default priority).
-hipStream_t worker_stream(int device)
+// Streams of size classes 1..kMaxGroups-1 for batches issued from the calling
+// thread on `device` (created once per thread and device, default priority).
+// Each class index keeps its stream from call to call, so a caller uses the
+// same kMaxGroups streams (and hardware queues) every time.
+hipStream_t group_stream(int device, int k)
 {
-    thread_local std::map<int, hipStream_t> streams;
+    thread_local std::map<int, std::array<hipStream_t, kMaxGroups>> streams;
     auto it = streams.find(device);
-    if (it != streams.end()) return it->second;
-    hipStream_t st;
-    CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    streams[device] = st;
-    return st;
+    if (it == streams.end()) {
+        std::array<hipStream_t, kMaxGroups> a{};
+        for (int q = 1; q < kMaxGroups; q++) CG_HIP(hipStreamCreateWithFlags(&a[q], hipStreamNonBlocking));
+        it = streams.emplace(device, a).first;
+    }
+    return it->second[k];
 }
 
 void splr_solve_group(const cgot_splr_problem* probs, int B, void* ws, hipStream_t stream, cgot_splr_info* info)
@@ -1520,12 +1596,14 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
     std::vector<std::string> err(G);
     std::vector<hipEvent_t> done(G, nullptr);
     std::vector<std::function<void()>> jobs;
+    std::vector<hipStream_t> gst(G, nullptr);
+    for (int k = 1; k < G; k++) gst[k] = group_stream(device, k);
     for (int k = 0; k < G; k++) sub[k].assign(g.idx[k].size(), cgot_splr_info{});
     for (int k = 1; k < G; k++) {
         jobs.push_back([&, k]() {
             try {
                 CG_HIP(hipSetDevice(device));
-                hipStream_t st = worker_stream(device);
+                hipStream_t st = gst[k];
                 CG_HIP(hipStreamWaitEvent(st, start, 0));
                 splr_solve_group(g.probs[k].data(), (int)g.probs[k].size(), (char*)workspace + g.off[k], st,
                                  sub[k].data());

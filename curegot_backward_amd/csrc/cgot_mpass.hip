@@ -25,11 +25,11 @@ CG_DEV double cg_point(const CgInstD& I, int mode, double t, int k)
 CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const double* ctl,
                            int& b, int& rb, int& st)
 {
-    const int t = blockIdx.x;
-    b = tile_inst[t];
+    const int2 e = cg_map_entry(tile_inst);
+    b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
     const CgInstD& I = cg_inst(insts, b);
-    const int lt = t - I.tile0;
+    const int lt = e.y;
     rb = lt / I.nstrips;
     st = lt - rb * I.nstrips;
     return true;

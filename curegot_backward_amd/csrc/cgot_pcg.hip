@@ -46,10 +46,11 @@ CG_DEV bool cg_inst_reduce_p(const CgInstD& I, int wgi, double (&v)[NV], double 
 
 CG_DEV bool cg_pcg_decode(const CgInst* insts, const int* wg_inst, const double* ctl, int& b, int& wgi)
 {
-    b = wg_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wg_inst);
+    b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
     if (insts[b].scal[PCG_STATUS] != 0.0) return false;
-    wgi = blockIdx.x - insts[b].vec0;
+    wgi = e.y;
     return true;
 }
 
@@ -58,10 +59,11 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ i
                                                     const int* __restrict__ wg_inst,
                                                     const double* __restrict__ ctl)
 {
-    const int b = wg_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wg_inst);
+    const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInstD& I = cg_inst(insts, b);
-    const int wgi = blockIdx.x - I.vec0;
+    const int wgi = e.y;
     const int q = wgi * CG_WG + threadIdx.x;
     const int n = I.n, m = I.m;
     double v[3] = {0.0, 0.0, 0.0};   // ||mu||^2, r.z, any(mu != 0)

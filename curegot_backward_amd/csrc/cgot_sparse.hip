@@ -20,11 +20,11 @@ CG_DEV unsigned long long cg_kth(const CgInstD& I) { return (unsigned long long)
 CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const double* ctl,
                             int& b, int& rb, int& st)
 {
-    const int t = blockIdx.x;
-    b = tile_inst[t];
+    const int2 e = cg_map_entry(tile_inst);
+    b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
     const CgInstD& I = cg_inst(insts, b);
-    const int lt = t - I.tile0;
+    const int lt = e.y;
     rb = lt / I.nstrips;
     st = lt - rb * I.nstrips;
     return true;
@@ -127,11 +127,12 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restri
                                                           const int* __restrict__ row_inst,
                                                           const double* __restrict__ ctl)
 {
-    const int b = row_inst[blockIdx.x];
+    const int2 e = cg_map_entry(row_inst);
+    const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
+    const int i = e.y * CG_WAVES + w;
     if (i >= I.n) return;
     const unsigned long long kth = cg_kth(I);
     const int mm = I.m - 1;
@@ -203,11 +204,12 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restri
                                                           const int* __restrict__ row_inst,
                                                           const double* __restrict__ ctl)
 {
-    const int b = row_inst[blockIdx.x];
+    const int2 e = cg_map_entry(row_inst);
+    const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = (blockIdx.x - I.row0) * CG_WAVES + w;
+    const int i = e.y * CG_WAVES + w;
     if (i >= I.n) return;
     const unsigned long long kth = cg_kth(I);
     const int mm = I.m - 1;
@@ -361,10 +363,11 @@ __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restri
                                                           const int* __restrict__ wave_inst,
                                                           const double* __restrict__ ctl)
 {
-    const int b = wave_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wave_inst);
+    const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInstD& I = cg_inst(insts, b);
-    const int q = (blockIdx.x - I.row0) * CG_WAVES + (threadIdx.x >> 6);
+    const int q = e.y * CG_WAVES + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
     if (q < I.n) {
@@ -386,9 +389,10 @@ __global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ in
                                                    double* __restrict__ vals, int* __restrict__ cols,
                                                    int* __restrict__ rowptr)
 {
-    const int b = wg_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wg_inst);
+    const int b = e.x;
     const CgInstD& I = cg_inst(insts, b);
-    const int q = (blockIdx.x - I.vec0) * CG_WG + threadIdx.x;
+    const int q = e.y * CG_WG + threadIdx.x;
     const int n = I.n;
     if (q < n) {
         vals[q] = I.Da[q];

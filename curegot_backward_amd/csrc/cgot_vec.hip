@@ -1,7 +1,7 @@
 // Finalize and vector kernels: they turn the fixed-order partials of the M
 // passes into row/column sums, duals, gradients and per-instance scalars, and
 // run the O(n+m) parts of the SPLR step.  Grid: the "vec" map, i.e.
-// ceil((n+m)/256) workgroups per instance; wg_inst[blockIdx.x] names the
+// ceil((n+m)/256) workgroups per instance; the map entry of a workgroup names the
 // instance.  Per-instance scalar reductions use a last-workgroup-done counter:
 // every workgroup stores its partial, the last one to arrive sums all partials
 // in workgroup order, so the result does not depend on scheduling.
@@ -10,9 +10,10 @@
 
 CG_DEV bool cg_decode_vec(const CgInst* insts, const int* wg_inst, const double* ctl, int& b, int& wgi)
 {
-    b = wg_inst[blockIdx.x];
+    const int2 e = cg_map_entry(wg_inst);
+    b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return false;
-    wgi = blockIdx.x - insts[b].vec0;
+    wgi = e.y;
     return true;
 }
 
