@@ -18,7 +18,7 @@
 //   k_chol_update(kb): trailing update S22 -= L21 L21^T on lower tiles, the
 //                      forward-substitution update of the right-hand sides,
 //                      the factor/inverse of diagonal block kb+1 (final here)
-//                      and, once that is published, the panel of block kb+1;
+//                      and, once that is handed over, the panel of block kb+1;
 //   k_trsv_flow:       backward substitution L^T x = y, one workgroup per block,
 //                      chained through per-block flags within one launch.
 // Grids are compact over a ragged batch: pref[b]..pref[b+1] are instance b's
@@ -121,7 +121,49 @@ CG_DEV void cg_mfma_abt64(double (*A)[CG_NB + 1], double (*Bm)[CG_NB + 1], cg_d4
 CG_DEV int cg_mfma_row(int a, int r) { return 32 * ((threadIdx.x >> 6) >> 1) + 16 * a + 4 * r + ((threadIdx.x & 63) >> 4); }
 CG_DEV int cg_mfma_col(int c) { return 32 * ((threadIdx.x >> 6) & 1) + 16 * c + (threadIdx.x & 15); }
 
-// Zero the lower block tiles of S.
+// Values handed between workgroups of one launch (the blocks of W = L^{-1} of
+// a factorization, the blocks of a triangular solve's solution) are written
+// exactly once per solve into storage that held CG_SENT before the launch.
+// The writer stores them with relaxed agent-scope atomics and the reader polls
+// each value with relaxed agent-scope atomic loads until it differs from
+// CG_SENT: every location is written once, so per-location coherence is all
+// the hand-off needs (no flag, no release/acquire fences and their cache
+// write-back and invalidation).  CG_SENT is a NaN payload that arithmetic
+// never produces; a stored value with that bit pattern is replaced by the
+// default NaN.  The poll is bounded: on overrun it sets scal[CS_PIVOT] = 2
+// and returns 0.
+#define CG_SENT 0x7FF4C0DE5E9D1E00ull
+
+CG_DEV void cg_sent_store(CG_AS1 double* p, double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    if (u == CG_SENT) u = 0x7FF8000000000000ull;
+    __hip_atomic_store((CG_AS1 unsigned long long*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+CG_DEV unsigned long long cg_sent_peek(const CG_AS1 double* p)
+{
+    return __hip_atomic_load((const CG_AS1 unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+CG_DEV double cg_sent_wait(const CG_AS1 double* p, unsigned long long u, CG_AS1 double* scal)
+{
+    int spins = 0;
+    while (u == CG_SENT) {
+        __builtin_amdgcn_s_sleep(1);
+        u = cg_sent_peek(p);
+        if (++spins > (1 << 25)) {
+            scal[CS_PIVOT] = 2.0;
+            return 0.0;
+        }
+    }
+    return __longlong_as_double((long long)u);
+}
+
+CG_DEV double cg_sent_load(const CG_AS1 double* p, CG_AS1 double* scal) { return cg_sent_wait(p, cg_sent_peek(p), scal); }
+
+// Zero the lower block tiles of S; the diagonal tiles also set their block of
+// Winv to CG_SENT (the factor's diagonal workgroup hands W over through it).
 __global__ __launch_bounds__(CG_WG) void k_chol_fill(const CgInst* __restrict__ insts, int B,
                                                      const int* __restrict__ pref)
 {
@@ -133,6 +175,10 @@ __global__ __launch_bounds__(CG_WG) void k_chol_fill(const CgInst* __restrict__ 
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = TI * CG_NB + e / CG_NB, c = TJ * CG_NB + e % CG_NB;
         if (r < P && c < P) I.S[(size_t)r * P + c] = 0.0;
+    }
+    if (TI == TJ) {
+        auto Wd = reinterpret_cast<CG_AS1 unsigned long long*>(I.Winv + (size_t)TI * CG_NB * CG_NB);
+        for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) Wd[e] = CG_SENT;
     }
 }
 
@@ -501,22 +547,20 @@ CG_DEV bool cg_lds_potrf_inv(double (*L)[CG_NB + 1], double (*W)[CG_NB + 1], int
 // y_kb = W_kb b_kb for the two right-hand sides (b already holds the updates).
 // L and W are LDS blocks.
 CG_DEV void cg_diag_block(const CgInstD& I, int kb, double (*L)[CG_NB + 1], double (*W)[CG_NB + 1],
-                          double* dinv, double* ytmp, int fepoch)
+                          double* dinv, double* ytmp)
 {
     const int P = I.P;
     const int c0 = kb * CG_NB;
     const int nb = min(CG_NB, P - c0);
     const bool ok = cg_lds_potrf_inv(L, W, nb, dinv);
     if (!ok && threadIdx.x == 0) I.scal[CS_PIVOT] = 1.0;
+    // hand W_kb to the panel workgroups of the same launch (Winv block kb
+    // holds CG_SENT until then)
     auto Wd = I.Winv + (size_t)kb * CG_NB * CG_NB;
     for (int e = threadIdx.x; e < CG_NB * CG_NB; e += CG_WG) {
         const int r = e / CG_NB, c = e % CG_NB;
-        Wd[e] = (r < nb && c <= r) ? W[r][c] : 0.0;
+        cg_sent_store(Wd + e, (r < nb && c <= r) ? W[r][c] : 0.0);
     }
-    // publish W_kb to the panel workgroups of the same launch
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&I.flags[kb], fepoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < 2 * CG_NB) {
         const int r = threadIdx.x / CG_NB, q = threadIdx.x % CG_NB;
         ytmp[threadIdx.x] = (q < nb) ? I.rhs[(size_t)r * P + c0 + q] : 0.0;
@@ -563,7 +607,7 @@ CG_DEV void cg_publish_flag(CG_AS1 int* flags, int k, int value)
 
 // Block 0: factor/invert the first diagonal block (one workgroup per instance).
 __global__ __launch_bounds__(CG_WG) void k_chol_diag0(const CgInst* __restrict__ insts, int B,
-                                                      const int* __restrict__ pref, int fepoch)
+                                                      const int* __restrict__ pref)
 {
     const int b = cg_find_inst(pref, B, blockIdx.x);
     const CgInstD& I = cg_inst(insts, b);
@@ -574,8 +618,11 @@ __global__ __launch_bounds__(CG_WG) void k_chol_diag0(const CgInst* __restrict__
     __shared__ double dinv[CG_NB];
     __shared__ double ytmp[2 * CG_NB];
     cg_tile_to_lds(L, I.S, P, nb, nb, true);
+    // the backward substitution hands its solution blocks over through ltmp
+    auto x = reinterpret_cast<CG_AS1 unsigned long long*>(I.ltmp);
+    for (int e = threadIdx.x; e < 2 * P; e += CG_WG) x[e] = CG_SENT;
     __syncthreads();
-    cg_diag_block(I, 0, L, W, dinv, ytmp, fepoch);
+    cg_diag_block(I, 0, L, W, dinv, ytmp);
 }
 
 // Panel rows of block column kb (row tile rt + 1 below the diagonal):
@@ -613,7 +660,7 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
 // panel rows of blocks TI and TJ (C is loaded before the product).  Diagonal
 // tiles also apply the forward-substitution update rhs_TI -= A y_kb.  The tile
 // (kb+1, kb+1) is final after this update: it factors and inverts itself,
-// publishes W_{kb+1} (flags[kb+1] = fepoch) and solves its rhs block.  The
+// hands W_{kb+1} over through Winv (see CG_SENT) and solves its rhs block.  The
 // tiles (TI, kb+1) below it are the next panel: after their update they wait
 // for W_{kb+1} and store X = C W_{kb+1}^T in place.
 // The launch holds three segments, each with its instance prefix table
@@ -622,8 +669,7 @@ __global__ __launch_bounds__(CG_WG) void k_chol_panel(const CgInst* __restrict__
 // wait on nothing; and the next-panel tiles last, which wait only on a
 // diagonal tile of the first segment (dispatched before them).
 __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict__ insts, int B,
-                                                       const int* __restrict__ pref, int seg_stride, int kb,
-                                                       int fepoch)
+                                                       const int* __restrict__ pref, int seg_stride, int kb)
 {
     const int* prefG = pref + seg_stride;
     const int* prefP = pref + 2 * seg_stride;
@@ -723,13 +769,22 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
             }
     if (TI == TJ) {
         __syncthreads();
-        cg_diag_block(I, kb + 1, As, Bs, dinv, ytmp, fepoch);   // Bs is free now: it holds W
+        cg_diag_block(I, kb + 1, As, Bs, dinv, ytmp);   // Bs is free now: it holds W
         return;
     }
-    // next panel: X = C W^T, W = W_{kb+1} once published
-    cg_wait_flag(I, kb + 1, fepoch);
-    const auto Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
-    cg_tile_to_lds(Bs, Wd, CG_NB, CG_NB, CG_NB, false);
+    // next panel: X = C W^T, W = W_{kb+1} once handed over (all of a
+    // thread's loads in flight, then the ones still at CG_SENT are polled)
+    {
+        const auto Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
+        unsigned long long u[CG_NB * CG_NB / CG_WG];
+#pragma unroll
+        for (int q = 0; q < CG_NB * CG_NB / CG_WG; q++) u[q] = cg_sent_peek(Wd + threadIdx.x + CG_WG * q);
+#pragma unroll
+        for (int q = 0; q < CG_NB * CG_NB / CG_WG; q++) {
+            const int e = threadIdx.x + CG_WG * q;
+            Bs[e / CG_NB][e % CG_NB] = cg_sent_wait(Wd + e, u[q], I.scal);
+        }
+    }
     __syncthreads();
     cg_mfma_abt64(As, Bs, acc);
 #pragma unroll
@@ -758,11 +813,11 @@ struct CgTriArgs {
 };
 
 // Backward substitution L^T x = y, block kb of a dataflow launch: accumulate
-// L_{j,kb}^T x_j for j = nblk-1 .. kb+1 as each x_j is published (flags[j] >=
-// target; the tile is loaded before the wait), then x_kb = W_kb^T (y_kb - sum),
-// published with flags[kb] = target.
+// L_{j,kb}^T x_j for j = nblk-1 .. kb+1 as each x_j arrives (x holds CG_SENT
+// before the launch; the tile is loaded before the poll), then
+// x_kb = W_kb^T (y_kb - sum), handed on through x.
 template <int NR>
-CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb, int target)
+CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb)
 {
     const int P = A.P;
     const int nblk = (P + CG_NB - 1) / CG_NB;
@@ -789,10 +844,9 @@ CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb, int target)
             const int rr = g + CG_WAVES * q;
             sv[q] = (rr < nbj) ? A.S[(size_t)(r0 + rr) * P + c0 + c] : 0.0;
         }
-        cg_wait_flag_p(A.flags, A.scal, j, target);
         if (tid < NR * CG_NB) {
             const int r = tid / CG_NB, q = tid % CG_NB;
-            xs[r][q] = (q < nbj) ? A.x[(size_t)r * P + r0 + q] : 0.0;
+            xs[r][q] = (q < nbj) ? cg_sent_load(A.x + (size_t)r * P + r0 + q, A.scal) : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -832,9 +886,8 @@ CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb, int target)
     if (tid < NR * CG_NB) {
         const int r = tid / CG_NB, q = tid % CG_NB;
         const double x = (part[r][0][q] + part[r][1][q]) + (part[r][2][q] + part[r][3][q]);
-        if (q < nb) A.x[(size_t)r * P + c0 + q] = x;
+        if (q < nb) cg_sent_store(A.x + (size_t)r * P + c0 + q, x);
     }
-    cg_publish_flag(A.flags, kb, target);
 }
 
 // Backward substitution L^T x = y of the Newton solve as a dataflow over the
@@ -845,16 +898,17 @@ CG_DEV void cg_bwd_block(const CgTriArgs& A, int kb, int target)
 // starts at once.  A workgroup waits only on lower levels, i.e. on workgroups
 // with smaller launch indices, which were dispatched before it and themselves
 // wait only on earlier ones, so every wait ends; the spin is also bounded and
-// flags CS_PIVOT = 2 instead of hanging.  y lives in rhs, x goes to ltmp.
+// flags CS_PIVOT = 2 instead of hanging.  y lives in rhs, x goes to ltmp
+// (set to CG_SENT by k_chol_diag0).
 __global__ __launch_bounds__(CG_WG) void k_trsv_flow(const CgInst* __restrict__ insts, const int* __restrict__ order,
-                                                     const int* __restrict__ lvl_pref, int nlvl, int epoch)
+                                                     const int* __restrict__ lvl_pref, int nlvl)
 {
     const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
     const int b = order[blockIdx.x - lvl_pref[t]];
     const CgInstD& I = cg_inst(insts, b);
     const int nblk = (I.P + CG_NB - 1) / CG_NB;
     const CgTriArgs A{I.S, I.Winv, I.P, I.rhs, I.ltmp, I.flags, I.scal};
-    cg_bwd_block<2>(A, nblk - 1 - t, epoch);
+    cg_bwd_block<2>(A, nblk - 1 - t);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,7 +951,11 @@ __global__ __launch_bounds__(CG_WG) void k_pc_rhs(const CgInst* __restrict__ ins
         }
     }
     acc = cg_wave_sum(acc);
-    if (lane == 0) I.prhs[s] = r[F.beta_side ? n + s : s] - acc;
+    if (lane == 0) {
+        I.prhs[s] = r[F.beta_side ? n + s : s] - acc;
+        // the backward substitution hands its solution blocks over through pltmp
+        reinterpret_cast<CG_AS1 unsigned long long*>(I.pltmp)[s] = CG_SENT;
+    }
 }
 
 // Forward substitution L y = rhs, block kb of a dataflow launch (y replaces rhs
@@ -970,7 +1028,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_fwd(const CgInst* __restrict__ ins
 }
 
 __global__ __launch_bounds__(CG_WG) void k_pc_bwd(const CgInst* __restrict__ insts, const int* __restrict__ order,
-                                                  const int* __restrict__ lvl_pref, int nlvl, int target)
+                                                  const int* __restrict__ lvl_pref, int nlvl)
 {
     const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
     const CgInstD& I = cg_inst(insts, order[blockIdx.x - lvl_pref[t]]);
@@ -978,7 +1036,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_bwd(const CgInst* __restrict__ ins
     const CgInstD& F = cg_fwd_of(I);
     const int nblk = (F.P + CG_NB - 1) / CG_NB;
     const CgTriArgs A{F.S, F.Winv, F.P, I.prhs, I.pltmp, I.flags, I.scal};
-    cg_bwd_block<1>(A, nblk - 1 - t, target);
+    cg_bwd_block<1>(A, nblk - 1 - t);
 }
 
 // q = A^{-1} r in the N layout into z (sv): the small side is the Schur

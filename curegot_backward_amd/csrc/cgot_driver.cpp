@@ -663,7 +663,6 @@ struct SplrSolver {
     Batch& bt;
     std::vector<HostInst> hi;
 
-    int epoch = 0;   // search-direction launches so far (block flags of this solve)
 
     SplrSolver(Batch& b) : bt(b) {}
 
@@ -799,13 +798,9 @@ struct SplrSolver {
             if (on[b] && bt.h[b].P <= CG_SCHUR_LDS) pcap = std::max(pcap, bt.h[b].P);
         bt.rows0(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
         bt.rows0(k_schur_rhs, dc);
-        // block flags: factor of diagonal block k published = 2*epoch-1, solution
-        // block k published = 2*epoch (flags start at 0 for the solve)
-        epoch++;
-        const int fepoch = 2 * epoch - 1;
         if (nblk_max > 0 && total(1 + 4 * nblk_max) > 0)
             hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 4 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                               dtab(1 + 4 * nblk_max), fepoch);
+                               dtab(1 + 4 * nblk_max));
         for (int kb = 0; kb < nblk_max; kb++) {
             // panel of block 0 on its own; later panels are formed by the
             // previous step's update launch
@@ -828,14 +823,14 @@ struct SplrSolver {
                 }
                 ProfScope ps(bt.stream, PROF_CHOL_UPDATE, fl);
                 hipLaunchKernelGGL(k_chol_update, dim3(tu), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
-                                   dtab(1 + nblk_max + kb), nblk_max * (B + 1), kb, fepoch);
+                                   dtab(1 + nblk_max + kb), nblk_max * (B + 1), kb);
             }
         }
         const int ntrsv = (nblk_max > 0) ? pref[lvl_off + nblk_max] : 0;
         if (ntrsv > 0) {
             ProfScope ps(bt.stream, PROF_TRSV, 0.0);
             hipLaunchKernelGGL(k_trsv_flow, dim3(ntrsv), dim3(CG_WG), 0, bt.stream, bt.d_insts,
-                               dtab(2 + 4 * nblk_max), dtab0 + lvl_off, nblk_max, 2 * epoch);
+                               dtab(2 + 4 * nblk_max), dtab0 + lvl_off, nblk_max);
         }
         bt.rows0(k_schur_back, dc);
     }
@@ -1181,8 +1176,8 @@ struct PcSolver {
     }
 
     // r = yv -> z = sv for the instances still running; then r.z and beta
-    // (init: beta = 0).  Flags of one application: forward blocks 2e-1,
-    // backward blocks 2e.
+    // (init: beta = 0).  Flags of one application: forward blocks 2e-1 (the
+    // backward blocks are handed over through pltmp).
     void apply(const double* ctlp, int init)
     {
         if (ord.empty()) return;
@@ -1190,8 +1185,7 @@ struct PcSolver {
         bt.rows0(k_pc_rhs, ctlp);
         hipLaunchKernelGGL(k_pc_fwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
                            2 * epoch - 1);
-        hipLaunchKernelGGL(k_pc_bwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
-                           2 * epoch);
+        hipLaunchKernelGGL(k_pc_bwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl);
         bt.rows0(k_pc_back, ctlp);
         bt.vec0(k_pc_wdots, ctlp);
         bt.vec0(k_pc_wapply, ctlp);

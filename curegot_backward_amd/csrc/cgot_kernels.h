@@ -63,17 +63,17 @@ __global__ void k_schur_rhs(const CgInst*, const int*, const double*);          
 __global__ void k_schur_back(const CgInst*, const int*, const double*);           // wave grid
 // compact ragged grids: pref[b]..pref[b+1] are instance b's workgroups
 __global__ void k_chol_fill(const CgInst*, int B, const int* pref);
-__global__ void k_chol_diag0(const CgInst*, int B, const int* pref, int fepoch);
+__global__ void k_chol_diag0(const CgInst*, int B, const int* pref);
 __global__ void k_chol_panel(const CgInst*, int B, const int* pref, int kb);
-__global__ void k_chol_update(const CgInst*, int B, const int* pref, int seg_stride, int kb, int fepoch);
+__global__ void k_chol_update(const CgInst*, int B, const int* pref, int seg_stride, int kb);
 __global__ void k_pc_rhs(const CgInst*, const int* wave_inst, const double* ctl);
 __global__ void k_pc_fwd(const CgInst*, const int* order, const int* lvl_pref, int nlvl, int target);
-__global__ void k_pc_bwd(const CgInst*, const int* order, const int* lvl_pref, int nlvl, int target);
+__global__ void k_pc_bwd(const CgInst*, const int* order, const int* lvl_pref, int nlvl);
 __global__ void k_pc_back(const CgInst*, const int* wave_inst, const double* ctl);
 __global__ void k_pc_wdots(const CgInst*, const int* wg_inst, const double* ctl);
 __global__ void k_pc_wapply(const CgInst*, const int* wg_inst, const double* ctl);
 __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, int init);
-__global__ void k_trsv_flow(const CgInst*, const int* order, const int* lvl_pref, int nlvl, int epoch);
+__global__ void k_trsv_flow(const CgInst*, const int* order, const int* lvl_pref, int nlvl);
 
 // ---- backward PCG (vec grid) ----
 __global__ void k_pcg_init(const CgInst*, const int*, const double*);
