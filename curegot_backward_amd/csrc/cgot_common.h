@@ -237,6 +237,104 @@ CG_DEV void cg_ordered_sum2(const CG_AS1 double* p, const CG_AS1 double* q, int 
     }
 }
 
+// Values handed between workgroups of one launch (the blocks of W = L^{-1} of
+// a factorization, the blocks of a triangular solve's solution, the partials
+// of a last-workgroup reduction) are written exactly once per use into storage
+// that held CG_SENT before.
+// The writer stores them with relaxed agent-scope atomics and the reader polls
+// each value with relaxed agent-scope atomic loads until it differs from
+// CG_SENT: every location is written once, so per-location coherence is all
+// the hand-off needs (no flag, no release/acquire fences and their cache
+// write-back and invalidation).  CG_SENT is a NaN payload that arithmetic
+// never produces; a stored value with that bit pattern is replaced by the
+// default NaN.  The poll is bounded: on overrun it sets scal[CS_PIVOT] = 2
+// and returns 0.
+#define CG_SENT 0x7FF4C0DE5E9D1E00ull
+
+CG_DEV void cg_sent_store(CG_AS1 double* p, double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    if (u == CG_SENT) u = 0x7FF8000000000000ull;
+    __hip_atomic_store((CG_AS1 unsigned long long*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+CG_DEV unsigned long long cg_sent_peek(const CG_AS1 double* p)
+{
+    return __hip_atomic_load((const CG_AS1 unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+CG_DEV double cg_sent_wait(const CG_AS1 double* p, unsigned long long u, CG_AS1 double* scal)
+{
+    int spins = 0;
+    while (u == CG_SENT) {
+        __builtin_amdgcn_s_sleep(1);
+        u = cg_sent_peek(p);
+        if (++spins > (1 << 25)) {
+            scal[CS_PIVOT] = 2.0;
+            return 0.0;
+        }
+    }
+    return __longlong_as_double((long long)u);
+}
+
+CG_DEV double cg_sent_load(const CG_AS1 double* p, CG_AS1 double* scal) { return cg_sent_wait(p, cg_sent_peek(p), scal); }
+
+CG_DEV void cg_sent_reset(CG_AS1 double* p)
+{
+    __hip_atomic_store((CG_AS1 unsigned long long*)p, CG_SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Ordered sum of p[0..count) as each value arrives, resetting every slot to
+// CG_SENT for the next use (CG_SUM_U loads in flight).
+CG_DEV double cg_ordered_take(CG_AS1 double* p, int count, CG_AS1 double* scal)
+{
+    double s = 0.0;
+    for (int k = 0; k < count; k += CG_SUM_U) {
+        unsigned long long t[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) t[u] = (k + u < count) ? cg_sent_peek(p + k + u) : 0ull;
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++)
+            if (k + u < count) {
+                s += cg_sent_wait(p + k + u, t[u], scal);
+                cg_sent_reset(p + k + u);
+            }
+    }
+    return s;
+}
+
+// Per-instance sums of NV per-thread values over all vec workgroups of the
+// instance (wgi = this workgroup's index in it).  Every workgroup hands its
+// block sums over through wgpart (CG_SENT-initialised) and counts itself in
+// counter[slot] (relaxed); the last to arrive sums the partials in workgroup
+// order, so the result does not depend on scheduling, and resets wgpart and the
+// counter.  Returns true in that workgroup, where out[] holds the totals (all
+// threads).
+template <int NV>
+CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
+{
+    __shared__ double lds4[4];
+    __shared__ double tot[NV];
+    __shared__ int is_last;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = cg_block_sum(v[k], lds4);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) cg_sent_store(I.wgpart + (size_t)k * I.nvec + wgi, v[k]);
+        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (prev == (unsigned int)(I.nvec - 1));
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    if (threadIdx.x < NV) tot[threadIdx.x] = cg_ordered_take(I.wgpart + (size_t)threadIdx.x * I.nvec, I.nvec, I.scal);
+    if (threadIdx.x == 0) __hip_atomic_store(&I.counter[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = tot[k];
+    return true;
+}
+
+
 // Ordered LSE merge of count (max, sum) partials, loads batched as above.
 CG_DEV void cg_ordered_lse(const CG_AS1 double* pm, const CG_AS1 double* ps, int count, size_t stride, double& mx,
                            double& s)

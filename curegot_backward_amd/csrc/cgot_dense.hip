@@ -121,47 +121,6 @@ CG_DEV void cg_mfma_abt64(double (*A)[CG_NB + 1], double (*Bm)[CG_NB + 1], cg_d4
 CG_DEV int cg_mfma_row(int a, int r) { return 32 * ((threadIdx.x >> 6) >> 1) + 16 * a + 4 * r + ((threadIdx.x & 63) >> 4); }
 CG_DEV int cg_mfma_col(int c) { return 32 * ((threadIdx.x >> 6) & 1) + 16 * c + (threadIdx.x & 15); }
 
-// Values handed between workgroups of one launch (the blocks of W = L^{-1} of
-// a factorization, the blocks of a triangular solve's solution) are written
-// exactly once per solve into storage that held CG_SENT before the launch.
-// The writer stores them with relaxed agent-scope atomics and the reader polls
-// each value with relaxed agent-scope atomic loads until it differs from
-// CG_SENT: every location is written once, so per-location coherence is all
-// the hand-off needs (no flag, no release/acquire fences and their cache
-// write-back and invalidation).  CG_SENT is a NaN payload that arithmetic
-// never produces; a stored value with that bit pattern is replaced by the
-// default NaN.  The poll is bounded: on overrun it sets scal[CS_PIVOT] = 2
-// and returns 0.
-#define CG_SENT 0x7FF4C0DE5E9D1E00ull
-
-CG_DEV void cg_sent_store(CG_AS1 double* p, double v)
-{
-    unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    if (u == CG_SENT) u = 0x7FF8000000000000ull;
-    __hip_atomic_store((CG_AS1 unsigned long long*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-CG_DEV unsigned long long cg_sent_peek(const CG_AS1 double* p)
-{
-    return __hip_atomic_load((const CG_AS1 unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-CG_DEV double cg_sent_wait(const CG_AS1 double* p, unsigned long long u, CG_AS1 double* scal)
-{
-    int spins = 0;
-    while (u == CG_SENT) {
-        __builtin_amdgcn_s_sleep(1);
-        u = cg_sent_peek(p);
-        if (++spins > (1 << 25)) {
-            scal[CS_PIVOT] = 2.0;
-            return 0.0;
-        }
-    }
-    return __longlong_as_double((long long)u);
-}
-
-CG_DEV double cg_sent_load(const CG_AS1 double* p, CG_AS1 double* scal) { return cg_sent_wait(p, cg_sent_peek(p), scal); }
-
 // Zero the lower block tiles of S; the diagonal tiles also set their block of
 // Winv to CG_SENT (the factor's diagonal workgroup hands W over through it).
 __global__ __launch_bounds__(CG_WG) void k_chol_fill(const CgInst* __restrict__ insts, int B,

@@ -276,6 +276,7 @@ struct Batch {
     {
         for (int b = 0; b < B; b++) h[b].scal = d_scal + (size_t)b * CG_NSCAL;
         CG_HIP(hipMemcpyAsync(d_insts, h.data(), sizeof(CgInst) * B, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_wgpart_init, dim3(B), dim3(CG_WG), 0, stream, d_insts);
         int nt, nv, nr;
         write_maps(d_full, std::vector<char>(B, 1), nt, nv, nr);
         set_maps(d_full, nt, nv, nr);

@@ -65,6 +65,7 @@ __global__ void k_schur_back(const CgInst*, const int*, const double*);         
 __global__ void k_chol_fill(const CgInst*, int B, const int* pref);
 __global__ void k_chol_diag0(const CgInst*, int B, const int* pref);
 __global__ void k_chol_panel(const CgInst*, int B, const int* pref, int kb);
+__global__ void k_wgpart_init(const CgInst*);
 __global__ void k_chol_update(const CgInst*, int B, const int* pref, int seg_stride, int kb);
 __global__ void k_pc_rhs(const CgInst*, const int* wave_inst, const double* ctl);
 __global__ void k_pc_fwd(const CgInst*, const int* order, const int* lvl_pref, int nlvl, int target);

@@ -16,34 +16,6 @@
 #include "cgot_common.h"
 #include "cgot_kernels.h"
 
-template <int NV>
-CG_DEV bool cg_inst_reduce_p(const CgInstD& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
-{
-    __shared__ double lds4[4];
-    __shared__ double tot[NV];
-    __shared__ int is_last;
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = cg_block_sum(v[k], lds4);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) I.wgpart[(size_t)k * I.nvec + wgi] = v[k];
-        // release of this workgroup's partials and, in the last workgroup,
-        // acquire of everyone's (one acq_rel read-modify-write by one lane)
-        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == (unsigned int)(I.nvec - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    if (threadIdx.x < NV) {
-        tot[threadIdx.x] = cg_ordered_sum(I.wgpart + (size_t)threadIdx.x * I.nvec, I.nvec, 1);
-    }
-    if (threadIdx.x == 0) I.counter[slot] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; k++) out[k] = tot[k];
-    return true;
-}
-
 CG_DEV bool cg_pcg_decode(const CgInst* insts, const int* wg_inst, const double* ctl, int& b, int& wgi)
 {
     const int2 e = cg_map_entry(wg_inst);
@@ -90,7 +62,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_init(const CgInst* __restrict__ i
         if (q < m - 1) setk(n + q, a);
     }
     double tot[3];
-    if (cg_inst_reduce_p<3>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
+    if (cg_inst_reduce<3>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
         I.scal[PCG_BNORM] = sqrt(tot[0]);
         I.scal[PCG_RZ] = tot[1];
         I.scal[PCG_BETA] = 0.0;
@@ -130,7 +102,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ 
         v[0] += p * w;
     }
     double tot[1];
-    if (cg_inst_reduce_p<1>(I, wgi, v, tot, 1) && threadIdx.x == 0) {
+    if (cg_inst_reduce<1>(I, wgi, v, tot, 1) && threadIdx.x == 0) {
         if (tot[0] <= 0.0) {
             I.scal[PCG_STATUS] = PCG_BREAKDOWN;
         } else {
@@ -162,7 +134,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_update(const CgInst* __restrict__
         }
     }
     double tot[2];
-    if (cg_inst_reduce_p<2>(I, wgi, v, tot, 2) && threadIdx.x == 0) {
+    if (cg_inst_reduce<2>(I, wgi, v, tot, 2) && threadIdx.x == 0) {
         const double it = I.scal[PCG_ITER] + 1.0;
         I.scal[PCG_ITER] = it;
         const double tol = ctl[b * CG_NCTL + CTL_STEP];
@@ -212,7 +184,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_wdots(const CgInst* __restrict__ i
         v[2] = F.yv[k] * I.sv[k];
     }
     double tot[3];
-    if (cg_inst_reduce_p<3>(I, wgi, v, tot, 3) && threadIdx.x == 0) {
+    if (cg_inst_reduce<3>(I, wgi, v, tot, 3) && threadIdx.x == 0) {
         I.scal[CS_SG] = tot[0];
         I.scal[CS_YAY] = tot[1];
         I.scal[CS_YAG] = tot[2];
@@ -247,7 +219,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_rz(const CgInst* __restrict__ ins
     const int k = wgi * CG_WG + threadIdx.x;
     double v[1] = {(k < I.N) ? I.yv[k] * I.sv[k] : 0.0};
     double tot[1];
-    if (cg_inst_reduce_p<1>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
+    if (cg_inst_reduce<1>(I, wgi, v, tot, 0) && threadIdx.x == 0) {
         if (init) {
             I.scal[PCG_BETA] = 0.0;
         } else {

@@ -2,9 +2,8 @@
 // passes into row/column sums, duals, gradients and per-instance scalars, and
 // run the O(n+m) parts of the SPLR step.  Grid: the "vec" map, i.e.
 // ceil((n+m)/256) workgroups per instance; the map entry of a workgroup names the
-// instance.  Per-instance scalar reductions use a last-workgroup-done counter:
-// every workgroup stores its partial, the last one to arrive sums all partials
-// in workgroup order, so the result does not depend on scheduling.
+// instance.  Per-instance scalar reductions use cg_inst_reduce (last
+// workgroup sums the partials in workgroup order).
 #include "cgot_common.h"
 #include "cgot_kernels.h"
 
@@ -17,35 +16,14 @@ CG_DEV bool cg_decode_vec(const CgInst* insts, const int* wg_inst, const double*
     return true;
 }
 
-// Reduce NV per-thread values over all vec workgroups of the instance.  Returns
-// true in the last workgroup, where out[] holds the totals (all threads).
-template <int NV>
-CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&out)[NV], int slot)
+// wgpart of every instance to CG_SENT (once per solve, after the descriptors
+// are uploaded; cg_inst_reduce leaves it that way after each use).
+__global__ __launch_bounds__(CG_WG) void k_wgpart_init(const CgInst* __restrict__ insts)
 {
-    __shared__ double lds4[4];
-    __shared__ double tot[NV];
-    __shared__ int is_last;
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = cg_block_sum(v[k], lds4);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) I.wgpart[(size_t)k * I.nvec + wgi] = v[k];
-        // release of this workgroup's partials and, in the last workgroup,
-        // acquire of everyone's (one acq_rel read-modify-write by one lane)
-        const unsigned int prev = __hip_atomic_fetch_add(&I.counter[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == (unsigned int)(I.nvec - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    if (threadIdx.x < NV) {
-        const int k = threadIdx.x;
-        tot[k] = cg_ordered_sum(I.wgpart + (size_t)k * I.nvec, I.nvec, 1);
-    }
-    if (threadIdx.x == 0) I.counter[slot] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; k++) out[k] = tot[k];
-    return true;
+    const CgInstD& I = cg_inst(insts, blockIdx.x);
+    if (!I.wgpart) return;
+    auto p = reinterpret_cast<CG_AS1 unsigned long long*>(I.wgpart);
+    for (int e = threadIdx.x; e < 8 * I.nvec; e += CG_WG) p[e] = CG_SENT;
 }
 
 // ---------------------------------------------------------------------------
