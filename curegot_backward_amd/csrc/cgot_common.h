@@ -155,18 +155,73 @@ enum {
 // ---------------------------------------------------------------------------
 // wave64 / workgroup reductions (fixed order => deterministic)
 // ---------------------------------------------------------------------------
+// Broadcast lane `src` (wave-uniform) of a double.
+CG_DEV double cg_readlane_d(double v, int src)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// A double moved across lanes by the DPP control CTRL (within 16-lane rows).
+template <int CTRL>
+CG_DEV double cg_dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// Wave reductions, all 64 lanes active; the result is the same in every lane.
+// Inside each 16-lane row: DPP exchanges with lane^1 and lane^2 (quad
+// permutes), then with the half-row and row mirrors; the four row results are
+// combined by readlane.  No LDS traffic.
 CG_DEV double cg_wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += cg_dpp_d<0xB1>(v);
+    v += cg_dpp_d<0x4E>(v);
+    v += cg_dpp_d<0x141>(v);
+    v += cg_dpp_d<0x140>(v);
+    return (cg_readlane_d(v, 0) + cg_readlane_d(v, 16)) + (cg_readlane_d(v, 32) + cg_readlane_d(v, 48));
 }
 
 CG_DEV double cg_wave_max(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmax(v, cg_dpp_d<0xB1>(v));
+    v = fmax(v, cg_dpp_d<0x4E>(v));
+    v = fmax(v, cg_dpp_d<0x141>(v));
+    v = fmax(v, cg_dpp_d<0x140>(v));
+    return fmax(fmax(cg_readlane_d(v, 0), cg_readlane_d(v, 16)), fmax(cg_readlane_d(v, 32), cg_readlane_d(v, 48)));
+}
+
+// exp on the M passes: Tang's table-driven method (P. T. P. Tang, "Table-driven
+// implementation of the exponential function in IEEE floating-point
+// arithmetic", ACM TOMS 15(2), 1989) with 64 table entries.  y = (64q + j)
+// ln2/64 + r with |r| <= ln2/128; exp(y) = 2^q 2^(j/64) e^r, e^r - 1 by its
+// degree-5 Taylor polynomial (truncation below 4e-17 relative).  ln2/64 is
+// split in two parts so that r is exact to rounding.  tab[j] = 2^(j/64) sits
+// in LDS (CG_EXP_TABLE).  For finite y; exp of large negative y underflows to
+// 0 through the final ldexp, of large positive y overflows to +inf.
+#define CG_EXP_TABLE(tab)                                                       \
+    __shared__ double tab[64];                                                  \
+    if (threadIdx.x < 64) tab[threadIdx.x] = exp2((double)threadIdx.x * 0.015625); \
+    __syncthreads()
+
+CG_DEV double cg_texp(double y, const double* tab)
+{
+    const double kf = __builtin_rint(y * 92.332482616893656877);   // 64/ln2
+    const int k = (int)kf;
+    double r = fma(-kf, 0.010830424696223417, y);                  // ln2/64, leading bits
+    r = fma(-kf, 2.572804622327669e-14, r);                        // ln2/64, remainder
+    double p = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma(r, p, 1.0 / 6.0);
+    p = fma(r, p, 0.5);
+    p = fma(r, p, 1.0);
+    p = r * p;
+    const double t = tab[k & 63];
+    return __builtin_amdgcn_ldexp(fma(t, p, t), k >> 6);
 }
 
 // Sum over a 256-thread workgroup; result valid in every thread.

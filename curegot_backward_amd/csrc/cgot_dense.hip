@@ -321,15 +321,6 @@ __global__ __launch_bounds__(CG_WG) void k_schur_back(const CgInst* __restrict__
 // ---------------------------------------------------------------------------
 // Blocked Cholesky
 // ---------------------------------------------------------------------------
-// Broadcast lane `src` (wave-uniform) of a double.
-CG_DEV double cg_readlane_d(double v, int src)
-{
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
-    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
 // sqrt(x) and 1/sqrt(x) for x > 0: the hardware reciprocal-square-root
 // estimate refined by two steps of the coupled Goldschmidt iteration
 // (g -> sqrt(x), h -> 1/(2 sqrt(x)); each step squares the relative error).

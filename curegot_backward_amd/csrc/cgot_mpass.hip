@@ -43,7 +43,8 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
 // whole group before the exponentials, so R*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
 template <int C>
-CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, double (*lds)[CG_CMAX * 64])
+CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, double (*lds)[CG_CMAX * 64],
+                          const double* tab)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
@@ -78,7 +79,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
             double rs = 0.0;
 #pragma unroll
             for (int k = 0; k < C; k++) {
-                const double v = ok[k] ? exp((ai[r] + bj[k] - mv[r][k]) * ir) : 0.0;
+                const double v = ok[k] ? cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab) : 0.0;
                 rs += v;
                 if (i < n) cs[k] += v;
             }
@@ -99,63 +100,88 @@ __global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ inst
                                                  const int* __restrict__ tile_inst,
                                                  const double* __restrict__ ctl, int mode)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
     switch (I.c) {
-        case 1: cg_tpass_tile<1>(I, mode, t, rb, st, lds); break;
-        case 2: cg_tpass_tile<2>(I, mode, t, rb, st, lds); break;
-        default: cg_tpass_tile<CG_CMAX>(I, mode, t, rb, st, lds); break;
+        case 1: cg_tpass_tile<1>(I, mode, t, rb, st, lds, tab); break;
+        case 2: cg_tpass_tile<2>(I, mode, t, rb, st, lds, tab); break;
+        default: cg_tpass_tile<CG_CMAX>(I, mode, t, rb, st, lds, tab); break;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Row log-sum-exp partials of (beta_j - M_ij)/reg for the alpha update.
 // Reference: optimal_alpha_kernel, src/sinkhorn_bcd_kernel.cu:115-192.
-// Per (row, strip): wave max, then sum of exp(x - max).
+// Per (row, strip): wave max, then sum of exp(x - max).  Each wave takes its
+// CG_RPW rows in groups of R with the loads of a whole group issued first.
 // ---------------------------------------------------------------------------
+template <int C>
+CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw;
+    const int m = I.m, n = I.n;
+    const double ir = I.inv_reg;
+    const auto beta = (mode == CG_PT_BCD) ? (I.gamma_bcd + n) : (I.gamma + n);
+    double bj[C];
+    bool ok[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int j = s0 + lane + 64 * k;
+        ok[k] = j < m;
+        bj[k] = ok[k] ? beta[j] * ir : 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    constexpr int R = (C >= 4) ? 4 : 8;   // R*C loads per lane in flight
+#pragma unroll
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        double mv[R][C];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = i0 + r0 + r;
+            double x[C], mx = -INFINITY;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                x[k] = ok[k] ? fma(-mv[r][k], ir, bj[k]) : -INFINITY;
+                mx = fmax(mx, x[k]);
+            }
+            mx = cg_wave_max(mx);
+            double sm = 0.0;
+#pragma unroll
+            for (int k = 0; k < C; k++)
+                if (ok[k]) sm += cg_texp(x[k] - mx, tab);
+            sm = cg_wave_sum(sm);
+            if (lane == 0 && i < n) {
+                I.rp0[(size_t)i * I.nstrips + st] = mx;
+                I.rp1[(size_t)i * I.nstrips + st] = sm;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInstD I = cg_inst(insts, b);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int s0 = st * I.sw;
-    const int m = I.m, n = I.n;
-    const auto beta = (mode == CG_PT_BCD) ? (I.gamma_bcd + n) : (I.gamma + n);
-    double bj[CG_CMAX];
-#pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) {
-        const int j = s0 + lane + 64 * k;
-        bj[k] = (k < I.c && j < m) ? beta[j] : 0.0;
-    }
-    const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= n) break;
-        const auto Mi = I.M + (size_t)i * m;
-        double x[CG_CMAX];
-        double mx = -INFINITY;
-#pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            x[k] = (k < I.c && j < m) ? (bj[k] - Mi[j]) * I.inv_reg : -INFINITY;
-            mx = fmax(mx, x[k]);
-        }
-        mx = cg_wave_max(mx);
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < CG_CMAX; k++)
-            if (x[k] != -INFINITY) s += exp(x[k] - mx);
-        s = cg_wave_sum(s);
-        if (lane == 0) {
-            I.rp0[(size_t)i * I.nstrips + st] = mx;
-            I.rp1[(size_t)i * I.nstrips + st] = s;
-        }
+    const CgInstD& I = cg_inst(insts, b);
+    switch (I.c) {
+        case 1: cg_rowlse_tile<1>(I, mode, rb, st, tab); break;
+        case 2: cg_rowlse_tile<2>(I, mode, rb, st, tab); break;
+        default: cg_rowlse_tile<CG_CMAX>(I, mode, rb, st, tab); break;
     }
 }
 
@@ -163,46 +189,57 @@ __global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ in
 // Column log-sum-exp partials of (alpha_i - M_ij)/reg for the beta update.
 // Reference: optimal_beta_kernel, src/sinkhorn_bcd_kernel.cu:35-112 (strided
 // column walk there; here each lane streams its columns down the row block
-// with an online max/sum, reads stay row-major and coalesced).
+// with an online max/sum, reads stay row-major and coalesced, R rows' loads
+// in flight at a time).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ insts,
-                                                   const int* __restrict__ tile_inst,
-                                                   const double* __restrict__ ctl, int mode)
+template <int C>
+CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab,
+                           double (*lmx)[CG_CMAX * 64], double (*lsm)[CG_CMAX * 64])
 {
-    int b, rb, st;
-    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInstD I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
+    const double ir = I.inv_reg;
     const auto alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
-    double mx[CG_CMAX], sm[CG_CMAX];
+    double mx[C], sm[C];
+    bool ok[C];
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) { mx[k] = -INFINITY; sm[k] = 0.0; }
+    for (int k = 0; k < C; k++) {
+        mx[k] = -INFINITY;
+        sm[k] = 0.0;
+        ok[k] = s0 + lane + 64 * k < m;
+    }
     const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= n) break;
-        const double ai = alpha[i];
-        const auto Mi = I.M + (size_t)i * m;
+    constexpr int R = (C >= 4) ? 4 : 8;
 #pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < m) {
-                const double x = (ai - Mi[j]) * I.inv_reg;
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        double mv[R][C], ai[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = min(i0 + r0 + r, n - 1);
+            ai[r] = alpha[i] * ir;
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (i0 + r0 + r >= n) break;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                if (!ok[k]) continue;
+                const double x = fma(-mv[r][k], ir, ai[r]);
                 if (x > mx[k]) {
-                    sm[k] = (mx[k] == -INFINITY) ? 1.0 : sm[k] * exp(mx[k] - x) + 1.0;
+                    sm[k] = (mx[k] == -INFINITY) ? 1.0 : fma(sm[k], cg_texp(mx[k] - x, tab), 1.0);
                     mx[k] = x;
                 } else {
-                    sm[k] += exp(x - mx[k]);
+                    sm[k] += cg_texp(x - mx[k], tab);
                 }
             }
         }
     }
-    __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
-    __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) { lmx[w][lane + 64 * k] = mx[k]; lsm[w][lane + 64 * k] = sm[k]; }
+    for (int k = 0; k < C; k++) { lmx[w][lane + 64 * k] = mx[k]; lsm[w][lane + 64 * k] = sm[k]; }
     __syncthreads();
     for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
         const int j = s0 + q;
@@ -214,6 +251,23 @@ __global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ in
             I.cp0[(size_t)rb * m + j] = a;
             I.cp1[(size_t)rb * m + j] = s;
         }
+    }
+}
+
+__global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ tile_inst,
+                                                   const double* __restrict__ ctl, int mode)
+{
+    CG_EXP_TABLE(tab);
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
+    __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
+    switch (I.c) {
+        case 1: cg_collse_tile<1>(I, mode, rb, st, tab, lmx, lsm); break;
+        case 2: cg_collse_tile<2>(I, mode, rb, st, tab, lmx, lsm); break;
+        default: cg_collse_tile<CG_CMAX>(I, mode, rb, st, tab, lmx, lsm); break;
     }
 }
 
@@ -232,6 +286,7 @@ __global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
                                               const double* const* __restrict__ vecs,
                                               const double* const* __restrict__ Gs, int mode)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD I = cg_inst(insts, b);
@@ -268,7 +323,7 @@ __global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
             if (k < I.c && j < m) {
-                const double tv = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
+                const double tv = cg_texp((ai + bj[k] - Mi[j]) * I.inv_reg, tab);
                 if (mv) {
                     rs += tv * vb[k];
                     cs[k] += tv * va;
@@ -320,6 +375,7 @@ __global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts
                                                 const int* __restrict__ tile_inst,
                                                 const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD I = cg_inst(insts, b);
@@ -344,7 +400,7 @@ __global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < m) Pi[j] = exp((ai + bj[k] - Mi[j]) * I.inv_reg);
+            if (k < I.c && j < m) Pi[j] = cg_texp((ai + bj[k] - Mi[j]) * I.inv_reg, tab);
         }
     }
 }
@@ -358,6 +414,7 @@ __global__ __launch_bounds__(CG_WG) void k_grad_assemble(const CgInst* __restric
                                                          const double* const* __restrict__ Gs,
                                                          double* const* __restrict__ outs)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD I = cg_inst(insts, b);
@@ -386,7 +443,7 @@ __global__ __launch_bounds__(CG_WG) void k_grad_assemble(const CgInst* __restric
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
             if (k < I.c && j < m) {
-                const double tv = exp((ai + bj[k] - I.M[row + j]) * I.inv_reg);
+                const double tv = cg_texp((ai + bj[k] - I.M[row + j]) * I.inv_reg, tab);
                 out[row + j] = (tv * (la + lb[k] - G[row + j])) / I.reg;
             }
         }

@@ -10,9 +10,9 @@
 #include "cgot_common.h"
 #include "cgot_kernels.h"
 
-CG_DEV double cg_T(const CgInstD& I, int i, int j)
+CG_DEV double cg_T(const CgInstD& I, int i, int j, const double* tab)
 {
-    return exp((I.gamma[i] + I.gamma[I.n + j] - I.M[(size_t)i * I.m + j]) * I.inv_reg);
+    return cg_texp((I.gamma[i] + I.gamma[I.n + j] - I.M[(size_t)i * I.m + j]) * I.inv_reg, tab);
 }
 
 CG_DEV unsigned long long cg_kth(const CgInstD& I) { return (unsigned long long)__double_as_longlong(I.scal[CS_KTH_HI]); }
@@ -36,6 +36,7 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
                                                 const int* __restrict__ tile_inst,
                                                 const double* __restrict__ ctl, int pass)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
         for (int k = 0; k < I.c; k++) {
             const int j = s0 + lane + 64 * k;
             if (j < mm) {
-                const unsigned long long key = cg_key(cg_T(I, i, j));
+                const unsigned long long key = cg_key(cg_T(I, i, j, tab));
                 if (pass == 0 || (key >> (shift + CG_HBITS)) == prefix)
                     atomicAdd(&h[(key >> shift) & mask], 1u);
             }
@@ -127,6 +128,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restri
                                                           const int* __restrict__ row_inst,
                                                           const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     const int2 e = cg_map_entry(row_inst);
     const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restri
     const int mm = I.m - 1;
     int gt = 0, tie = 0;
     for (int j = lane; j < mm; j += 64) {
-        const unsigned long long key = cg_key(cg_T(I, i, j));
+        const unsigned long long key = cg_key(cg_T(I, i, j, tab));
         gt += (key > kth);
         tie += (key == kth);
     }
@@ -204,6 +206,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restri
                                                           const int* __restrict__ row_inst,
                                                           const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     const int2 e = cg_map_entry(row_inst);
     const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restri
     for (int j0 = 0; j0 < mm; j0 += 64) {
         const int j = j0 + lane;
         const bool ok = j < mm;
-        const unsigned long long key = ok ? cg_key(cg_T(I, i, j)) : 0ull;
+        const unsigned long long key = ok ? cg_key(cg_T(I, i, j, tab)) : 0ull;
         const bool isgt = ok && key > kth;
         const bool istie = ok && key == kth;
         const unsigned long long tmask = __ballot(istie);
@@ -249,6 +252,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
                                                           const int* __restrict__ tile_inst,
                                                           const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < mm) cnt[k] += cg_selected(cg_key(cg_T(I, i, j)), kth, j, cut);
+            if (k < I.c && j < mm) cnt[k] += cg_selected(cg_key(cg_T(I, i, j, tab)), kth, j, cut);
         }
     }
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
@@ -318,6 +322,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
                                                           const int* __restrict__ tile_inst,
                                                           const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     int b, rb, st;
     if (!cg_decode_tile2(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
@@ -335,7 +340,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < mm && cg_selected(cg_key(cg_T(I, i, j)), kth, j, cut)) bits[k] |= (1u << r);
+            if (k < I.c && j < mm && cg_selected(cg_key(cg_T(I, i, j, tab)), kth, j, cut)) bits[k] |= (1u << r);
         }
     }
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
@@ -363,6 +368,7 @@ __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restri
                                                           const int* __restrict__ wave_inst,
                                                           const double* __restrict__ ctl)
 {
+    CG_EXP_TABLE(tab);
     const int2 e = cg_map_entry(wave_inst);
     const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
@@ -371,11 +377,11 @@ __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restri
     const int lane = threadIdx.x & 63;
     const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
     if (q < I.n) {
-        for (int p = I.w_rowptr[q] + lane; p < I.w_rowptr[q + 1]; p += 64) I.w_rval[p] = cg_T(I, q, I.w_colj[p]);
+        for (int p = I.w_rowptr[q] + lane; p < I.w_rowptr[q + 1]; p += 64) I.w_rval[p] = cg_T(I, q, I.w_colj[p], tab);
         if (lane == 0) I.Da[q] = I.rowsum[q] + sh;
     } else if (q < I.N) {
         const int j = q - I.n;
-        for (int p = I.w_colptr[j] + lane; p < I.w_colptr[j + 1]; p += 64) I.w_cval[p] = cg_T(I, I.w_rowi[p], j);
+        for (int p = I.w_colptr[j] + lane; p < I.w_colptr[j + 1]; p += 64) I.w_cval[p] = cg_T(I, I.w_rowi[p], j, tab);
         if (lane == 0) I.Db[j] = I.colsum[j] + sh;
     }
 }
