@@ -8,6 +8,12 @@ from __future__ import annotations
 import ctypes
 import os
 
+# torch first: its bundled HIP runtime (soname libamdhip64.so.7) is then the
+# one the library binds to.  Loading the library first would bring in the
+# system runtime under that soname and torch would load a second runtime
+# beside it, and the two do not share devices.
+import torch  # noqa: F401
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcuregot_amd.so")
 

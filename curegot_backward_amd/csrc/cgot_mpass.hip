@@ -60,9 +60,10 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
     const double ir = I.inv_reg;
-    // rows per load group: 4*C loads per lane in flight at C <= 4, 2*C at C = 8
-    constexpr int R = (C >= 8) ? 2 : 4;
-#pragma unroll
+    // rows per load group: R*C loads per lane in flight; the group loop stays
+    // rolled so that only one group's loads hold registers
+    constexpr int R = (C >= 4) ? 2 : 4;
+#pragma unroll 1
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C], ai[R];
 #pragma unroll
@@ -210,7 +211,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
         ok[k] = s0 + lane + 64 * k < m;
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
-    constexpr int R = (C >= 4) ? 4 : 8;
+    constexpr int R = (C >= 4) ? 2 : 4;
 #pragma unroll
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C], ai[R];
