@@ -339,8 +339,37 @@ CG_DEV void cg_sent_reset(CG_AS1 double* p)
     __hip_atomic_store((CG_AS1 unsigned long long*)p, CG_SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Ordered sum of p[0..count) as each value arrives, resetting every slot to
-// CG_SENT for the next use (CG_SUM_U loads in flight).
+// Completes NQ hand-off reads of one thread: u[q] holds a first read of slot
+// addr(q) (0 for slots not read); every slot still at CG_SENT is read again,
+// all of them in flight together, until none is left (bounded like
+// cg_sent_wait).  Values handed over by another workgroup land in a burst, so
+// once the first has arrived the rest come in one more round trip instead of
+// one round trip each.
+template <int NQ, class Addr>
+CG_DEV void cg_sent_gather(unsigned long long (&u)[NQ], Addr addr, CG_AS1 double* scal)
+{
+    int spins = 0;
+    for (;;) {
+        bool pend = false;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) pend = pend || (u[q] == CG_SENT);
+        if (!pend) return;
+        if (++spins > (1 << 25)) {
+            scal[CS_PIVOT] = 2.0;
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (u[q] == CG_SENT) u[q] = 0ull;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            if (u[q] == CG_SENT) u[q] = cg_sent_peek(addr(q));
+    }
+}
+
+// Ordered sum of p[0..count) once the values have arrived, resetting every
+// slot to CG_SENT for the next use (CG_SUM_U loads in flight).
 CG_DEV double cg_ordered_take(CG_AS1 double* p, int count, CG_AS1 double* scal)
 {
     double s = 0.0;
@@ -348,10 +377,11 @@ CG_DEV double cg_ordered_take(CG_AS1 double* p, int count, CG_AS1 double* scal)
         unsigned long long t[CG_SUM_U];
 #pragma unroll
         for (int u = 0; u < CG_SUM_U; u++) t[u] = (k + u < count) ? cg_sent_peek(p + k + u) : 0ull;
+        cg_sent_gather(t, [&](int u) { return p + k + u; }, scal);
 #pragma unroll
         for (int u = 0; u < CG_SUM_U; u++)
             if (k + u < count) {
-                s += cg_sent_wait(p + k + u, t[u], scal);
+                s += __longlong_as_double((long long)t[u]);
                 cg_sent_reset(p + k + u);
             }
     }

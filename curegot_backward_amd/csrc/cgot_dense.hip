@@ -723,16 +723,18 @@ __global__ __launch_bounds__(CG_WG) void k_chol_update(const CgInst* __restrict_
         return;
     }
     // next panel: X = C W^T, W = W_{kb+1} once handed over (all of a
-    // thread's loads in flight, then the ones still at CG_SENT are polled)
+    // thread's loads in flight, then the ones still at CG_SENT are read again
+    // together)
     {
         const auto Wd = I.Winv + (size_t)(kb + 1) * CG_NB * CG_NB;
         unsigned long long u[CG_NB * CG_NB / CG_WG];
 #pragma unroll
         for (int q = 0; q < CG_NB * CG_NB / CG_WG; q++) u[q] = cg_sent_peek(Wd + threadIdx.x + CG_WG * q);
+        cg_sent_gather(u, [&](int q) { return Wd + threadIdx.x + CG_WG * q; }, I.scal);
 #pragma unroll
         for (int q = 0; q < CG_NB * CG_NB / CG_WG; q++) {
             const int e = threadIdx.x + CG_WG * q;
-            Bs[e / CG_NB][e % CG_NB] = cg_sent_wait(Wd + e, u[q], I.scal);
+            Bs[e / CG_NB][e % CG_NB] = __longlong_as_double((long long)u[q]);
         }
     }
     __syncthreads();
