@@ -169,8 +169,10 @@ template <int CTRL>
 CG_DEV double cg_dpp_d(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    // every control used here is a full permutation inside each row, so no
+    // lane keeps its old value and none needs initialising
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
@@ -223,6 +225,33 @@ CG_DEV double cg_texp(double y, const double* tab)
     const double t = tab[k & 63];
     return __builtin_amdgcn_ldexp(fma(t, p, t), k >> 6);
 }
+
+// cg_texp for -745 <= y <= 710 only: k = y*64/ln2 rounded to nearest comes
+// from one fma, as adding 1.5*2^52 leaves k in the low word (two's
+// complement) and k - 1.5*2^52 exactly representable (|k| < 2^17 here).
+CG_DEV double cg_texp_m(double y, const double* tab)
+{
+    const double sh = fma(y, 92.332482616893656877, 6755399441055744.0);   // 64/ln2 and 1.5*2^52
+    const int k = __double2loint(sh);
+    const double kf = sh - 6755399441055744.0;
+    double r = fma(-kf, 0.010830424696223417, y);
+    r = fma(-kf, 2.572804622327669e-14, r);
+    double p = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma(r, p, 1.0 / 6.0);
+    p = fma(r, p, 0.5);
+    p = fma(r, p, 1.0);
+    p = r * p;
+    const double t = tab[k & 63];
+    return __builtin_amdgcn_ldexp(fma(t, p, t), k >> 6);
+}
+
+// exp(y) for y <= 0, -inf included: below -745 exp is under half the
+// smallest subnormal, and the clamped argument gives 0 or that subnormal.
+CG_DEV double cg_texp_lo(double y, const double* tab) { return cg_texp_m(fmax(y, -745.0), tab); }
+
+// exp(y) for any y, -inf included: 0 or the smallest subnormal below -745,
+// +inf above 710 (exp overflows from 709.79 on).
+CG_DEV double cg_texp_c(double y, const double* tab) { return cg_texp_m(fmin(fmax(y, -745.0), 710.0), tab); }
 
 // Sum over a 256-thread workgroup; result valid in every thread.
 CG_DEV double cg_block_sum(double v, double* lds4)

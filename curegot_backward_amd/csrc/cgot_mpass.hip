@@ -8,6 +8,9 @@
 //   row partials  rp[i*nstrips + strip]   (one per row and strip)
 //   col partials  cp[rowblock*m + j]      (one per row block and column)
 // so the finalize step sums a fixed number of partials in a fixed order.
+// The T pass and the log-sum-exp kernels are held to 64 VGPRs (8 waves per
+// SIMD): these streams are latency-bound, and the waves in flight hide the
+// HBM latency.
 #include "cgot_common.h"
 #include "cgot_kernels.h"
 
@@ -78,11 +81,19 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
         for (int r = 0; r < R; r++) {
             const int i = i0 + r0 + r;
             double rs = 0.0;
+            // columns past m (mv = bj = 0 there) take a finite argument and
+            // are zeroed after the exp, so no lane branches
+            double v[C];
+#pragma unroll
+            for (int k = 0; k < C; k++) v[k] = cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab);
 #pragma unroll
             for (int k = 0; k < C; k++) {
-                const double v = ok[k] ? cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab) : 0.0;
-                rs += v;
-                if (i < n) cs[k] += v;
+                v[k] = ok[k] ? v[k] : 0.0;
+                rs += v[k];
+            }
+            if (i < n) {
+#pragma unroll
+                for (int k = 0; k < C; k++) cs[k] += v[k];
             }
             rs = cg_wave_sum(rs);
             if (lane == 0 && i < n) I.rp0[(size_t)i * I.nstrips + st] = rs;
@@ -97,7 +108,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
     }
 }
 
-__global__ __launch_bounds__(CG_WG) void k_tpass(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_tpass(const CgInst* __restrict__ insts,
                                                  const int* __restrict__ tile_inst,
                                                  const double* __restrict__ ctl, int mode)
 {
@@ -158,10 +169,14 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
                 mx = fmax(mx, x[k]);
             }
             mx = cg_wave_max(mx);
-            double sm = 0.0;
+            // x = -inf past m: cg_texp_lo gives 0 (or the smallest subnormal,
+            // far below the rounding of a sum that contains exp(0) = 1)
+            double e[C];
 #pragma unroll
-            for (int k = 0; k < C; k++)
-                if (ok[k]) sm += cg_texp(x[k] - mx, tab);
+            for (int k = 0; k < C; k++) e[k] = cg_texp_lo(x[k] - mx, tab);
+            double sm = e[0];
+#pragma unroll
+            for (int k = 1; k < C; k++) sm += e[k];
             sm = cg_wave_sum(sm);
             if (lane == 0 && i < n) {
                 I.rp0[(size_t)i * I.nstrips + st] = mx;
@@ -171,7 +186,7 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
     }
 }
 
-__global__ __launch_bounds__(CG_WG) void k_row_lse(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_row_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
 {
@@ -223,19 +238,20 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
             for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
         }
+        // online update without lane branches: e = exp(-|x - mx|) serves
+        // both cases (a new maximum rescales the sum by e and adds 1, any
+        // other x adds e); the first row meets mx = -inf, gets e = 0 and
+        // sum 1.  Columns past m compute on mv = 0 and are never stored.
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (i0 + r0 + r >= n) break;
 #pragma unroll
             for (int k = 0; k < C; k++) {
-                if (!ok[k]) continue;
                 const double x = fma(-mv[r][k], ir, ai[r]);
-                if (x > mx[k]) {
-                    sm[k] = (mx[k] == -INFINITY) ? 1.0 : fma(sm[k], cg_texp(mx[k] - x, tab), 1.0);
-                    mx[k] = x;
-                } else {
-                    sm[k] += cg_texp(x - mx[k], tab);
-                }
+                const double d = x - mx[k];
+                const double e = cg_texp_lo(-fabs(d), tab);
+                sm[k] = (d > 0.0) ? fma(sm[k], e, 1.0) : sm[k] + e;
+                mx[k] = fmax(mx[k], x);
             }
         }
     }
@@ -255,7 +271,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
     }
 }
 
-__global__ __launch_bounds__(CG_WG) void k_col_lse(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_col_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
 {
