@@ -1211,8 +1211,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
     double* const* outs = (double* const*)(bt.d_ptrs + 2 * B);
     PcSolver pc(bt);
     auto precondition = [&](const double* ctlp, int init) { pc.apply(ctlp, init); };
-    hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc,
-                       (const double* const*)nullptr, Gs, (int)CG_QV_RHS);
+    hipLaunchKernelGGL(k_qv_rhs, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc, Gs);
     bt.vec0(k_pcg_init, dc);
     precondition(dc, 1);
     bt.check_launch();
@@ -1231,8 +1230,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         const double* d2 = bt.ctl(c2);
         for (int k = 0; k < chunk; k++) {
             ProfScope ps(bt.stream, PROF_QV, bt.mpass_bytes(bt.hmap[d2], 24.0));
-            hipLaunchKernelGGL(k_qv, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, d2,
-                               (const double* const*)nullptr, (const double* const*)nullptr, (int)CG_QV_PCG);
+            hipLaunchKernelGGL(k_qv_pcg, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, d2);
             bt.vec0(k_pcg_q_fin, d2);
             bt.vec0(k_pcg_update, d2);
             precondition(d2, 0);

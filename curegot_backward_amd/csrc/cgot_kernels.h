@@ -16,14 +16,13 @@ enum {
 };
 
 enum { CG_PT_GAMMA = 0, CG_PT_TRIAL = 1, CG_PT_BCD = 2 };
-enum { CG_QV_MATVEC = 0, CG_QV_RHS = 1, CG_QV_SUMS = 2, CG_QV_PCG = 3 };
 
 // ---- M passes (grid = total tiles; tile_inst maps tile -> instance) ----
 __global__ void k_tpass(const CgInst*, const int*, const double*, int mode);
 __global__ void k_row_lse(const CgInst*, const int*, const double*, int mode);
 __global__ void k_col_lse(const CgInst*, const int*, const double*, int mode);
-__global__ void k_qv(const CgInst*, const int*, const double*, const double* const*,
-                     const double* const*, int mode);
+__global__ void k_qv_rhs(const CgInst*, const int*, const double*, const double* const* Gs);
+__global__ void k_qv_pcg(const CgInst*, const int*, const double*);
 __global__ void k_plan(const CgInst*, const int*, const double*);
 __global__ void k_grad_assemble(const CgInst*, const int*, const double*, const double* const*,
                                 const double* const*, double* const*);

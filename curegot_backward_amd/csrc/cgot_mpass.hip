@@ -25,6 +25,14 @@ CG_DEV double cg_point(const CgInstD& I, int mode, double t, int k)
     return (k < I.N) ? (I.gamma_prev[k] + t * I.dir[k]) : 0.0;
 }
 
+// Columns per lane for strip st: the instance's c, or fewer for a last strip
+// that is narrower (its idle lanes would otherwise still issue the work).
+CG_DEV int cg_strip_c(const CgInstD& I, int st)
+{
+    const int cw = min(I.sw, I.m - st * I.sw);
+    return (cw <= 64) ? 1 : (cw <= 128) ? 2 : CG_CMAX;
+}
+
 CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const double* ctl,
                            int& b, int& rb, int& st)
 {
@@ -118,7 +126,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
-    switch (I.c) {
+    switch (cg_strip_c(I, st)) {
         case 1: cg_tpass_tile<1>(I, mode, t, rb, st, lds, tab); break;
         case 2: cg_tpass_tile<2>(I, mode, t, rb, st, lds, tab); break;
         default: cg_tpass_tile<CG_CMAX>(I, mode, t, rb, st, lds, tab); break;
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
-    switch (I.c) {
+    switch (cg_strip_c(I, st)) {
         case 1: cg_rowlse_tile<1>(I, mode, rb, st, tab); break;
         case 2: cg_rowlse_tile<2>(I, mode, rb, st, tab); break;
         default: cg_rowlse_tile<CG_CMAX>(I, mode, rb, st, tab); break;
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
     __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
-    switch (I.c) {
+    switch (cg_strip_c(I, st)) {
         case 1: cg_collse_tile<1>(I, mode, rb, st, tab, lmx, lsm); break;
         case 2: cg_collse_tile<2>(I, mode, rb, st, tab, lmx, lsm); break;
         default: cg_collse_tile<CG_CMAX>(I, mode, rb, st, tab, lmx, lsm); break;
@@ -289,98 +297,145 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint pass with T recomputed from the duals (gamma):
-//   CG_QV_MATVEC: row partial sum_j T_ij v_beta_j (j < m-1), col partial sum_i T_ij v_alpha_i
-//   CG_QV_PCG:    the same with v = z + beta*p_prev formed on the fly (PCG direction)
-//   CG_QV_RHS:    rp0/cp0 = sums of G_ij T_ij, rp1/cp1 = sums of T_ij
-//   CG_QV_SUMS:   rp0/cp0 = sums of T_ij
+// Adjoint passes with T recomputed from the duals (gamma):
+//   k_qv_pcg: row partial sum_j T_ij v_beta_j (j < m-1), col partial
+//             sum_i T_ij v_alpha_i, with v = z + beta*p_prev (the PCG
+//             direction) formed on the fly
+//   k_qv_rhs: rp0/cp0 = sums of G_ij T_ij, rp1/cp1 = sums of T_ij
 // Reference: q_matvec / rhs_Q_scale, curegot/_backward_core.py:31-45 (3-4 torch
 // passes over a stored T there, one pass over M here).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CG_WG) void k_qv(const CgInst* __restrict__ insts,
-                                              const int* __restrict__ tile_inst,
-                                              const double* __restrict__ ctl,
-                                              const double* const* __restrict__ vecs,
-                                              const double* const* __restrict__ Gs, int mode)
+template <int C, bool RHS>
+CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
+                       const double* tab)
 {
-    CG_EXP_TABLE(tab);
-    int b, rb, st;
-    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
-    const CgInstD I = cg_inst(insts, b);
-    if (mode == CG_QV_PCG && I.scal[PCG_STATUS] != 0.0) return;
+    constexpr bool MV = !RHS;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
     const int m = I.m, n = I.n;
-    const bool mv = (mode == CG_QV_MATVEC || mode == CG_QV_PCG);
-    const CG_AS1 double* v = (mode == CG_QV_MATVEC) ? (const CG_AS1 double*)vecs[b] : nullptr;
-    const double pbeta = (mode == CG_QV_PCG) ? I.scal[PCG_BETA] : 0.0;
-    const CG_AS1 double* G = (mode == CG_QV_RHS) ? (const CG_AS1 double*)Gs[b] : nullptr;
-    auto vat = [&](int k) -> double {
-        return (mode == CG_QV_PCG) ? (I.sv[k] + pbeta * I.uv[k]) : v[k];
-    };
-    double bj[CG_CMAX], vb[CG_CMAX], cs[CG_CMAX], ct[CG_CMAX];
+    const double ir = I.inv_reg;
+    const double pbeta = MV ? I.scal[PCG_BETA] : 0.0;
+    const CG_AS1 double* G = RHS ? (const CG_AS1 double*)Gs[b] : nullptr;
+    auto vat = [&](int k) -> double { return I.sv[k] + pbeta * I.uv[k]; };
+    double bj[C], vb[C], cs[C], ct[C];
+    bool ok[C];
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) {
+    for (int k = 0; k < C; k++) {
         const int j = s0 + lane + 64 * k;
-        const bool ok = (k < I.c && j < m);
-        bj[k] = ok ? I.gamma[n + j] : 0.0;
-        vb[k] = (ok && mv && j < m - 1) ? vat(n + j) : 0.0;
+        ok[k] = j < m;
+        bj[k] = ok[k] ? I.gamma[n + j] : 0.0;
+        vb[k] = (MV && ok[k] && j < m - 1) ? vat(n + j) : 0.0;   // beta_{m-1} is fixed
         cs[k] = 0.0;
         ct[k] = 0.0;
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= n) break;
-        const double ai = I.gamma[i];
-        const double va = mv ? vat(i) : 0.0;
-        const auto Mi = I.M + (size_t)i * m;
-        double rs = 0.0, rt = 0.0;
+    // R rows' loads in flight (RHS loads G beside M); the group loop stays
+    // rolled (one group's loads hold registers at a time)
+    constexpr int R = (C >= 4) ? 1 : 2;
+#pragma unroll 1
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        double mv[R][C], gv[RHS ? R : 1][C], ai[R], va[R];
 #pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < m) {
-                const double tv = cg_texp((ai + bj[k] - Mi[j]) * I.inv_reg, tab);
-                if (mv) {
-                    rs += tv * vb[k];
-                    cs[k] += tv * va;
-                } else if (mode == CG_QV_RHS) {
-                    const double gt = G[(size_t)i * m + j] * tv;
-                    rs += gt;
-                    cs[k] += gt;
-                    rt += tv;
-                    ct[k] += tv;
-                } else {
-                    rs += tv;
-                    cs[k] += tv;
-                }
+        for (int r = 0; r < R; r++) {
+            const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
+            ai[r] = I.gamma[i];
+            va[r] = MV ? vat(i) : 0.0;
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            if (RHS) {
+                const auto Gi = G + (size_t)i * m + s0 + lane;
+#pragma unroll
+                for (int k = 0; k < C; k++) gv[RHS ? r : 0][k] = ok[k] ? Gi[64 * k] : 0.0;
             }
         }
-        rs = cg_wave_sum(rs);
-        if (mode == CG_QV_RHS) rt = cg_wave_sum(rt);
-        if (lane == 0) {
-            I.rp0[(size_t)i * I.nstrips + st] = rs;
-            if (mode == CG_QV_RHS) I.rp1[(size_t)i * I.nstrips + st] = rt;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = i0 + r0 + r;
+            const bool live = i < n;
+            // columns past m take a finite argument and are zeroed after the exp
+            double tv[C];
+#pragma unroll
+            for (int k = 0; k < C; k++) tv[k] = cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab);
+#pragma unroll
+            for (int k = 0; k < C; k++) tv[k] = ok[k] ? tv[k] : 0.0;
+            double rs = 0.0, rt = 0.0;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                if (MV) {
+                    rs += tv[k] * vb[k];
+                    if (live) cs[k] += tv[k] * va[r];
+                } else {
+                    const double gt = gv[RHS ? r : 0][k] * tv[k];
+                    rs += gt;
+                    rt += tv[k];
+                    if (live) {
+                        cs[k] += gt;
+                        ct[k] += tv[k];
+                    }
+                }
+            }
+            rs = cg_wave_sum(rs);
+            if (RHS) rt = cg_wave_sum(rt);
+            if (lane == 0 && live) {
+                I.rp0[(size_t)i * I.nstrips + st] = rs;
+                if (RHS) I.rp1[(size_t)i * I.nstrips + st] = rt;
+            }
         }
     }
-    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cs[k];
+    for (int k = 0; k < C; k++) lds[w][lane + 64 * k] = cs[k];
     __syncthreads();
     for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
         const int j = s0 + q;
         if (j < m) I.cp0[(size_t)rb * m + j] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
     }
-    if (mode == CG_QV_RHS) {
+    if (RHS) {
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = ct[k];
+        for (int k = 0; k < C; k++) lds[w][lane + 64 * k] = ct[k];
         __syncthreads();
         for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
             const int j = s0 + q;
             if (j < m) I.cp1[(size_t)rb * m + j] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
         }
     }
+}
+
+template <bool RHS>
+CG_DEV void cg_qv_strip(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
+                        const double* tab)
+{
+    switch (cg_strip_c(I, st)) {
+        case 1: cg_qv_tile<1, RHS>(I, b, rb, st, Gs, lds, tab); break;
+        case 2: cg_qv_tile<2, RHS>(I, b, rb, st, Gs, lds, tab); break;
+        default: cg_qv_tile<CG_CMAX, RHS>(I, b, rb, st, Gs, lds, tab); break;
+    }
+}
+
+// once per adjoint solve (loads G beside M; no register cap)
+__global__ __launch_bounds__(CG_WG) void k_qv_rhs(const CgInst* __restrict__ insts, const int* __restrict__ tile_inst,
+                                                  const double* __restrict__ ctl, const double* const* __restrict__ Gs)
+{
+    CG_EXP_TABLE(tab);
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
+    cg_qv_strip<true>(I, b, rb, st, Gs, lds, tab);
+}
+
+// every PCG iteration; only instances whose PCG is running
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_qv_pcg(
+    const CgInst* __restrict__ insts, const int* __restrict__ tile_inst, const double* __restrict__ ctl)
+{
+    CG_EXP_TABLE(tab);
+    int b, rb, st;
+    if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
+    const CgInstD& I = cg_inst(insts, b);
+    if (I.scal[PCG_STATUS] != 0.0) return;
+    __shared__ double lds[CG_WAVES][CG_CMAX * 64];
+    cg_qv_strip<false>(I, b, rb, st, nullptr, lds, tab);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,7 +9,7 @@
 // Vector roles inside CgInst for a backward instance:
 //   dir = x (lambda), yv = r, sv = z, uv = p, grad = w = Q p,
 //   rowsum/colsum = T 1 / T^T 1, Da = inverse diagonal preconditioner (or null).
-// Per-iteration launches: k_qv(CG_QV_PCG) -> k_pcg_q_fin -> k_pcg_update.
+// Per-iteration launches: k_qv_pcg -> k_pcg_q_fin -> k_pcg_update.
 // Every kernel skips an instance whose status is not "running", so the host can
 // launch iterations in chunks and read the statuses once per chunk; the
 // iterates are exactly those of the sequential loop.
