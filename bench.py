@@ -393,6 +393,7 @@ def adjoint_main(args):
     the polished base duals.  Reported for h = 1e-6 (the specified step) and
     h = 1e-4."""
     import curegot_backward_amd as cg
+    from curegot_backward_amd import _lib
     from curegot_backward_amd.workloads import adjoint_problem
 
     dev = torch.device("cuda", 0)
@@ -423,6 +424,18 @@ def adjoint_main(args):
         pre[name] = {"ms": 1000 * (time.perf_counter() - t0), "iterations": r["iterations"],
                      "converged": r["converged"]}
     log(f"forward {1000 * tf:.1f} ms (niter {out['niter']}), backward {1000 * tb:.1f} ms; preconditioners {pre}")
+    # adjoint pass throughput: the diag PCG's k_qv_pcg launches, HIP events on the solve stream
+    _lib.profile_reset()
+    _lib.profile_enable(True)
+    cg.solve_backward(out["plan"], G, out["context"], eta, preconditioner="diag")
+    torch.cuda.synchronize()
+    _lib.profile_enable(False)
+    qv_ms, qv_bytes, qv_n = _lib.profile_read("qv")
+    qv_roof = {"kernel": "k_qv_pcg", "launches": qv_n, "avg_launch_us": 1000 * qv_ms / max(qv_n, 1),
+               "achieved": qv_bytes / qv_ms / 1e6 if qv_ms > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "note": "M is 128 MiB: it stays in the 256 MB MALL across PCG iterations"}
+    qv_roof["frac"] = qv_roof["achieved"] / HBM_PEAK_GBS
+    log(f"k_qv_pcg: {qv_n} launches, {qv_roof['avg_launch_us']:.1f} us avg, {qv_roof['achieved']:.0f} GB/s")
 
     def polished(Mx, x0):
         # a fixed number of BCD sweeps past the SPLR solution: the marginals are
@@ -435,11 +448,11 @@ def adjoint_main(args):
     ctx_out = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 10, x0=dual)
     g = cg.solve_backward(ctx_out["plan"], 2 * ctx_out["plan"] + R, ctx_out["context"], eta, tol=1e-14)["grad"]
     rng = np.random.default_rng(args.seed + 1)
-    idx = [(int(rng.integers(n)), int(rng.integers(m))) for _ in range(args.fd)]
+    idx = [(int(rng.integers(n)), int(rng.integers(m))) for _ in range(args.fd)]   # --fd 0: no FD check
     gs = np.array([float(g[i, j]) for (i, j) in idx])
     fd_stats = {}
     t0 = time.perf_counter()
-    for h in (1e-6, 1e-4):
+    for h in ((1e-6, 1e-4) if idx else ()):
         fds = []
         for k, (i, j) in enumerate(idx):
             Mp, Mm = M.clone(), M.clone()
@@ -464,10 +477,11 @@ def adjoint_main(args):
                                "eta=0.05, loss=sum(T^2)+<T,R>, R~N(0,1), tol 1e-8", "seed": args.seed},
         "forward_ms": 1000 * tf, "backward_ms": 1000 * tb, "forward_niter": out["niter"],
         "backward_preconditioners": pre,
+        "roofline_qv": qv_roof,
         "autograd_vs_manual_grad_rel": float((grad_autograd - cg.solve_backward(
             out["plan"], G, out["context"], eta)["grad"]).abs().max() / grad_autograd.abs().max()),
-        "fd": {"entries": len(idx), "bcd_sweeps": args.fd_sweeps, "max_abs_grad": float(np.abs(gs).max()),
-               **fd_stats},
+        "fd": {"entries": len(idx), "bcd_sweeps": args.fd_sweeps,
+               "max_abs_grad": float(np.abs(gs).max()) if idx else None, **fd_stats},
     }
     print(json.dumps(result), flush=True)
 

@@ -449,9 +449,14 @@ CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&
 }
 
 
-// Ordered LSE merge of count (max, sum) partials, loads batched as above.
+// Log-sum-exp of count (max, sum) partials (pm[k*stride], ps[k*stride]) as
+// one (mx, s) pair, partials taken in index order CG_SUM_U at a time: the
+// loads of a block are in flight together, the block is reduced against its
+// own maximum (independent exps), and the block's pair is merged into the
+// running one (one more exp).  Empty partials (pm = -inf, ps = 0) contribute
+// nothing.  tab: CG_EXP_TABLE.
 CG_DEV void cg_ordered_lse(const CG_AS1 double* pm, const CG_AS1 double* ps, int count, size_t stride, double& mx,
-                           double& s)
+                           double& s, const double* tab)
 {
     mx = -INFINITY;
     s = 0.0;
@@ -463,8 +468,21 @@ CG_DEV void cg_ordered_lse(const CG_AS1 double* pm, const CG_AS1 double* ps, int
             a[u] = ok ? pm[(size_t)(k + u) * stride] : -INFINITY;
             c[u] = ok ? ps[(size_t)(k + u) * stride] : 0.0;
         }
+        double bm = a[0];
 #pragma unroll
-        for (int u = 0; u < CG_SUM_U; u++) cg_lse_merge(mx, s, a[u], c[u]);
+        for (int u = 1; u < CG_SUM_U; u++) bm = fmax(bm, a[u]);
+        if (bm == -INFINITY) continue;
+        double e[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) e[u] = c[u] * cg_texp_lo(a[u] - bm, tab);
+        double bs = e[0];
+#pragma unroll
+        for (int u = 1; u < CG_SUM_U; u++) bs += e[u];
+        // merge (bm, bs) into (mx, s); mx = -inf gives d = +inf, f = 0, s = bs
+        const double d = bm - mx;
+        const double f = cg_texp_lo(-fabs(d), tab);
+        s = (d > 0.0) ? fma(s, f, bs) : fma(bs, f, s);
+        mx = fmax(mx, bm);
     }
 }
 

@@ -95,13 +95,14 @@ __global__ __launch_bounds__(CG_WG) void k_rowlse_fin(const CgInst* __restrict__
                                                       const int* __restrict__ wg_inst,
                                                       const double* __restrict__ ctl, int mode)
 {
+    CG_EXP_TABLE(tab);
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
     const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.n) return;
     double mx, s;
-    cg_ordered_lse(I.rp0 + (size_t)q * I.nstrips, I.rp1 + (size_t)q * I.nstrips, I.nstrips, 1, mx, s);
+    cg_ordered_lse(I.rp0 + (size_t)q * I.nstrips, I.rp1 + (size_t)q * I.nstrips, I.nstrips, 1, mx, s, tab);
     auto alpha = (mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma;
     alpha[q] = I.reg * (I.logab[q] - (mx + log(s)));
 }
@@ -112,13 +113,14 @@ __global__ __launch_bounds__(CG_WG) void k_collse_fin(const CgInst* __restrict__
                                                       const int* __restrict__ wg_inst,
                                                       const double* __restrict__ ctl, int mode)
 {
+    CG_EXP_TABLE(tab);
     int b, wgi;
     if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
     const CgInstD& I = cg_inst(insts, b);
     const int q = wgi * CG_WG + threadIdx.x;
     if (q >= I.m) return;
     double mx, s;
-    cg_ordered_lse(I.cp0 + q, I.cp1 + q, I.nrb, I.m, mx, s);
+    cg_ordered_lse(I.cp0 + q, I.cp1 + q, I.nrb, I.m, mx, s, tab);
     auto beta = ((mode == CG_PT_BCD) ? I.gamma_bcd : I.gamma) + I.n;
     beta[q] = I.reg * (I.logab[I.n + q] - (mx + log(s)));
 }
