@@ -107,8 +107,11 @@ long long g_prof_count[PROF_NCLS] = {0};
 hipEvent_t prof_event()
 {
     if (!g_prof_pool.empty()) { hipEvent_t e = g_prof_pool.back(); g_prof_pool.pop_back(); return e; }
+    // timing only: without the system-scope fence that a default event
+    // performs when it is recorded (an L2 write-back and invalidation that
+    // would add to the interval and slow the work after it)
     hipEvent_t e;
-    CG_HIP(hipEventCreate(&e));
+    CG_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     return e;
 }
 
