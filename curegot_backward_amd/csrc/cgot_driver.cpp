@@ -716,6 +716,7 @@ struct SplrSolver {
             bt.per_inst0(k_sel_rows_scan, dc);
             bt.rows0(k_sel_rows_write, dc);
             bt.tiles0(k_sel_cols_count, dc);
+            bt.vec0(k_sel_cols_prefix, dc);
             bt.per_inst0(k_sel_cols_scan, dc);
             bt.tiles0(k_sel_cols_write, dc);
         }

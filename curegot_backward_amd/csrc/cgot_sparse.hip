@@ -283,8 +283,40 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
     }
 }
 
-// One workgroup per instance: column pointers of the CSC and the first output
-// slot of every (row block, column).
+// Thread per column j < m-1 (vector-map workgroups): exclusive prefix of the
+// column's counts over the row blocks, in place in ccnt (the first slot of
+// every (row block, column) inside the column), and the column total into
+// w_colptr[j].  Loads of CG_SUM_U row blocks are in flight at a time and
+// coalesced across the threads' columns.
+__global__ __launch_bounds__(CG_WG) void k_sel_cols_prefix(const CgInst* __restrict__ insts,
+                                                           const int* __restrict__ wg_inst,
+                                                           const double* __restrict__ ctl)
+{
+    const int2 e = cg_map_entry(wg_inst);
+    const int b = e.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const int m = I.m, mm = m - 1, nrb = I.nrb;
+    const int j = e.y * CG_WG + threadIdx.x;
+    if (j >= mm) return;
+    int run = 0;
+    for (int r0 = 0; r0 < nrb; r0 += CG_SUM_U) {
+        int c[CG_SUM_U];
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++) c[u] = (r0 + u < nrb) ? I.ccnt[(size_t)(r0 + u) * m + j] : 0;
+#pragma unroll
+        for (int u = 0; u < CG_SUM_U; u++)
+            if (r0 + u < nrb) {
+                I.ccnt[(size_t)(r0 + u) * m + j] = run;
+                run += c[u];
+            }
+    }
+    I.w_colptr[j] = run;   // column total for now
+}
+
+// One workgroup per instance: exclusive scan of the column totals into the
+// CSC column pointers.  A slot of (row block rb, column j) is then
+// w_colptr[j] + ccnt[rb*m + j].
 __global__ __launch_bounds__(CG_WG) void k_sel_cols_scan(const CgInst* __restrict__ insts,
                                                          const double* __restrict__ ctl)
 {
@@ -293,25 +325,15 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_scan(const CgInst* __restric
     const CgInstD& I = cg_inst(insts, b);
     __shared__ long long lds[CG_WG];
     __shared__ long long total;
-    const int mm = I.m - 1, m = I.m;
+    const int mm = I.m - 1;
     const int per = (mm + CG_WG - 1) / CG_WG;
     const int c0 = threadIdx.x * per, c1 = min(mm, c0 + per);
     long long csum = 0;
-    for (int j = c0; j < c1; j++) {
-        int run = 0;
-        for (int r = 0; r < I.nrb; r++) {
-            const int c = I.ccnt[(size_t)r * m + j];
-            I.ccnt[(size_t)r * m + j] = run;
-            run += c;
-        }
-        I.w_colptr[j] = run;   // column total for now
-        csum += run;
-    }
+    for (int j = c0; j < c1; j++) csum += I.w_colptr[j];
     long long off = cg_block_excl_scan(csum, lds, &total);
     for (int j = c0; j < c1; j++) {
         const int c = I.w_colptr[j];
         I.w_colptr[j] = (int)off;
-        for (int r = 0; r < I.nrb; r++) I.ccnt[(size_t)r * m + j] += (int)off;
         off += c;
     }
     if (threadIdx.x == 0) I.w_colptr[mm > 0 ? mm : 0] = (int)total;
@@ -352,7 +374,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
         const int q = lane + 64 * k;
         const int j = s0 + q;
         if (k < I.c && j < mm && bits[k]) {
-            int pos = I.ccnt[(size_t)rb * I.m + j];
+            int pos = I.w_colptr[j] + I.ccnt[(size_t)rb * I.m + j];
             for (int ww = 0; ww < w; ww++) pos += lds[ww][q];
             for (int r = 0; r < CG_RPW; r++)
                 if (bits[k] & (1u << r)) I.w_rowi[pos++] = i0 + r;
