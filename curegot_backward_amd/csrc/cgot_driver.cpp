@@ -25,6 +25,7 @@
 #include <cstring>
 #include <limits>
 #include <array>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -1493,17 +1494,48 @@ private:
     int workers_ = 0, idle_ = 0;
 };
 
-// Streams of size classes 1..kMaxGroups-1 for batches issued from the calling
+// CUs kept free of size classes 1.. when a batch is split: the largest
+// class, whose factorization chain (one small launch per block step) sets the
+// batch time, then finds CUs for its step launches without waiting behind the
+// other classes' wide launches.  5/16 of the CUs (80 of 256: ragged B = 128
+// batch 117 -> 110 ms; 0, 16, 32, 64, 80, 96, 112, 128 reserved gave 117,
+// 115.5, 114, 112, 110, 110.4, 119, 121 ms); CGOT_CU_RESERVE=<count>
+// overrides it, 0 turns it off.
+int cu_reserve(int ncu)
+{
+    static const int env = [] {
+        const char* e = std::getenv("CGOT_CU_RESERVE");
+        return e ? std::max(0, std::atoi(e)) : -1;
+    }();
+    return (env >= 0) ? env : ncu * 5 / 16;
+}
+
+// Streams of size classes 0..kMaxGroups-1 for batches issued from the calling
 // thread on `device` (created once per thread and device, default priority).
 // Each class index keeps its stream from call to call, so a caller uses the
-// same kMaxGroups streams (and hardware queues) every time.
+// same streams (and hardware queues) every time.  With CUs reserved, classes
+// 1.. run on CU-masked streams (blocking streams: they also order against
+// null-stream work of other threads) and class 0 on a stream of its own
+// (unmasked, non-blocking), so that no class runs on a caller's null stream;
+// with no reserve, class 0 runs on the caller's stream.
 hipStream_t group_stream(int device, int k)
 {
     thread_local std::map<int, std::array<hipStream_t, kMaxGroups>> streams;
     auto it = streams.find(device);
     if (it == streams.end()) {
         std::array<hipStream_t, kMaxGroups> a{};
-        for (int q = 1; q < kMaxGroups; q++) CG_HIP(hipStreamCreateWithFlags(&a[q], hipStreamNonBlocking));
+        int ncu = 0;
+        CG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        const int res = cu_reserve(ncu);
+        if (res > 0 && res < ncu) {
+            CG_HIP(hipStreamCreateWithFlags(&a[0], hipStreamNonBlocking));
+            std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+            for (int c = 0; c < ncu - res; c++) mask[c / 32] |= 1u << (c % 32);
+            for (int q = 1; q < kMaxGroups; q++)
+                CG_HIP(hipExtStreamCreateWithCUMask(&a[q], (uint32_t)mask.size(), mask.data()));
+        } else {
+            for (int q = 1; q < kMaxGroups; q++) CG_HIP(hipStreamCreateWithFlags(&a[q], hipStreamNonBlocking));
+        }
         it = streams.emplace(device, a).first;
     }
     return it->second[k];
@@ -1595,6 +1627,10 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
     std::vector<std::function<void()>> jobs;
     std::vector<hipStream_t> gst(G, nullptr);
     for (int k = 1; k < G; k++) gst[k] = group_stream(device, k);
+    // class 0: the caller's stream, or its own stream when CUs are reserved
+    hipStream_t st0 = group_stream(device, 0);
+    if (st0 != nullptr) CG_HIP(hipStreamWaitEvent(st0, start, 0));
+    else st0 = (hipStream_t)stream;
     for (int k = 0; k < G; k++) sub[k].assign(g.idx[k].size(), cgot_splr_info{});
     for (int k = 1; k < G; k++) {
         jobs.push_back([&, k]() {
@@ -1613,13 +1649,17 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
     }
     std::thread host([&]() { GroupPool::get().run(jobs); });
     try {
-        splr_solve_group(g.probs[0].data(), (int)g.probs[0].size(), (char*)workspace + g.off[0],
-                         (hipStream_t)stream, sub[0].data());
+        splr_solve_group(g.probs[0].data(), (int)g.probs[0].size(), (char*)workspace + g.off[0], st0,
+                         sub[0].data());
+        if (st0 != (hipStream_t)stream) {
+            CG_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+            CG_HIP(hipEventRecord(done[0], st0));
+        }
     } catch (const std::exception& e) {
         err[0] = e.what();
     }
     host.join();
-    for (int k = 1; k < G; k++) {
+    for (int k = 0; k < G; k++) {
         if (done[k]) {
             (void)hipStreamWaitEvent((hipStream_t)stream, done[k], 0);
             (void)hipEventDestroy(done[k]);
