@@ -20,7 +20,7 @@ from . import _lib
 
 _lib.load()   # fail loudly if the HIP library is not built
 
-from . import _native, autograd, backward  # noqa: E402
+from . import _native, autograd, backward, numpy_api  # noqa: E402
 from ._native import (  # noqa: E402
     SPLRBackwardContext,
     sinkhorn_bcd,
@@ -41,11 +41,18 @@ for _name in ("splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT", "solve_
     setattr(torch, _name, globals()[_name])
 _sys.modules[torch.__name__] = torch
 
+# Host-array interface, like the reference's numpy submodule
+# (src/bindings.cpp:112-122): NumPy in, NumPy out, solved on the GPU.
+numpy = _types.ModuleType(__name__ + ".numpy")
+numpy.sinkhorn_bcd = numpy_api.sinkhorn_bcd
+numpy.sinkhorn_splr = numpy_api.sinkhorn_splr
+_sys.modules[numpy.__name__] = numpy
+
 # Test hooks, like the reference's curegot.tests (src/tests.cpp:36-200).
 tests = _types.ModuleType(__name__ + ".tests")
 tests.test_T_computation_sparsify = _native.T_sparsify
 _sys.modules[tests.__name__] = tests
 
-__all__ = ["torch", "tests", "splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT",
+__all__ = ["torch", "numpy", "tests", "splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT",
            "SPLROTFunction", "solve_backward", "solve_backward_batch", "sinkhorn_splr_bwd",
            "sinkhorn_splr_bwd_batch", "sinkhorn_splr", "sinkhorn_bcd", "SPLRBackwardContext"]

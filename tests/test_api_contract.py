@@ -34,6 +34,22 @@ def test_scalar_validation(kw, exc):
         cg.torch.SPLROT(**kw)
 
 
+def test_numpy_interface_shape_errors():
+    """The numpy submodule (reference src/bindings.cpp:112-122) raises the
+    reference's messages (src/solvers_numpy.cpp:28-66) before any device work."""
+    assert cg.numpy.__name__ == "curegot_backward_amd.numpy"
+    M, a, b = [x.numpy() for x in make_problem(6, 5, 0)]
+    for f in (cg.numpy.sinkhorn_bcd, cg.numpy.sinkhorn_splr):
+        with pytest.raises(RuntimeError, match="M must be a 2D array"):
+            f(M.ravel(), a, b, 0.1)
+        with pytest.raises(RuntimeError, match="a and b must be 1D arrays"):
+            f(M, a[:, None], b, 0.1)
+        with pytest.raises(RuntimeError, match="Shape mismatch"):
+            f(M, a[:-1], b, 0.1)
+        with pytest.raises(RuntimeError, match=r"x0 must be a 1D array of length n \+ m"):
+            f(M, a, b, 0.1, x0=np.zeros(3))
+
+
 def test_bad_preconditioner_name():
     with pytest.raises(ValueError):
         cg.torch.SPLROT(0.1, backward_preconditioner="chol")

@@ -153,6 +153,30 @@ def test_sinkhorn_bcd_vs_oracle(cg, n, m, eta):
     assert rel(out["plan"], ref["plan"]) < 1e-9
 
 
+def test_numpy_interface_matches_torch(cg):
+    """numpy in, numpy out (reference src/solvers_numpy.cpp:13-244): the same
+    native solve as the torch interface, bitwise, and BCD against the oracle."""
+    from curegot_backward_amd.workloads import make_problem
+    n, m, eta = 80, 60, 0.05
+    M, a, b = make_problem(n, m, 3)
+    out = cg.numpy.sinkhorn_splr(M.numpy(), a.numpy(), b.numpy(), eta, 1e-9, 1000, density=0.2, shift=1e-4)
+    ref = cg.sinkhorn_splr(M.cuda(), a.cuda(), b.cuda(), eta, 1e-9, 1000, density=0.2, shift=1e-4)
+    assert set(out) == {"niter", "plan", "dual"} and out["niter"] == ref["niter"]
+    assert isinstance(out["plan"], np.ndarray) and out["plan"].dtype == np.float64 and out["plan"].shape == (n, m)
+    assert np.array_equal(out["plan"], ref["plan"].cpu().numpy())
+    assert np.array_equal(out["dual"], ref["dual"].cpu().numpy())
+    x0 = out["dual"]
+    bo = cg.numpy.sinkhorn_bcd(M.numpy(), a.numpy(), b.numpy(), eta, 1e-9, 20000, x0=x0)
+    bt = cg.sinkhorn_bcd(M.cuda(), a.cuda(), b.cuda(), eta, 1e-9, 20000, x0=torch.from_numpy(x0).cuda())
+    assert bo["niter"] == bt["niter"] and np.array_equal(bo["plan"], bt["plan"].cpu().numpy())
+    orc = oracle.sinkhorn_bcd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-9, max_iter=20000)
+    assert rel(torch.from_numpy(bo["plan"]), orc["plan"]) < 1e-8
+    # any real dtype is converted to float64, like py::array_t<double>
+    oi = cg.numpy.sinkhorn_splr(M.numpy(), a.numpy(), b.numpy(), eta, 1e-9, 1000, density=0.2, shift=1e-4,
+                                x0=list(np.zeros(n + m)))
+    assert oi["plan"].dtype == np.float64 and oi["niter"] > 0
+
+
 @pytest.mark.parametrize("n,m,eta,seed", [(100, 120, 0.05, 3), (230, 150, 0.02, 8), (70, 300, 0.1, 9)])
 def test_splr_preconditioner_apply_vs_dense(cg, n, m, eta, seed):
     """z = B^{-1} r of the stored factor against numpy: A from the oracle's
