@@ -71,10 +71,19 @@ class BcdProblem(ctypes.Structure):
     ]
 
 
+class PrecondProfile(ctypes.Structure):
+    """cgot_precond_profile (microseconds)."""
+    _fields_ = [(name, ctypes.c_double) for name in (
+        "host_total_us", "host_memcpy_in_us", "host_solve_launch_us", "host_lowrank_us", "host_sync_wait_us",
+        "host_memcpy_out_us", "gpu_solve_us", "gpu_lowrank_reduce_us", "gpu_lowrank_correction_us",
+        "repeated_total_us")]
+
+
 EXPORTS = (
     "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
-    "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply",
+    "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply", "cgot_splr_precond_apply_profiled",
+    "cgot_splr_context_csr",
     "cgot_bcd_workspace_size", "cgot_bcd_solve",
     "cgot_lse_workspace_size", "cgot_lse_update",
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
@@ -111,6 +120,11 @@ def load():
     L.cgot_splr_precond_workspace_size.restype = sz
     L.cgot_splr_precond_apply.argtypes = [vp, i32, i32, vp, vp, vp, sz, vp]
     L.cgot_splr_precond_apply.restype = ctypes.c_int
+    L.cgot_splr_precond_apply_profiled.argtypes = [vp, i32, i32, vp, vp, i32, i32, ctypes.POINTER(PrecondProfile),
+                                                   vp, sz, vp]
+    L.cgot_splr_precond_apply_profiled.restype = ctypes.c_int
+    L.cgot_splr_context_csr.argtypes = [vp, i32, i32, vp, vp, vp, ctypes.POINTER(i64), vp]
+    L.cgot_splr_context_csr.restype = ctypes.c_int
     L.cgot_bcd_workspace_size.argtypes = [i32, ctypes.POINTER(BcdProblem)]
     L.cgot_bcd_workspace_size.restype = sz
     L.cgot_bcd_solve.argtypes = [i32, ctypes.POINTER(BcdProblem), vp, sz, vp, ctypes.POINTER(i32)]

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 from typing import Sequence
 
 import torch
@@ -121,10 +122,43 @@ class SPLRBackwardContext:
         return self._inv_diag.clone()
 
     def secant_s(self) -> torch.Tensor:
+        """Copy of s (reference solvers_torch.cpp:611-621: unavailable without
+        the rank-2 term)."""
+        if not self._low_rank:
+            raise RuntimeError("SPLRBackwardContext: low-rank correction is disabled, s is unavailable")
         return self._s.clone()
 
     def secant_y(self) -> torch.Tensor:
+        """Copy of y (reference solvers_torch.cpp:623-633)."""
+        if not self._low_rank:
+            raise RuntimeError("SPLRBackwardContext: low-rank correction is disabled, y is unavailable")
         return self._y.clone()
+
+    def _csr(self):
+        # A of the last factorized iterate in the reference's lower CSR
+        # (reference solvers_torch.cpp:583-610): copies on the device
+        f = self.factor_ptr()
+        L = _lib.load()
+        dev = self._M.device
+        nnz = ctypes.c_int64(0)
+        with torch.cuda.device(dev):
+            _lib.check(L.cgot_splr_context_csr(f, self._n, self._m, None, None, None, ctypes.byref(nnz),
+                                               _stream(dev)), "csr")
+            vals = torch.empty(nnz.value, dtype=torch.float64, device=dev)
+            cols = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+            rowptr = torch.empty(self.dim + 1, dtype=torch.int32, device=dev)
+            _lib.check(L.cgot_splr_context_csr(f, self._n, self._m, vals.data_ptr(), cols.data_ptr(),
+                                               rowptr.data_ptr(), ctypes.byref(nnz), _stream(dev)), "csr")
+        return vals, cols, rowptr
+
+    def csr_values(self) -> torch.Tensor:
+        return self._csr()[0]
+
+    def csr_colind(self) -> torch.Tensor:
+        return self._csr()[1]
+
+    def csr_rowptr(self) -> torch.Tensor:
+        return self._csr()[2]
 
     def ctx_point(self) -> torch.Tensor:
         return self._dual[: self.dim].clone()
@@ -153,6 +187,80 @@ class SPLRBackwardContext:
         if self._factor is None:
             raise RuntimeError("SPLRBackwardContext: no stored factor (the forward took no step)")
         return self._factor
+
+    def _profiled(self, r, base_only: bool, repeats: int):
+        self._check_r(r)
+        if int(repeats) <= 0:
+            raise RuntimeError("repeats must be positive")
+        f = self.factor_ptr()
+        rr = r.to(torch.float64).contiguous()
+        z = torch.empty_like(rr)
+        prof = _lib.PrecondProfile()
+        L = _lib.load()
+        dev = self._M.device
+        with torch.cuda.device(dev):
+            nbytes = L.cgot_splr_precond_workspace_size(f, self._n, self._m)
+            ws = _workspace(nbytes, dev)
+            rc = L.cgot_splr_precond_apply_profiled(f, self._n, self._m, rr.data_ptr(), z.data_ptr(),
+                                                    1 if base_only else 0, int(repeats), ctypes.byref(prof),
+                                                    ws.data_ptr(), nbytes, _stream(dev))
+            _lib.check(rc, "preconditioner profile")
+        return z, prof
+
+    # ---- debug/benchmark methods (reference solvers_torch.cpp:380-580).  The
+    # dicts carry the reference's keys.  The stages differ here: there is no
+    # host copy and no cuDSS, and the rank-2 term needs no host scalar or sync.
+    # The copy keys hold the time to enqueue the device copies, the cuDSS keys
+    # the Schur triangular solves, and the keys of stages this design does not
+    # have are 0.
+    def profile_preconditioner(self, r: torch.Tensor) -> dict:
+        t0 = time.perf_counter()
+        z, p = self._profiled(r, False, 1)
+        wrapper = 1e6 * (time.perf_counter() - t0)
+        return {"output": z, "host_wrapper_us": wrapper, "host_memcpy_in_us": p.host_memcpy_in_us,
+                "host_solve_launch_us": p.host_solve_launch_us, "host_lowrank_total_us": p.host_lowrank_us,
+                "host_final_sync_wait_us": p.host_sync_wait_us, "host_memcpy_out_us": p.host_memcpy_out_us,
+                "host_total_us": p.host_total_us, "gpu_cudss_solve_us": p.gpu_solve_us,
+                "low_rank_active": self._low_rank, "lowrank_setup_launch_host_us": p.host_lowrank_us,
+                "lowrank_sync_wait_host_us": 0.0, "lowrank_host_scalar_us": 0.0,
+                "lowrank_correction_launch_host_us": 0.0, "lowrank_reduce_gpu_us": p.gpu_lowrank_reduce_us,
+                "lowrank_correction_gpu_us": p.gpu_lowrank_correction_us}
+
+    def profile_base_solve(self, r: torch.Tensor) -> dict:
+        t0 = time.perf_counter()
+        z, p = self._profiled(r, True, 1)
+        wrapper = 1e6 * (time.perf_counter() - t0)
+        return {"output": z, "host_wrapper_us": wrapper, "host_memcpy_in_us": p.host_memcpy_in_us,
+                "host_cudss_execute_us": p.host_solve_launch_us, "host_sync_wait_us": p.host_sync_wait_us,
+                "host_memcpy_out_us": p.host_memcpy_out_us, "host_total_us": p.host_total_us,
+                "gpu_solve_us": p.gpu_solve_us, "low_rank_enabled": self._low_rank}
+
+    def benchmark_preconditioner_repeated(self, r: torch.Tensor, repeats: int) -> dict:
+        z, p = self._profiled(r, False, repeats)
+        return {"output": z, "repeats": int(repeats), "cpp_repeated_total_us": p.repeated_total_us,
+                "cpp_per_call_us": p.repeated_total_us / int(repeats)}
+
+    def benchmark_base_solve_repeated(self, r: torch.Tensor, repeats: int) -> dict:
+        z, p = self._profiled(r, True, repeats)
+        return {"output": z, "repeats": int(repeats), "cpp_repeated_total_us": p.repeated_total_us,
+                "cpp_per_call_us": p.repeated_total_us / int(repeats)}
+
+    def benchmark_diag_setup(self, repeats: int) -> float:
+        """Median microseconds of a validated diagonal set-up (reference
+        solvers_torch.cpp:380-391).  The forward already writes diag(A)^{-1};
+        the set-up left to time is its validation: a device check that every
+        entry is finite and positive, read back on the host."""
+        if int(repeats) <= 0:
+            raise RuntimeError("repeats must be positive")
+        ts = []
+        for _ in range(int(repeats)):
+            torch.cuda.synchronize(self._M.device)
+            t0 = time.perf_counter()
+            ok = bool(((self._inv_diag > 0) & torch.isfinite(self._inv_diag)).all())
+            ts.append(1e6 * (time.perf_counter() - t0))
+            if not ok:
+                raise RuntimeError("SPLRBackwardContext: non-positive or non-finite diagonal")
+        return float(sorted(ts)[len(ts) // 2])
 
     def apply_preconditioner(self, r: torch.Tensor) -> torch.Tensor:
         """z = B^{-1} r with B = A_k + the rank-2 BFGS term of the last forward

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -1187,16 +1188,25 @@ struct PcSolver {
     void apply(const double* ctlp, int init)
     {
         if (ord.empty()) return;
+        solve(ctlp);
+        lowrank_dots(ctlp);
+        lowrank_apply(ctlp);
+        hipLaunchKernelGGL(k_pcg_rz, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, ctlp, init);
+    }
+    // the stages of apply: z = A^{-1} r (Schur rhs, forward and backward
+    // substitution, back-substitution into z), then the rank-2 BFGS term
+    // (its dot products, then the correction of z)
+    void solve(const double* ctlp)
+    {
         epoch++;
         bt.rows0(k_pc_rhs, ctlp);
         hipLaunchKernelGGL(k_pc_fwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
                            2 * epoch - 1);
         hipLaunchKernelGGL(k_pc_bwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl);
         bt.rows0(k_pc_back, ctlp);
-        bt.vec0(k_pc_wdots, ctlp);
-        bt.vec0(k_pc_wapply, ctlp);
-        hipLaunchKernelGGL(k_pcg_rz, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, ctlp, init);
     }
+    void lowrank_dots(const double* ctlp) { bt.vec0(k_pc_wdots, ctlp); }
+    void lowrank_apply(const double* ctlp) { bt.vec0(k_pc_wapply, ctlp); }
 };
 
 void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info* info)
@@ -1745,9 +1755,22 @@ size_t cgot_splr_precond_workspace_size(const void* factor, int32_t n, int32_t m
 int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const double* r, double* z,
                             void* workspace, size_t workspace_bytes, void* stream)
 {
+    return cgot_splr_precond_apply_profiled(factor, n, m, r, z, 0, 1, nullptr, workspace, workspace_bytes, stream);
+}
+
+int cgot_splr_precond_apply_profiled(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                                     int32_t base_only, int32_t repeats, cgot_precond_profile* prof,
+                                     void* workspace, size_t workspace_bytes, void* stream)
+{
     CG_API_BEGIN
+    using clk = std::chrono::steady_clock;
+    const auto us = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double, std::micro>(b - a).count();
+    };
+    const auto t_start = clk::now();
     if (!factor || !r || !z) throw std::invalid_argument("factor, r and z must be non-null");
     if (n < 1 || m < 2) throw std::invalid_argument("n >= 1 and m >= 2 required");
+    if (repeats < 1) throw std::invalid_argument("repeats must be positive");
     Batch bt;
     bt.B = 1;
     bt.stream = (hipStream_t)stream;
@@ -1762,16 +1785,90 @@ int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const doub
     CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * CG_NSCAL, bt.stream));   // PCG status: running
     const CgInst& I = bt.h[0];
     const size_t N = (size_t)n + m - 1;
-    CG_HIP(hipMemcpyAsync(I.yv, r, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
     auto c = ctl_vec(1);
     c[CTL_ACTIVE] = 1.0;
     PcSolver pc(bt);
-    pc.apply(bt.ctl(c), 1);
-    CG_HIP(hipMemcpyAsync(z, I.sv, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+    const double* dc = bt.ctl(c);
+    // the production apply (prof == NULL): one application with r.z, as the
+    // PCG runs it; the profiled variants time its stages and leave r.z out
+    const bool stages = (prof != nullptr && repeats == 1);
+    hipEvent_t ev[4] = {};
+    if (stages)
+        for (auto& e : ev) CG_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    cgot_precond_profile pr{};
+    const auto t_loop = clk::now();
+    for (int rep = 0; rep < repeats; rep++) {
+        const auto a0 = clk::now();
+        CG_HIP(hipMemcpyAsync(I.yv, r, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+        const auto a1 = clk::now();
+        if (prof == nullptr) {
+            pc.apply(dc, 1);
+        } else {
+            if (stages) CG_HIP(hipEventRecord(ev[0], bt.stream));
+            pc.solve(dc);
+            if (stages) CG_HIP(hipEventRecord(ev[1], bt.stream));
+            const auto a2 = clk::now();
+            if (!base_only) {
+                pc.lowrank_dots(dc);
+                if (stages) CG_HIP(hipEventRecord(ev[2], bt.stream));
+                pc.lowrank_apply(dc);
+                if (stages) CG_HIP(hipEventRecord(ev[3], bt.stream));
+            }
+            const auto a3 = clk::now();
+            pr.host_solve_launch_us += us(a1, a2);
+            pr.host_lowrank_us += us(a2, a3);
+        }
+        const auto a4 = clk::now();
+        CG_HIP(hipMemcpyAsync(z, I.sv, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+        pr.host_memcpy_in_us += us(a0, a1);
+        pr.host_memcpy_out_us += us(a4, clk::now());
+    }
     bt.check_launch();
-    bt.read_scal();
+    const auto w0 = clk::now();
+    bt.read_scal();   // stream sync
+    const auto w1 = clk::now();
     if (bt.sc(0, CS_PIVOT) >= 2.0)
         throw std::runtime_error("internal: a block-flag wait of the preconditioner exceeded its bound");
+    if (prof != nullptr) {
+        pr.host_sync_wait_us = us(w0, w1);
+        pr.host_total_us = us(t_start, w1);
+        pr.repeated_total_us = us(t_loop, w1);
+        if (stages) {
+            float ms = 0.f;
+            CG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            pr.gpu_solve_us = 1000.0 * ms;
+            if (!base_only) {
+                CG_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+                pr.gpu_lowrank_reduce_us = 1000.0 * ms;
+                CG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+                pr.gpu_lowrank_correction_us = 1000.0 * ms;
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+        *prof = pr;
+    }
+    return 0;
+    CG_API_END
+}
+
+int cgot_splr_context_csr(const void* factor, int32_t n, int32_t m, double* csr_values, int32_t* csr_colind,
+                          int32_t* csr_rowptr, int64_t* nnz, void* stream)
+{
+    CG_API_BEGIN
+    if (!factor || !nnz) throw std::invalid_argument("factor and nnz must be non-null");
+    CgInst F;
+    CG_HIP(hipMemcpy(&F, factor, sizeof(CgInst), hipMemcpyDeviceToHost));
+    if (F.n != n || F.m != m) throw std::invalid_argument("factor does not match the problem shape");
+    int K = 0;
+    if (m >= 2) CG_HIP(hipMemcpy(&K, F.w_colptr + (m - 1), sizeof(int), hipMemcpyDeviceToHost));
+    *nnz = (int64_t)n + (m - 1) + K;
+    if (csr_values != nullptr) {
+        if (!csr_colind || !csr_rowptr) throw std::invalid_argument("csr_colind and csr_rowptr must be non-null");
+        const int rows = std::max(n, m - 1);
+        hipLaunchKernelGGL(k_ctx_csr, dim3(cdiv(rows, CG_WG)), dim3(CG_WG), 0, (hipStream_t)stream,
+                           (const CgInst*)factor, csr_values, (int*)csr_colind, (int*)csr_rowptr);
+        CG_HIP(hipGetLastError());
+    }
     return 0;
     CG_API_END
 }

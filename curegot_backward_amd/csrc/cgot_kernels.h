@@ -52,6 +52,7 @@ __global__ void k_sel_rows_count(const CgInst*, const int* row_inst, const doubl
 __global__ void k_sel_rows_scan(const CgInst*, const double*);                       // grid = B
 __global__ void k_sel_rows_write(const CgInst*, const int* row_inst, const double*); // row grid
 __global__ void k_sel_cols_count(const CgInst*, const int*, const double*);
+__global__ void k_ctx_csr(const CgInst* fwd, double* vals, int* cols, int* rowptr);   // stored context
 __global__ void k_sel_cols_prefix(const CgInst*, const int*, const double*);      // vector map
 __global__ void k_sel_cols_scan(const CgInst*, const double*);                    // grid = B
 __global__ void k_sel_cols_write(const CgInst*, const int*, const double*);

@@ -411,16 +411,9 @@ __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restri
 // Lower CSR of Hsl in the reference layout (test hook): rows 0..n-1 hold their
 // diagonal, row n+j holds the selected T_ij (ascending i) and then its diagonal.
 // Reference: the CSR built by launch_sphess, src/sinkhorn_splr_kernel.cu:839-906.
-__global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ insts,
-                                                   const int* __restrict__ wg_inst,
-                                                   const double* __restrict__ ctl,
-                                                   double* __restrict__ vals, int* __restrict__ cols,
-                                                   int* __restrict__ rowptr)
+CG_DEV void cg_hsl_csr_entry(const CgInstD& I, int q, double* __restrict__ vals, int* __restrict__ cols,
+                              int* __restrict__ rowptr)
 {
-    const int2 e = cg_map_entry(wg_inst);
-    const int b = e.x;
-    const CgInstD& I = cg_inst(insts, b);
-    const int q = e.y * CG_WG + threadIdx.x;
     const int n = I.n;
     if (q < n) {
         vals[q] = I.Da[q];
@@ -435,4 +428,22 @@ __global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ in
         cols[p] = n + q;
     }
     if (q == 0) rowptr[I.N] = n + I.w_colptr[I.m - 1] + (I.m - 1);
+}
+
+__global__ __launch_bounds__(CG_WG) void k_hsl_csr(const CgInst* __restrict__ insts,
+                                                   const int* __restrict__ wg_inst,
+                                                   const double* __restrict__ ctl,
+                                                   double* __restrict__ vals, int* __restrict__ cols,
+                                                   int* __restrict__ rowptr)
+{
+    const int2 e = cg_map_entry(wg_inst);
+    cg_hsl_csr_entry(cg_inst(insts, e.x), e.y * CG_WG + threadIdx.x, vals, cols, rowptr);
+}
+
+// The same CSR for one stored forward descriptor (the context of a finished
+// solve), grid = ceil(max(n, m-1) / CG_WG).
+__global__ __launch_bounds__(CG_WG) void k_ctx_csr(const CgInst* __restrict__ fwd, double* __restrict__ vals,
+                                                   int* __restrict__ cols, int* __restrict__ rowptr)
+{
+    cg_hsl_csr_entry(cg_inst(fwd, 0), blockIdx.x * CG_WG + threadIdx.x, vals, cols, rowptr);
 }

@@ -118,7 +118,6 @@ size_t cgot_adjoint_workspace_size(int32_t B, const cgot_adjoint_problem* probs)
 int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* workspace,
                        size_t workspace_bytes, void* stream, cgot_adjoint_info* info);
 
-/* Sinkhorn/BCD solve (legacy sinkhorn_bcd).  dual is not gauge-fixed. */
 /* SPLR preconditioner of one converged forward solve applied to a vector:
  * z = B^{-1} r with B = A + rank-2 BFGS term of the last iteration, A^{-1} by
  * the stored Schur factor (factor = cgot_splr_info.factor, its workspace
@@ -128,6 +127,47 @@ size_t cgot_splr_precond_workspace_size(const void* factor, int32_t n, int32_t m
 int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const double* r, double* z,
                             void* workspace, size_t workspace_bytes, void* stream);
 
+/* Timings of cgot_splr_precond_apply_profiled, in microseconds.  host_*: time
+ * in the calling thread (enqueueing, and the final wait); gpu_*: HIP-event
+ * intervals on the stream.  There is no host copy in this design, so the
+ * copy fields hold the time to enqueue the device copies of r and z. */
+typedef struct cgot_precond_profile {
+    double host_total_us;
+    double host_memcpy_in_us;       /* enqueue r -> workspace */
+    double host_solve_launch_us;    /* enqueue the A^{-1} r launches */
+    double host_lowrank_us;         /* enqueue the rank-2 launches */
+    double host_sync_wait_us;       /* wait for the stream */
+    double host_memcpy_out_us;      /* enqueue workspace -> z */
+    double gpu_solve_us;            /* A^{-1} r: Schur rhs, triangular solves, back-substitution */
+    double gpu_lowrank_reduce_us;   /* rank-2 dot products */
+    double gpu_lowrank_correction_us;
+    double repeated_total_us;       /* repeats > 1: host time of all repeats, synchronized */
+} cgot_precond_profile;
+
+/* Profiled / repeated / base-only variants of cgot_splr_precond_apply (the
+ * same workspace size).  base_only != 0 applies A^{-1} only (no rank-2
+ * term).  repeats >= 1 applies it that many times back to back on the same r
+ * (z holds the last); with repeats == 1 the per-stage timings are filled in,
+ * with repeats > 1 only repeated_total_us.  Replaces the debug/benchmark
+ * methods profile_preconditioner, benchmark_preconditioner_repeated,
+ * profile_base_solve and benchmark_base_solve_repeated
+ * (src/solvers_torch.cpp:397-580; backward_context.cu apply_*_profiled). */
+int cgot_splr_precond_apply_profiled(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                                     int32_t base_only, int32_t repeats, cgot_precond_profile* prof,
+                                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* The sparsified Hessian A of the stored factor (the last factorized
+ * iterate) in the reference's lower CSR layout: rows 0..n-1 hold their
+ * diagonal; row n+j holds its selected entries (ascending column) and then
+ * its diagonal.  nnz = n + (m-1) + K.  With csr_values == NULL only *nnz is
+ * written (one device read); otherwise csr_values [nnz], csr_colind [nnz]
+ * and csr_rowptr [n+m] (device) are written on the stream.  Replaces the
+ * validation accessors csr_values / csr_colind / csr_rowptr,
+ * src/solvers_torch.cpp:583-610 (backward_context.h csr_*). */
+int cgot_splr_context_csr(const void* factor, int32_t n, int32_t m, double* csr_values, int32_t* csr_colind,
+                          int32_t* csr_rowptr, int64_t* nnz, void* stream);
+
+/* Sinkhorn/BCD solve (legacy sinkhorn_bcd).  dual is not gauge-fixed. */
 typedef struct cgot_bcd_problem {
     const double* M; const double* a; const double* b;
     int32_t n, m;

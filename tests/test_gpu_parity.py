@@ -214,6 +214,65 @@ def test_splr_preconditioner_apply_vs_dense(cg, n, m, eta, seed):
 
 
 @pytest.mark.parametrize("n,m,eta,seed", [(100, 120, 0.05, 3), (230, 150, 0.02, 8)])
+def test_context_csr_vs_oracle(cg, n, m, eta, seed):
+    """csr_values / csr_colind / csr_rowptr of the context (reference
+    solvers_torch.cpp:583-610): A of the last factorized iterate in the lower
+    CSR layout, against the oracle's context."""
+    from curegot_backward_amd.workloads import make_problem
+    print(f"case n={n} m={m} eta={eta} seed={seed}", flush=True)
+    M, a, b = make_problem(n, m, seed)
+    ref = oracle.splr_bwd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-8, max_iter=500)
+    out = cg.sinkhorn_splr_bwd(M.cuda(), a.cuda(), b.cuda(), eta, 1e-8, 500)
+    if out["niter"] != ref["niter"]:
+        pytest.skip("trajectory split at the f-resolution limit: the two contexts hold different A")
+    ctx = out["context"]
+    vals, cols, rowptr = ctx.csr_values(), ctx.csr_colind(), ctx.csr_rowptr()
+    assert vals.dtype == torch.float64 and cols.dtype == torch.int32 and rowptr.dtype == torch.int32
+    assert rowptr.shape[0] == ctx.dim + 1 and int(rowptr[-1]) == vals.shape[0] == cols.shape[0]
+    assert np.array_equal(rowptr.cpu().numpy(), np.asarray(ref["csr_rowptr"]))
+    assert np.array_equal(cols.cpu().numpy(), np.asarray(ref["csr_colind"]))
+    assert rel(vals, np.asarray(ref["csr_values"])) < 1e-9
+
+
+def test_context_profile_and_benchmark_methods(cg):
+    """The debug/benchmark methods (reference solvers_torch.cpp:380-580):
+    the profiled and repeated applies return the production result bitwise,
+    the base solve is A^{-1} r of the context's own CSR, and every dict
+    carries the reference's keys."""
+    from curegot_backward_amd.workloads import make_problem
+    n, m, eta = 100, 120, 0.05
+    M, a, b = make_problem(n, m, 3)
+    ctx = cg.sinkhorn_splr_bwd(M.cuda(), a.cuda(), b.cuda(), eta, 1e-8, 500)["context"]
+    r = torch.from_numpy(np.random.default_rng(5).normal(size=ctx.dim)).cuda()
+    z = ctx.apply_preconditioner(r)
+    p = ctx.profile_preconditioner(r)
+    assert torch.equal(p["output"], z)
+    assert {"output", "host_wrapper_us", "host_memcpy_in_us", "host_solve_launch_us", "host_lowrank_total_us",
+            "host_final_sync_wait_us", "host_memcpy_out_us", "host_total_us", "gpu_cudss_solve_us",
+            "low_rank_active", "lowrank_setup_launch_host_us", "lowrank_sync_wait_host_us",
+            "lowrank_host_scalar_us", "lowrank_correction_launch_host_us", "lowrank_reduce_gpu_us",
+            "lowrank_correction_gpu_us"} <= set(p)
+    assert p["gpu_cudss_solve_us"] > 0 and p["low_rank_active"] == ctx.low_rank_enabled
+    rep = ctx.benchmark_preconditioner_repeated(r, 4)
+    assert torch.equal(rep["output"], z) and rep["repeats"] == 4 and rep["cpp_per_call_us"] > 0
+    base = ctx.profile_base_solve(r)
+    assert {"output", "host_wrapper_us", "host_memcpy_in_us", "host_cudss_execute_us", "host_sync_wait_us",
+            "host_memcpy_out_us", "host_total_us", "gpu_solve_us", "low_rank_enabled"} <= set(base)
+    A = _dense.csr_dense_sym(ctx.csr_values().cpu().numpy(), ctx.csr_colind().cpu().numpy(),
+                             ctx.csr_rowptr().cpu().numpy(), ctx.dim)
+    zb, rn = base["output"].cpu().numpy(), r.cpu().numpy()
+    berr = np.abs(A @ zb - rn).max() / (np.abs(A).sum(1).max() * np.abs(zb).max() + np.abs(rn).max())
+    assert berr < 1e-13, berr
+    brep = ctx.benchmark_base_solve_repeated(r, 3)
+    assert torch.equal(brep["output"], base["output"])
+    if ctx.low_rank_enabled:
+        assert not torch.equal(base["output"], z)
+    assert ctx.benchmark_diag_setup(3) > 0
+    with pytest.raises(RuntimeError):
+        ctx.benchmark_preconditioner_repeated(r, 0)
+
+
+@pytest.mark.parametrize("n,m,eta,seed", [(100, 120, 0.05, 3), (230, 150, 0.02, 8)])
 def test_backward_splr_preconditioner_vs_oracle(cg, n, m, eta, seed):
     from curegot_backward_amd.workloads import make_problem
     print(f"case n={n} m={m} eta={eta} seed={seed}", flush=True)
