@@ -414,7 +414,20 @@ def adjoint_main(args):
 
     fwd_bwd()                                   # warm-up
     tf, tb, grad_autograd = fwd_bwd()
-    out = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 1000)
+    # the autograd op's overhead over the native forward it wraps (the
+    # reference's target is < 5%: benchmark_stage4_autograd.py:37-42)
+    t_nat, t_ag = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 1000)
+        torch.cuda.synchronize()
+        t_nat.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        cg.torch.splr_ot(M, a, b, eta=eta, tol=1e-8)
+        torch.cuda.synchronize()
+        t_ag.append(time.perf_counter() - t0)
+    ag_overhead = min(t_ag) / min(t_nat) - 1.0
     G = 2 * out["plan"] + R
     pre = {}
     for name in ("diag", "splr", "none"):
@@ -476,6 +489,7 @@ def adjoint_main(args):
         "config": {"workload": "n=m=4096, d=64 N(0,I) points, M=mean squared distance, uniform marginals, "
                                "eta=0.05, loss=sum(T^2)+<T,R>, R~N(0,1), tol 1e-8", "seed": args.seed},
         "forward_ms": 1000 * tf, "backward_ms": 1000 * tb, "forward_niter": out["niter"],
+        "autograd_forward_overhead": ag_overhead,
         "backward_preconditioners": pre,
         "roofline_qv": qv_roof,
         "autograd_vs_manual_grad_rel": float((grad_autograd - cg.solve_backward(
