@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
     ap.add_argument("--serve", action="store_true", help="measure the serving loop on the mixed-size stream")
     ap.add_argument("--single", action="store_true", help="single n=m=2048 instance latency beside the CPU oracle")
+    ap.add_argument("--lse", action="store_true",
+                    help="row/column LSE and T pass in isolation on one n=m=16384 instance (2 GiB of M)")
     ap.add_argument("--adjoint", action="store_true",
                     help="n=m=4096 forward + backward times and dM vs central finite differences")
     ap.add_argument("--fd", type=int, default=64, help="entries checked by finite differences (--adjoint)")
@@ -92,6 +94,9 @@ def main():
         return
     if args.adjoint:
         adjoint_main(args)
+        return
+    if args.lse:
+        lse_main(args)
         return
     if args.single:
         single_main(args)
@@ -376,6 +381,70 @@ def single_main(args):
         "tpass_hbm": {"achieved": ach, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                       "note": "M is 32 MiB and stays in the 256 MB last-level cache between passes"},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+LSE_METRIC = "row/column log-sum-exp HBM throughput in isolation, n=m=16384 FP64 (2 GiB of M)"
+
+
+def lse_main(args):
+    """The library's LSE half-sweeps (k_row_lse + k_rowlse_fin, k_col_lse +
+    k_collse_fin) and one T pass, each on one n = m = 16384 instance: M is
+    2 GiB, far past the 256 MB MALL, so every pass streams M from HBM.
+    Per-launch HIP events on the solve stream; algorithmic bytes as in the
+    batch bench (LSE: 8nm + 8(n+m) + 8 max(n,m); T pass at a fixed point: 8nm + 16(n+m))."""
+    import curegot_backward_amd as cg
+    from curegot_backward_amd import _lib, _native
+
+    dev = torch.device("cuda", 0)
+    n = m = 16384
+    g = torch.Generator(device="cpu").manual_seed(args.seed)
+    X = torch.randn(n, 16, generator=g, dtype=torch.float64).to(dev)
+    Y = torch.randn(m, 16, generator=g, dtype=torch.float64).to(dev)
+    M = torch.cdist(X, Y).pow_(2)
+    M /= M.max()
+    a = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+    b = torch.full((m,), 1.0 / m, dtype=torch.float64, device=dev)
+    reg = 0.05
+    dual = torch.zeros(n + m, dtype=torch.float64, device=dev)
+    reps = max(args.steps, 2)
+    res = {}
+    for which, name in ((0, "row_lse"), (1, "col_lse")):
+        _native.lse_update(M, a, b, dual, reg, which, 2)          # warm-up
+        torch.cuda.synchronize()
+        _lib.profile_reset()
+        _lib.profile_enable(True)
+        _native.lse_update(M, a, b, dual, reg, which, reps)
+        torch.cuda.synchronize()
+        _lib.profile_enable(False)
+        ms, work, nl = _lib.profile_read(name)
+        ach = work / ms / 1e6
+        res[name] = {"kernel": f"k_{name}", "launches": nl, "avg_launch_us": 1000 * ms / nl, "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+        log(f"{name}: {nl} launches, {1000 * ms / nl:.1f} us avg, {ach:.0f} GB/s ({ach / HBM_PEAK_GBS:.3f})")
+    alpha, beta = dual[:n].clone(), dual[n:].clone()
+    beta -= beta[-1].item()
+    _native.T_sparsify(alpha, beta, M, a, b, reg, 0.0, 1000)          # warm-up
+    torch.cuda.synchronize()
+    _lib.profile_reset()
+    _lib.profile_enable(True)
+    for _ in range(3):
+        _native.T_sparsify(alpha, beta, M, a, b, reg, 0.0, 1000)
+    torch.cuda.synchronize()
+    _lib.profile_enable(False)
+    ms, work, nl = _lib.profile_read("tpass")
+    if nl:
+        ach = work / ms / 1e6
+        res["tpass"] = {"kernel": "k_tpass", "launches": nl, "avg_launch_us": 1000 * ms / nl, "achieved": ach,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+        log(f"tpass: {nl} launches, {1000 * ms / nl:.1f} us avg, {ach:.0f} GB/s ({ach / HBM_PEAK_GBS:.3f})")
+    result = {
+        "metric": LSE_METRIC, "value": res["row_lse"]["achieved"], "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "n=m=16384, d=16 N(0,I) points, M=|x-y|^2/max, uniform marginals, reg 0.05",
+                   "seed": args.seed, "repeats": reps},
+        "roofline_hbm": res,
     }
     print(json.dumps(result), flush=True)
 

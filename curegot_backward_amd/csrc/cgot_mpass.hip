@@ -33,6 +33,30 @@ CG_DEV int cg_strip_c(const CgInstD& I, int st)
     return (cw <= 64) ? 1 : (cw <= 128) ? 2 : CG_CMAX;
 }
 
+// Calls fn(tag) with tag::C the strip's columns per lane and tag::FULL true
+// when all 64*C columns of the strip lie inside m (every strip but, at most,
+// an instance's last): the tile bodies then drop their per-column bounds
+// tests, which otherwise put every load under its own exec-mask branch.
+template <int C_, bool FULL_>
+struct CgStrip {
+    static constexpr int C = C_;
+    static constexpr bool FULL = FULL_;
+};
+
+template <class Fn>
+CG_DEV void cg_strip_dispatch(const CgInstD& I, int st, Fn&& fn)
+{
+    const int c = cg_strip_c(I, st);
+    const bool full = I.m - st * I.sw >= 64 * c;
+    if (c == 1) {
+        if (full) fn(CgStrip<1, true>{}); else fn(CgStrip<1, false>{});
+    } else if (c == 2) {
+        if (full) fn(CgStrip<2, true>{}); else fn(CgStrip<2, false>{});
+    } else {
+        if (full) fn(CgStrip<CG_CMAX, true>{}); else fn(CgStrip<CG_CMAX, false>{});
+    }
+}
+
 CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const double* ctl,
                            int& b, int& rb, int& st)
 {
@@ -53,7 +77,7 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
 // Each wave takes its CG_RPW rows in groups of R and issues the loads of a
 // whole group before the exponentials, so R*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, bool FULL>
 CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, double (*lds)[CG_CMAX * 64],
                           const double* tab)
 {
@@ -65,7 +89,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
 #pragma unroll
     for (int k = 0; k < C; k++) {
         const int j = s0 + lane + 64 * k;
-        ok[k] = j < m;
+        ok[k] = FULL || j < m;
         bj[k] = ok[k] ? cg_point(I, mode, t, n + j) : 0.0;
         cs[k] = 0.0;
     }
@@ -126,11 +150,10 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
-    switch (cg_strip_c(I, st)) {
-        case 1: cg_tpass_tile<1>(I, mode, t, rb, st, lds, tab); break;
-        case 2: cg_tpass_tile<2>(I, mode, t, rb, st, lds, tab); break;
-        default: cg_tpass_tile<CG_CMAX>(I, mode, t, rb, st, lds, tab); break;
-    }
+    cg_strip_dispatch(I, st, [&](auto tag) {
+        using S = decltype(tag);
+        cg_tpass_tile<S::C, S::FULL>(I, mode, t, rb, st, lds, tab);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -139,7 +162,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // Per (row, strip): wave max, then sum of exp(x - max).  Each wave takes its
 // CG_RPW rows in groups of R with the loads of a whole group issued first.
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, bool FULL>
 CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -152,7 +175,7 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
     for (int k = 0; k < C; k++) {
         const int j = s0 + lane + 64 * k;
-        ok[k] = j < m;
+        ok[k] = FULL || j < m;
         bj[k] = ok[k] ? beta[j] * ir : 0.0;
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
@@ -202,11 +225,10 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
-    switch (cg_strip_c(I, st)) {
-        case 1: cg_rowlse_tile<1>(I, mode, rb, st, tab); break;
-        case 2: cg_rowlse_tile<2>(I, mode, rb, st, tab); break;
-        default: cg_rowlse_tile<CG_CMAX>(I, mode, rb, st, tab); break;
-    }
+    cg_strip_dispatch(I, st, [&](auto tag) {
+        using S = decltype(tag);
+        cg_rowlse_tile<S::C, S::FULL>(I, mode, rb, st, tab);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -216,7 +238,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // with an online max/sum, reads stay row-major and coalesced, R rows' loads
 // in flight at a time).
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, bool FULL>
 CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab,
                            double (*lmx)[CG_CMAX * 64], double (*lsm)[CG_CMAX * 64])
 {
@@ -231,7 +253,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
     for (int k = 0; k < C; k++) {
         mx[k] = -INFINITY;
         sm[k] = 0.0;
-        ok[k] = s0 + lane + 64 * k < m;
+        ok[k] = FULL || s0 + lane + 64 * k < m;
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
     constexpr int R = (C >= 4) ? 2 : 4;
@@ -289,11 +311,10 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
     __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
-    switch (cg_strip_c(I, st)) {
-        case 1: cg_collse_tile<1>(I, mode, rb, st, tab, lmx, lsm); break;
-        case 2: cg_collse_tile<2>(I, mode, rb, st, tab, lmx, lsm); break;
-        default: cg_collse_tile<CG_CMAX>(I, mode, rb, st, tab, lmx, lsm); break;
-    }
+    cg_strip_dispatch(I, st, [&](auto tag) {
+        using S = decltype(tag);
+        cg_collse_tile<S::C, S::FULL>(I, mode, rb, st, tab, lmx, lsm);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // Reference: q_matvec / rhs_Q_scale, curegot/_backward_core.py:31-45 (3-4 torch
 // passes over a stored T there, one pass over M here).
 // ---------------------------------------------------------------------------
-template <int C, bool RHS>
+template <int C, bool FULL, bool RHS>
 CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
                        const double* tab)
 {
@@ -322,7 +343,7 @@ CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* co
 #pragma unroll
     for (int k = 0; k < C; k++) {
         const int j = s0 + lane + 64 * k;
-        ok[k] = j < m;
+        ok[k] = FULL || j < m;
         bj[k] = ok[k] ? I.gamma[n + j] : 0.0;
         vb[k] = (MV && ok[k] && j < m - 1) ? vat(n + j) : 0.0;   // beta_{m-1} is fixed
         cs[k] = 0.0;
@@ -406,11 +427,10 @@ template <bool RHS>
 CG_DEV void cg_qv_strip(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
                         const double* tab)
 {
-    switch (cg_strip_c(I, st)) {
-        case 1: cg_qv_tile<1, RHS>(I, b, rb, st, Gs, lds, tab); break;
-        case 2: cg_qv_tile<2, RHS>(I, b, rb, st, Gs, lds, tab); break;
-        default: cg_qv_tile<CG_CMAX, RHS>(I, b, rb, st, Gs, lds, tab); break;
-    }
+    cg_strip_dispatch(I, st, [&](auto tag) {
+        using S = decltype(tag);
+        cg_qv_tile<S::C, S::FULL, RHS>(I, b, rb, st, Gs, lds, tab);
+    });
 }
 
 // once per adjoint solve (loads G beside M; no register cap)
