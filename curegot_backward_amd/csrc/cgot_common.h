@@ -176,6 +176,15 @@ CG_DEV double cg_dpp_d(double v)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
+// Orders the LDS accesses of one wave: every lane's earlier LDS writes are
+// visible to every lane's later LDS reads (no workgroup barrier).
+CG_DEV void cg_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave reductions, all 64 lanes active; the result is the same in every lane.
 // Inside each 16-lane row: DPP exchanges with lane^1 and lane^2 (quad
 // permutes), then with the half-row and row mirrors; the four row results are

@@ -159,12 +159,6 @@ CG_DEV bool cg_decode_wave(const CgInst* insts, const int* wave_inst, const doub
 // one group after another, so every entry receives its terms in a fixed
 // order; the finished row is written once.  Longer rows update S in place, l
 // by l in ascending order with a fence between l steps.
-CG_DEV void cg_wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __global__ __launch_bounds__(CG_WG) void k_schur_build(const CgInst* __restrict__ insts,
                                                        const int* __restrict__ wave_inst,

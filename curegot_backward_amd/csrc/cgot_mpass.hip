@@ -159,11 +159,48 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // ---------------------------------------------------------------------------
 // Row log-sum-exp partials of (beta_j - M_ij)/reg for the alpha update.
 // Reference: optimal_alpha_kernel, src/sinkhorn_bcd_kernel.cu:115-192.
-// Per (row, strip): wave max, then sum of exp(x - max).  Each wave takes its
-// CG_RPW rows in groups of R with the loads of a whole group issued first.
+// Per (row, strip): max over the strip, then sum of exp(x - max).  A wave
+// takes its CG_RPW rows in groups of R with the loads of a group issued
+// first, and reduces the R rows together through LDS (cg_rows_reduce) instead
+// of one 64-lane DPP/readlane reduction per row and quantity.
 // ---------------------------------------------------------------------------
+
+// Reduction (max or sum) of R rows held as one partial per (row, lane) in
+// v[r]: the partials go to this wave's LDS block red ([R][64]); lane l then
+// takes row q = l / G (G = 64 / R lanes per row, gi = l % G), combines the
+// row's partials gi, gi + G, gi + 2G, ... in that order, and the G lanes of
+// the row finish with a DPP tree inside their 16-lane row (G = 8 or 16) and,
+// for G = 32, one exchange with lane ^ 16.  Every lane of row q's group ends
+// with the same value.
+template <int R, bool MAX>
+CG_DEV double cg_rows_reduce(const double (&v)[R], double* red)
+{
+    static_assert(R == 2 || R == 4 || R == 8, "32, 16 or 8 lanes per row");
+    constexpr int G = 64 / R;
+    const int lane = threadIdx.x & 63, q = lane / G, gi = lane % G;
+#pragma unroll
+    for (int r = 0; r < R; r++) red[r * 64 + lane] = v[r];
+    cg_wave_sync();
+    double a = red[q * 64 + gi];
+#pragma unroll
+    for (int j = 1; j < R; j++) {
+        const double b = red[q * 64 + gi + G * j];
+        a = MAX ? fmax(a, b) : a + b;
+    }
+    a = MAX ? fmax(a, cg_dpp_d<0xB1>(a)) : a + cg_dpp_d<0xB1>(a);     // lane ^ 1
+    a = MAX ? fmax(a, cg_dpp_d<0x4E>(a)) : a + cg_dpp_d<0x4E>(a);     // lane ^ 2
+    a = MAX ? fmax(a, cg_dpp_d<0x141>(a)) : a + cg_dpp_d<0x141>(a);   // half-row mirror: 8 lanes
+    if (G >= 16) a = MAX ? fmax(a, cg_dpp_d<0x140>(a)) : a + cg_dpp_d<0x140>(a);   // row mirror: 16 lanes
+    if (G == 32) {
+        const double o = __shfl_xor(a, 16);
+        a = MAX ? fmax(a, o) : a + o;
+    }
+    return a;
+}
+
 template <int C, bool FULL>
-CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab)
+CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab, double* red,
+                           double* rmx)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw;
@@ -179,7 +216,9 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
         bj[k] = ok[k] ? beta[j] * ir : 0.0;
     }
     const int i0 = rb * CG_RB + w * CG_RPW;
-    constexpr int R = (C >= 4) ? 4 : 8;   // R*C loads per lane in flight
+    constexpr int R = 8 / C;   // 8 loads per lane in flight per group of R rows
+    constexpr int G = 64 / R;
+    const int q = lane / G, gi = lane % G;
 #pragma unroll
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C];
@@ -190,29 +229,35 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
             for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
         }
+        // x = (beta_j - M_ij)/reg, -inf past m; formed again for the sums
+        // instead of being held
+        auto xat = [&](int r, int k) { return ok[k] ? fma(-mv[r][k], ir, bj[k]) : -INFINITY; };
+        double part[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = i0 + r0 + r;
-            double x[C], mx = -INFINITY;
+            double mx = xat(r, 0);
 #pragma unroll
-            for (int k = 0; k < C; k++) {
-                x[k] = ok[k] ? fma(-mv[r][k], ir, bj[k]) : -INFINITY;
-                mx = fmax(mx, x[k]);
-            }
-            mx = cg_wave_max(mx);
-            // x = -inf past m: cg_texp_lo gives 0 (or the smallest subnormal,
-            // far below the rounding of a sum that contains exp(0) = 1)
-            double e[C];
+            for (int k = 1; k < C; k++) mx = fmax(mx, xat(r, k));
+            part[r] = mx;
+        }
+        const double rowmax = cg_rows_reduce<R, true>(part, red);   // finite: column s0 is in the strip
+        if (gi == 0) rmx[q] = rowmax;
+        cg_wave_sync();
+        // exp(-inf) past m is 0 or the smallest subnormal (cg_texp_lo), far
+        // below the rounding of a sum that contains exp(0) = 1
 #pragma unroll
-            for (int k = 0; k < C; k++) e[k] = cg_texp_lo(x[k] - mx, tab);
-            double sm = e[0];
+        for (int r = 0; r < R; r++) {
+            const double mr = rmx[r];
+            double sm = cg_texp_lo(xat(r, 0) - mr, tab);
 #pragma unroll
-            for (int k = 1; k < C; k++) sm += e[k];
-            sm = cg_wave_sum(sm);
-            if (lane == 0 && i < n) {
-                I.rp0[(size_t)i * I.nstrips + st] = mx;
-                I.rp1[(size_t)i * I.nstrips + st] = sm;
-            }
+            for (int k = 1; k < C; k++) sm += cg_texp_lo(xat(r, k) - mr, tab);
+            part[r] = sm;
+        }
+        const double rowsum = cg_rows_reduce<R, false>(part, red);
+        const int i = i0 + r0 + q;
+        if (gi == 0 && i < n) {
+            I.rp0[(size_t)i * I.nstrips + st] = rowmax;
+            I.rp1[(size_t)i * I.nstrips + st] = rowsum;
         }
     }
 }
@@ -225,9 +270,12 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     int b, rb, st;
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
+    __shared__ double red[CG_WAVES][8 * 64];   // row partials of a group, per wave
+    __shared__ double rmx[CG_WAVES][8];        // row maxima of a group, per wave
+    const int w = threadIdx.x >> 6;
     cg_strip_dispatch(I, st, [&](auto tag) {
         using S = decltype(tag);
-        cg_rowlse_tile<S::C, S::FULL>(I, mode, rb, st, tab);
+        cg_rowlse_tile<S::C, S::FULL>(I, mode, rb, st, tab, red[w], rmx[w]);
     });
 }
 
