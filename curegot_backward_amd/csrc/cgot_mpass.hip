@@ -70,6 +70,39 @@ CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const doub
     return true;
 }
 
+// Reduction (max or sum) of R rows held as one partial per (row, lane) in
+// v[r]: the partials go to this wave's LDS block red ([R][64]); lane l then
+// takes row q = l / G (G = 64 / R lanes per row, gi = l % G), combines the
+// row's partials gi, gi + G, gi + 2G, ... in that order, and the G lanes of
+// the row finish with a DPP tree inside their 16-lane row (G = 8 or 16) and,
+// for G = 32, one exchange with lane ^ 16.  Every lane of row q's group ends
+// with the same value.
+template <int R, bool MAX>
+CG_DEV double cg_rows_reduce(const double (&v)[R], double* red)
+{
+    static_assert(R == 2 || R == 4 || R == 8, "32, 16 or 8 lanes per row");
+    constexpr int G = 64 / R;
+    const int lane = threadIdx.x & 63, q = lane / G, gi = lane % G;
+#pragma unroll
+    for (int r = 0; r < R; r++) red[r * 64 + lane] = v[r];
+    cg_wave_sync();
+    double a = red[q * 64 + gi];
+#pragma unroll
+    for (int j = 1; j < R; j++) {
+        const double b = red[q * 64 + gi + G * j];
+        a = MAX ? fmax(a, b) : a + b;
+    }
+    a = MAX ? fmax(a, cg_dpp_d<0xB1>(a)) : a + cg_dpp_d<0xB1>(a);     // lane ^ 1
+    a = MAX ? fmax(a, cg_dpp_d<0x4E>(a)) : a + cg_dpp_d<0x4E>(a);     // lane ^ 2
+    a = MAX ? fmax(a, cg_dpp_d<0x141>(a)) : a + cg_dpp_d<0x141>(a);   // half-row mirror: 8 lanes
+    if (G >= 16) a = MAX ? fmax(a, cg_dpp_d<0x140>(a)) : a + cg_dpp_d<0x140>(a);   // row mirror: 16 lanes
+    if (G == 32) {
+        const double o = __shfl_xor(a, 16);
+        a = MAX ? fmax(a, o) : a + o;
+    }
+    return a;
+}
+
 // ---------------------------------------------------------------------------
 // T pass: T_ij = exp((alpha_i + beta_j - M_ij)/reg); row and column partial sums.
 // Reference: T_fused_kernel / line_search_fused_kernel2,
@@ -98,9 +131,14 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
     // rows per load group: R*C loads per lane in flight; the group loop stays
     // rolled so that only one group's loads hold registers
     constexpr int R = (C >= 4) ? 2 : 4;
+    constexpr int G = 64 / R;
+    const int q = lane / G, gi = lane % G;
+    // the row sums of a group are reduced together through lds[w], which the
+    // column combine only uses after the loop (R*64 <= CG_CMAX*64)
+    static_assert(R * 64 <= CG_CMAX * 64, "group partials fit the wave's column buffer");
 #pragma unroll 1
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
-        double mv[R][C], ai[R];
+        double mv[R][C], ai[R], part[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
@@ -127,10 +165,13 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
 #pragma unroll
                 for (int k = 0; k < C; k++) cs[k] += v[k];
             }
-            rs = cg_wave_sum(rs);
-            if (lane == 0 && i < n) I.rp0[(size_t)i * I.nstrips + st] = rs;
+            part[r] = rs;
         }
+        const double rowsum = cg_rows_reduce<R, false>(part, lds[w]);
+        const int i = i0 + r0 + q;
+        if (gi == 0 && i < n) I.rp0[(size_t)i * I.nstrips + st] = rowsum;
     }
+    cg_wave_sync();   // the group reductions' reads of lds[w] before the column partials overwrite it
 #pragma unroll
     for (int k = 0; k < C; k++) lds[w][lane + 64 * k] = cs[k];
     __syncthreads();
@@ -164,39 +205,6 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // first, and reduces the R rows together through LDS (cg_rows_reduce) instead
 // of one 64-lane DPP/readlane reduction per row and quantity.
 // ---------------------------------------------------------------------------
-
-// Reduction (max or sum) of R rows held as one partial per (row, lane) in
-// v[r]: the partials go to this wave's LDS block red ([R][64]); lane l then
-// takes row q = l / G (G = 64 / R lanes per row, gi = l % G), combines the
-// row's partials gi, gi + G, gi + 2G, ... in that order, and the G lanes of
-// the row finish with a DPP tree inside their 16-lane row (G = 8 or 16) and,
-// for G = 32, one exchange with lane ^ 16.  Every lane of row q's group ends
-// with the same value.
-template <int R, bool MAX>
-CG_DEV double cg_rows_reduce(const double (&v)[R], double* red)
-{
-    static_assert(R == 2 || R == 4 || R == 8, "32, 16 or 8 lanes per row");
-    constexpr int G = 64 / R;
-    const int lane = threadIdx.x & 63, q = lane / G, gi = lane % G;
-#pragma unroll
-    for (int r = 0; r < R; r++) red[r * 64 + lane] = v[r];
-    cg_wave_sync();
-    double a = red[q * 64 + gi];
-#pragma unroll
-    for (int j = 1; j < R; j++) {
-        const double b = red[q * 64 + gi + G * j];
-        a = MAX ? fmax(a, b) : a + b;
-    }
-    a = MAX ? fmax(a, cg_dpp_d<0xB1>(a)) : a + cg_dpp_d<0xB1>(a);     // lane ^ 1
-    a = MAX ? fmax(a, cg_dpp_d<0x4E>(a)) : a + cg_dpp_d<0x4E>(a);     // lane ^ 2
-    a = MAX ? fmax(a, cg_dpp_d<0x141>(a)) : a + cg_dpp_d<0x141>(a);   // half-row mirror: 8 lanes
-    if (G >= 16) a = MAX ? fmax(a, cg_dpp_d<0x140>(a)) : a + cg_dpp_d<0x140>(a);   // row mirror: 16 lanes
-    if (G == 32) {
-        const double o = __shfl_xor(a, 16);
-        a = MAX ? fmax(a, o) : a + o;
-    }
-    return a;
-}
 
 template <int C, bool FULL>
 CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab, double* red,
