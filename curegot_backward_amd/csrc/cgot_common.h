@@ -258,10 +258,6 @@ CG_DEV double cg_texp_m(double y, const double* tab)
 // smallest subnormal, and the clamped argument gives 0 or that subnormal.
 CG_DEV double cg_texp_lo(double y, const double* tab) { return cg_texp_m(fmax(y, -745.0), tab); }
 
-// exp(y) for any y, -inf included: 0 or the smallest subnormal below -745,
-// +inf above 710 (exp overflows from 709.79 on).
-CG_DEV double cg_texp_c(double y, const double* tab) { return cg_texp_m(fmin(fmax(y, -745.0), 710.0), tab); }
-
 // Sum over a 256-thread workgroup; result valid in every thread.
 CG_DEV double cg_block_sum(double v, double* lds4)
 {
@@ -272,16 +268,6 @@ CG_DEV double cg_block_sum(double v, double* lds4)
     __syncthreads();
     double r = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
     return r;
-}
-
-CG_DEV double cg_block_max(double v, double* lds4)
-{
-    v = cg_wave_max(v);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds4[w] = v;
-    __syncthreads();
-    return fmax(fmax(lds4[0], lds4[1]), fmax(lds4[2], lds4[3]));
 }
 
 // Merge of two (max, sum-of-exp) log-sum-exp partials.
