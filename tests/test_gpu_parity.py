@@ -213,6 +213,29 @@ def test_splr_preconditioner_apply_vs_dense(cg, n, m, eta, seed):
     assert rel(z, zref) < 1e-14 * np.linalg.cond(B) + 1e-12
 
 
+@pytest.mark.parametrize("solver", ["sinkhorn_bcd", "sinkhorn_splr"])
+def test_warm_start(cg, solver):
+    """The reference's warm-start checks (test/test_sinkhorn_bcd.py:239-341):
+    restarting from the converged dual takes no more iterations and gives the
+    same plan (Frobenius difference <= 1e-8), the marginals hold to 1e-6, and a
+    perturbed start converges to the same plan.  Inputs: synthetic data of the
+    same shape and kind (uniform M, normalised uniform marginals)."""
+    rng = np.random.default_rng(789)
+    n, m, reg = 50, 40, 0.01
+    M = rng.random((n, m))
+    a = rng.random(n); a /= a.sum()
+    b = rng.random(m); b /= b.sum()
+    f = getattr(cg.numpy, solver)
+    r1 = f(M, a, b, reg, 1e-8, 2000)
+    r2 = f(M, a, b, reg, 1e-8, 2000, x0=r1["dual"])
+    r3 = f(M, a, b, reg, 1e-8, 2000, x0=r1["dual"] + 0.01 * rng.standard_normal(n + m))
+    print(solver, "niter", r1["niter"], r2["niter"], r3["niter"], flush=True)
+    assert r2["niter"] <= r1["niter"]
+    assert np.linalg.norm(r1["plan"] - r2["plan"]) <= 1e-8
+    assert np.linalg.norm(r1["plan"].sum(1) - a) <= 1e-6 and np.linalg.norm(r1["plan"].sum(0) - b) <= 1e-6
+    assert np.linalg.norm(r1["plan"] - r3["plan"]) <= 1e-6
+
+
 @pytest.mark.parametrize("n,m,eta,seed", [(100, 120, 0.05, 3), (230, 150, 0.02, 8)])
 def test_context_csr_vs_oracle(cg, n, m, eta, seed):
     """csr_values / csr_colind / csr_rowptr of the context (reference
