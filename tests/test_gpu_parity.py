@@ -236,6 +236,25 @@ def test_warm_start(cg, solver):
     assert np.linalg.norm(r1["plan"] - r3["plan"]) <= 1e-6
 
 
+@pytest.mark.parametrize("n,m,reg", [(80, 60, 0.01), (60, 80, 0.001), (800, 600, 0.01), (600, 800, 0.001)])
+def test_splr_vs_bcd(cg, n, m, reg):
+    """The reference's SPLR-against-BCD check (test/test_sinkhorn_splr.py:45-118,
+    cases 120-146): tol 1e-6, max_iter 5000, the plans agree to a relative
+    Frobenius error below 1e-2 (the reference fails above that), and the SPLR
+    marginals hold to 1e-4.  Synthetic inputs of the same kind (uniform M,
+    normalised uniform marginals)."""
+    rng = np.random.default_rng(n * 1000 + m)
+    M = rng.random((n, m))
+    a = rng.random(n); a /= a.sum()
+    b = rng.random(m); b /= b.sum()
+    bcd = cg.numpy.sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=5000)
+    splr = cg.numpy.sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=5000)
+    err = np.linalg.norm(splr["plan"] - bcd["plan"]) / np.linalg.norm(bcd["plan"])
+    print(f"n={n} m={m} reg={reg}: bcd {bcd['niter']} it, splr {splr['niter']} it, rel err {err:.2e}", flush=True)
+    assert err < 1e-2
+    assert np.linalg.norm(splr["plan"].sum(1) - a) < 1e-4 and np.linalg.norm(splr["plan"].sum(0) - b) < 1e-4
+
+
 @pytest.mark.parametrize("n,m,eta,seed", [(100, 120, 0.05, 3), (230, 150, 0.02, 8)])
 def test_context_csr_vs_oracle(cg, n, m, eta, seed):
     """csr_values / csr_colind / csr_rowptr of the context (reference
