@@ -213,6 +213,30 @@ def test_splr_preconditioner_apply_vs_dense(cg, n, m, eta, seed):
     assert rel(z, zref) < 1e-14 * np.linalg.cond(B) + 1e-12
 
 
+def test_bcd_convergence_mechanism(cg):
+    """The reference's BCD convergence checks (test/test_sinkhorn_bcd.py:123-237):
+    niter within max_iter for every tolerance, max_iter = 10 truncates at 10,
+    the marginal error is within 10 tol at tol 1e-4 and 1e-6, and a converged
+    run has smaller marginal error than one cut at 5 sweeps.  Synthetic inputs
+    of the same kind (uniform M, normalised uniform marginals)."""
+    rng = np.random.default_rng(456)
+    n, m, reg = 50, 40, 0.01
+    M = rng.random((n, m))
+    a = rng.random(n); a /= a.sum()
+    b = rng.random(m); b /= b.sum()
+    f = cg.numpy.sinkhorn_bcd
+
+    def merr(plan):
+        return float(np.hypot(np.linalg.norm(plan.sum(1) - a), np.linalg.norm(plan.sum(0) - b)))
+
+    its = [f(M, a, b, reg, tol=tol, max_iter=1000)["niter"] for tol in (1e-2, 1e-4, 1e-6, 1e-8)]
+    assert all(0 < it <= 1000 for it in its) and its == sorted(its)
+    assert f(M, a, b, reg, tol=1e-10, max_iter=10)["niter"] == 10
+    for tol in (1e-4, 1e-6):
+        assert merr(f(M, a, b, reg, tol=tol, max_iter=1000)["plan"]) <= 10 * tol
+    assert merr(f(M, a, b, reg, tol=1e-6, max_iter=1000)["plan"]) < merr(f(M, a, b, reg, tol=1e-6, max_iter=5)["plan"])
+
+
 @pytest.mark.parametrize("solver", ["sinkhorn_bcd", "sinkhorn_splr"])
 def test_warm_start(cg, solver):
     """The reference's warm-start checks (test/test_sinkhorn_bcd.py:239-341):
