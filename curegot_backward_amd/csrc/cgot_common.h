@@ -270,6 +270,40 @@ CG_DEV double cg_block_sum(double v, double* lds4)
     return r;
 }
 
+// Columns per lane for strip st of a pass over the columns j < mlim (m for
+// the M passes, m - 1 for the top-K kernels): the instance's c, or fewer for
+// a last strip that is narrower (its idle lanes would otherwise still issue
+// the work).
+CG_DEV int cg_strip_c(const CgInstD& I, int st, int mlim)
+{
+    const int cw = min(I.sw, mlim - st * I.sw);
+    return (cw <= 64) ? 1 : (cw <= 128) ? 2 : CG_CMAX;
+}
+
+// Calls fn(tag) with tag::C the strip's columns per lane and tag::FULL true
+// when all 64*C columns of the strip lie below mlim (every strip but, at
+// most, an instance's last): the tile bodies then drop their per-column
+// bounds tests, which otherwise put every load under its own exec-mask branch.
+template <int C_, bool FULL_>
+struct CgStrip {
+    static constexpr int C = C_;
+    static constexpr bool FULL = FULL_;
+};
+
+template <class Fn>
+CG_DEV void cg_strip_dispatch(const CgInstD& I, int st, int mlim, Fn&& fn)
+{
+    const int c = cg_strip_c(I, st, mlim);
+    const bool full = mlim - st * I.sw >= 64 * c;
+    if (c == 1) {
+        if (full) fn(CgStrip<1, true>{}); else fn(CgStrip<1, false>{});
+    } else if (c == 2) {
+        if (full) fn(CgStrip<2, true>{}); else fn(CgStrip<2, false>{});
+    } else {
+        if (full) fn(CgStrip<CG_CMAX, true>{}); else fn(CgStrip<CG_CMAX, false>{});
+    }
+}
+
 // Merge of two (max, sum-of-exp) log-sum-exp partials.
 CG_DEV void cg_lse_merge(double& mx, double& s, double mx2, double s2)
 {

@@ -25,38 +25,6 @@ CG_DEV double cg_point(const CgInstD& I, int mode, double t, int k)
     return (k < I.N) ? (I.gamma_prev[k] + t * I.dir[k]) : 0.0;
 }
 
-// Columns per lane for strip st: the instance's c, or fewer for a last strip
-// that is narrower (its idle lanes would otherwise still issue the work).
-CG_DEV int cg_strip_c(const CgInstD& I, int st)
-{
-    const int cw = min(I.sw, I.m - st * I.sw);
-    return (cw <= 64) ? 1 : (cw <= 128) ? 2 : CG_CMAX;
-}
-
-// Calls fn(tag) with tag::C the strip's columns per lane and tag::FULL true
-// when all 64*C columns of the strip lie inside m (every strip but, at most,
-// an instance's last): the tile bodies then drop their per-column bounds
-// tests, which otherwise put every load under its own exec-mask branch.
-template <int C_, bool FULL_>
-struct CgStrip {
-    static constexpr int C = C_;
-    static constexpr bool FULL = FULL_;
-};
-
-template <class Fn>
-CG_DEV void cg_strip_dispatch(const CgInstD& I, int st, Fn&& fn)
-{
-    const int c = cg_strip_c(I, st);
-    const bool full = I.m - st * I.sw >= 64 * c;
-    if (c == 1) {
-        if (full) fn(CgStrip<1, true>{}); else fn(CgStrip<1, false>{});
-    } else if (c == 2) {
-        if (full) fn(CgStrip<2, true>{}); else fn(CgStrip<2, false>{});
-    } else {
-        if (full) fn(CgStrip<CG_CMAX, true>{}); else fn(CgStrip<CG_CMAX, false>{});
-    }
-}
-
 CG_DEV bool cg_decode_tile(const CgInst* insts, const int* tile_inst, const double* ctl,
                            int& b, int& rb, int& st)
 {
@@ -191,7 +159,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
-    cg_strip_dispatch(I, st, [&](auto tag) {
+    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
         cg_tpass_tile<S::C, S::FULL>(I, mode, t, rb, st, lds, tab);
     });
@@ -281,7 +249,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     __shared__ double red[CG_WAVES][8 * 64];   // row partials of a group, per wave
     __shared__ double rmx[CG_WAVES][8];        // row maxima of a group, per wave
     const int w = threadIdx.x >> 6;
-    cg_strip_dispatch(I, st, [&](auto tag) {
+    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
         cg_rowlse_tile<S::C, S::FULL>(I, mode, rb, st, tab, red[w], rmx[w]);
     });
@@ -367,7 +335,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
     __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
-    cg_strip_dispatch(I, st, [&](auto tag) {
+    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
         cg_collse_tile<S::C, S::FULL>(I, mode, rb, st, tab, lmx, lsm);
     });
@@ -483,7 +451,7 @@ template <bool RHS>
 CG_DEV void cg_qv_strip(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
                         const double* tab)
 {
-    cg_strip_dispatch(I, st, [&](auto tag) {
+    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
         cg_qv_tile<S::C, S::FULL, RHS>(I, b, rb, st, Gs, lds, tab);
     });

@@ -31,7 +31,54 @@ CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const dou
 }
 
 // Radix histogram pass.  pass 0: top 12 bits (sign+exponent); pass p>=1: the
-// next 13 bits among keys matching the prefix found so far.
+// next 13 bits among keys matching the prefix found so far.  A wave takes its
+// CG_RPW rows in groups of R with the loads of a group issued first; the
+// strip's beta and the rows' alpha are read once.  T is formed exactly as
+// cg_T forms it, so the selection kernels see the same keys.
+template <int C, bool FULL>
+CG_DEV void cg_hist_tile(const CgInstD& I, int rb, int st, int pass, unsigned int* h, const double* tab)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw, mm = I.m - 1, m = I.m, n = I.n;
+    const int shift = (pass == 0) ? 52 : 52 - CG_HBITS * pass;
+    const unsigned long long mask = (pass == 0) ? 4095ull : (unsigned long long)(CG_HBINS - 1);
+    const unsigned long long prefix = cg_kth(I);
+    const double ir = I.inv_reg;
+    double bj[C];
+    bool ok[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int j = s0 + lane + 64 * k;
+        ok[k] = FULL || j < mm;
+        bj[k] = ok[k] ? I.gamma[n + j] : 0.0;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    constexpr int R = 8 / C;   // 8 loads per lane in flight per group
+#pragma unroll 1
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        if (i0 + r0 >= n) break;
+        double mv[R][C], ai[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not counted
+            ai[r] = I.gamma[i];
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const bool live = i0 + r0 + r < n;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                const unsigned long long key = cg_key(cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab));
+                if (live && ok[k] && (pass == 0 || (key >> (shift + CG_HBITS)) == prefix))
+                    atomicAdd(&h[(key >> shift) & mask], 1u);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts,
                                                 const int* __restrict__ tile_inst,
                                                 const double* __restrict__ ctl, int pass)
@@ -43,24 +90,10 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
     __shared__ unsigned int h[CG_HBINS];
     for (int q = threadIdx.x; q < CG_HBINS; q += CG_WG) h[q] = 0u;
     __syncthreads();
-    const int shift = (pass == 0) ? 52 : 52 - CG_HBITS * pass;
-    const unsigned long long mask = (pass == 0) ? 4095ull : (unsigned long long)(CG_HBINS - 1);
-    const unsigned long long prefix = cg_kth(I);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int s0 = st * I.sw, mm = I.m - 1;
-    const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= I.n) break;
-        for (int k = 0; k < I.c; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (j < mm) {
-                const unsigned long long key = cg_key(cg_T(I, i, j, tab));
-                if (pass == 0 || (key >> (shift + CG_HBITS)) == prefix)
-                    atomicAdd(&h[(key >> shift) & mask], 1u);
-            }
-        }
-    }
+    cg_strip_dispatch(I, st, I.m - 1, [&](auto tag) {
+        using S = decltype(tag);
+        cg_hist_tile<S::C, S::FULL>(I, rb, st, pass, h, tab);
+    });
     __syncthreads();
     const int nb = (pass == 0) ? 4096 : CG_HBINS;
     for (int q = threadIdx.x; q < nb; q += CG_WG) {
