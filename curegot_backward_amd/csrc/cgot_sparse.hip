@@ -280,6 +280,55 @@ CG_DEV bool cg_selected(unsigned long long key, unsigned long long kth, int j, i
     return key > kth || (key == kth && j < cut);
 }
 
+// Selected entries of a tile for the wave's CG_RPW rows: bit r of bits[k] is
+// set when (i0 + r, s0 + lane + 64k) is selected.  Rows in groups of R with
+// the loads of a group issued first; beta, alpha and the rows' tie cuts are
+// read once.  T is formed exactly as cg_T forms it.
+template <int C, bool FULL>
+CG_DEV void cg_sel_bits(const CgInstD& I, int rb, int st, const double* tab, unsigned int (&bits)[C])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s0 = st * I.sw, mm = I.m - 1, m = I.m, n = I.n;
+    const unsigned long long kth = cg_kth(I);
+    const double ir = I.inv_reg;
+    double bj[C];
+    bool ok[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const int j = s0 + lane + 64 * k;
+        ok[k] = FULL || j < mm;
+        bj[k] = ok[k] ? I.gamma[n + j] : 0.0;
+        bits[k] = 0u;
+    }
+    const int i0 = rb * CG_RB + w * CG_RPW;
+    constexpr int R = 8 / C;
+    static_assert(CG_RPW <= 32, "one bit per row of the wave");
+#pragma unroll 1
+    for (int r0 = 0; r0 < CG_RPW; r0 += R) {
+        if (i0 + r0 >= n) break;
+        double mv[R][C], ai[R];
+        int cut[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not taken
+            ai[r] = I.gamma[i];
+            cut[r] = I.rcut[i];
+            const auto Mi = I.M + (size_t)i * m + s0 + lane;
+#pragma unroll
+            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const bool live = i0 + r0 + r < n;
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                const unsigned long long key = cg_key(cg_texp((ai[r] + bj[k] - mv[r][k]) * ir, tab));
+                if (live && ok[k] && cg_selected(key, kth, s0 + lane + 64 * k, cut[r])) bits[k] |= 1u << (r0 + r);
+            }
+        }
+    }
+}
+
 // Tile pass: selected count per (row block, column).
 __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restrict__ insts,
                                                           const int* __restrict__ tile_inst,
@@ -291,25 +340,16 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
     const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw, mm = I.m - 1;
-    const unsigned long long kth = cg_kth(I);
-    int cnt[CG_CMAX];
-#pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) cnt[k] = 0;
-    const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= I.n) break;
-        const int cut = I.rcut[i];
-#pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < mm) cnt[k] += cg_selected(cg_key(cg_T(I, i, j, tab)), kth, j, cut);
-        }
-    }
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
+    cg_strip_dispatch(I, st, mm, [&](auto tag) {
+        using S = decltype(tag);
+        unsigned int bits[S::C];
+        cg_sel_bits<S::C, S::FULL>(I, rb, st, tab, bits);
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = cnt[k];
+        for (int k = 0; k < S::C; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
+    });
     __syncthreads();
+    // columns j < m-1 of the strip lie below 64*C of the strip's C
     for (int q = threadIdx.x; q < I.sw; q += CG_WG) {
         const int j = s0 + q;
         if (j < mm) I.ccnt[(size_t)rb * I.m + j] = lds[0][q] + lds[1][q] + lds[2][q] + lds[3][q];
@@ -383,36 +423,27 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
     const CgInstD& I = cg_inst(insts, b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw, mm = I.m - 1;
-    const unsigned long long kth = cg_kth(I);
-    unsigned int bits[CG_CMAX];
-#pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) bits[k] = 0u;
     const int i0 = rb * CG_RB + w * CG_RPW;
-    for (int r = 0; r < CG_RPW; r++) {
-        const int i = i0 + r;
-        if (i >= I.n) break;
-        const int cut = I.rcut[i];
-#pragma unroll
-        for (int k = 0; k < CG_CMAX; k++) {
-            const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < mm && cg_selected(cg_key(cg_T(I, i, j, tab)), kth, j, cut)) bits[k] |= (1u << r);
-        }
-    }
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
+    cg_strip_dispatch(I, st, mm, [&](auto tag) {
+        using S = decltype(tag);
+        unsigned int bits[S::C];
+        cg_sel_bits<S::C, S::FULL>(I, rb, st, tab, bits);
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
-    __syncthreads();
+        for (int k = 0; k < S::C; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < CG_CMAX; k++) {
-        const int q = lane + 64 * k;
-        const int j = s0 + q;
-        if (k < I.c && j < mm && bits[k]) {
-            int pos = I.w_colptr[j] + I.ccnt[(size_t)rb * I.m + j];
-            for (int ww = 0; ww < w; ww++) pos += lds[ww][q];
-            for (int r = 0; r < CG_RPW; r++)
-                if (bits[k] & (1u << r)) I.w_rowi[pos++] = i0 + r;
+        for (int k = 0; k < S::C; k++) {
+            const int q = lane + 64 * k;
+            const int j = s0 + q;
+            if (j < mm && bits[k]) {
+                int pos = I.w_colptr[j] + I.ccnt[(size_t)rb * I.m + j];
+                for (int ww = 0; ww < w; ww++) pos += lds[ww][q];
+                for (int r = 0; r < CG_RPW; r++)
+                    if (bits[k] & (1u << r)) I.w_rowi[pos++] = i0 + r;
+            }
         }
-    }
+    });
 }
 
 // Refresh W values on the current pattern and the diagonal blocks
