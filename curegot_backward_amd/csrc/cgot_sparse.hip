@@ -155,6 +155,32 @@ __global__ __launch_bounds__(CG_WG) void k_hist_scan(const CgInst* __restrict__ 
     for (int q = threadIdx.x; q < nb; q += CG_WG) I.hist[q] = 0u;
 }
 
+// Keys of U consecutive 64-column chunks of row i from column j0 (lane l
+// holds column j0 + l + 64u), with every load issued before the first
+// exponential.  Lanes past mm read column 0 of the row and report ok = false.
+// T is formed exactly as cg_T forms it.
+constexpr int CG_ROW_CHUNKS = 4;
+
+template <int U, bool FULL>
+CG_DEV void cg_row_keys(const CgInstD& I, int i, int j0, int mm, double ai, const double* tab,
+                        unsigned long long (&key)[U], bool (&ok)[U])
+{
+    const int lane = threadIdx.x & 63;
+    const auto Mi = I.M + (size_t)i * I.m;
+    const auto gb = I.gamma + I.n;
+    double mv[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = j0 + lane + 64 * u;
+        ok[u] = FULL || j < mm;
+        const int jc = ok[u] ? j : 0;
+        mv[u] = Mi[jc];
+        bv[u] = gb[jc];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) key[u] = cg_key(cg_texp((ai + bv[u] - mv[u]) * I.inv_reg, tab));
+}
+
 // Row-wave kernels: one wave per row; blockIdx.x -> (instance, 4 rows) via row_inst.
 // Count keys above the threshold and ties, per row.
 __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restrict__ insts,
@@ -171,12 +197,22 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_count(const CgInst* __restri
     if (i >= I.n) return;
     const unsigned long long kth = cg_kth(I);
     const int mm = I.m - 1;
+    const double ai = I.gamma[i];
     int gt = 0, tie = 0;
-    for (int j = lane; j < mm; j += 64) {
-        const unsigned long long key = cg_key(cg_T(I, i, j, tab));
-        gt += (key > kth);
-        tie += (key == kth);
-    }
+    auto tally = [&](int j0, auto tag) {
+        constexpr int U = decltype(tag)::C;
+        unsigned long long key[U];
+        bool ok[U];
+        cg_row_keys<U, decltype(tag)::FULL>(I, i, j0, mm, ai, tab, key, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            gt += (ok[u] && key[u] > kth);
+            tie += (ok[u] && key[u] == kth);
+        }
+    };
+    int j0 = 0;
+    for (; j0 + 64 * CG_ROW_CHUNKS <= mm; j0 += 64 * CG_ROW_CHUNKS) tally(j0, CgStrip<CG_ROW_CHUNKS, true>{});
+    for (; j0 < mm; j0 += 64) tally(j0, CgStrip<1, false>{});
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) { gt += __shfl_xor(gt, o, 64); tie += __shfl_xor(tie, o, 64); }
     if (lane == 0) { I.rcnt[i] = gt; I.rtie[i] = tie; }
@@ -252,24 +288,34 @@ __global__ __launch_bounds__(CG_WG) void k_sel_rows_write(const CgInst* __restri
     const int off = I.w_rowptr[i];
     const int take = I.rtie[i];
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const double ai = I.gamma[i];
     int tseen = 0, wpos = 0, cut = 0;
-    for (int j0 = 0; j0 < mm; j0 += 64) {
-        const int j = j0 + lane;
-        const bool ok = j < mm;
-        const unsigned long long key = ok ? cg_key(cg_T(I, i, j, tab)) : 0ull;
-        const bool isgt = ok && key > kth;
-        const bool istie = ok && key == kth;
-        const unsigned long long tmask = __ballot(istie);
-        const int trank = tseen + __popcll(tmask & lt);
-        const bool tsel = istie && trank < take;
-        const bool sel = isgt || tsel;
-        const unsigned long long smask = __ballot(sel);
-        const int pos = wpos + __popcll(smask & lt);
-        if (sel) I.w_colj[off + pos] = j;
-        if (tsel) cut = max(cut, j + 1);
-        tseen += __popcll(tmask);
-        wpos += __popcll(smask);
-    }
+    // chunks in column order: ties are taken, and entries placed, left to right
+    auto place = [&](int j0, auto tag) {
+        constexpr int U = decltype(tag)::C;
+        unsigned long long key[U];
+        bool ok[U];
+        cg_row_keys<U, decltype(tag)::FULL>(I, i, j0, mm, ai, tab, key, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = j0 + lane + 64 * u;
+            const bool isgt = ok[u] && key[u] > kth;
+            const bool istie = ok[u] && key[u] == kth;
+            const unsigned long long tmask = __ballot(istie);
+            const int trank = tseen + __popcll(tmask & lt);
+            const bool tsel = istie && trank < take;
+            const bool sel = isgt || tsel;
+            const unsigned long long smask = __ballot(sel);
+            const int pos = wpos + __popcll(smask & lt);
+            if (sel) I.w_colj[off + pos] = j;
+            if (tsel) cut = max(cut, j + 1);
+            tseen += __popcll(tmask);
+            wpos += __popcll(smask);
+        }
+    };
+    int j0 = 0;
+    for (; j0 + 64 * CG_ROW_CHUNKS <= mm; j0 += 64 * CG_ROW_CHUNKS) place(j0, CgStrip<CG_ROW_CHUNKS, true>{});
+    for (; j0 < mm; j0 += 64) place(j0, CgStrip<1, false>{});
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cut = max(cut, __shfl_xor(cut, o, 64));
     if (lane == 0) I.rcut[i] = cut;
