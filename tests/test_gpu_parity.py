@@ -108,6 +108,41 @@ def test_sparsify_hook_matches_dense(cg):
     np.testing.assert_allclose(r["sol"].cpu().numpy(), np.linalg.solve(Hs, rhs), rtol=1e-10)
 
 
+@pytest.mark.parametrize("n,m", [(37, 700), (29, 513), (300, 40)])
+def test_sparsify_ties_row_major(cg, n, m):
+    # M takes four values, so T has four classes of exactly equal entries; K
+    # ends inside the second class, and the selected ties must be its first
+    # ones in row-major order of T[:, :m-1] (the rule of
+    # src/sinkhorn_splr_kernel.cu:597-906 made deterministic).  m-1 = 699
+    # (full 256-column chunks and a tail), 512 (full chunks only) and 39 (one
+    # partial chunk) cover the row passes' chunk shapes.
+    from scipy.sparse import csr_matrix
+    print(f"ties n={n} m={m}", flush=True)
+    rng = np.random.default_rng(n * 1000 + m)
+    M = rng.integers(1, 5, size=(n, m)) * 0.1
+    reg, shift = 0.1, 1e-3
+    alpha, beta = np.zeros(n), np.zeros(m)
+    a = np.full(n, 1 / n); b = np.full(m, 1 / m)
+    c1 = int((M[:, :-1] == 0.1).sum()); c2 = int((M[:, :-1] == 0.2).sum())
+    K = c1 + c2 // 2 + 1
+    rhs = rng.normal(size=n + m - 1)
+    r = cg.tests.test_T_computation_sparsify(alpha, beta, torch.tensor(M), a, b, reg, shift, K, rhs=rhs)
+    T = np.exp(-M / reg)
+    # brute force: stable order by decreasing T over the row-major flattening
+    flat = T[:, :-1].ravel()
+    sel = np.lexsort((np.arange(flat.size), -flat))[:K]
+    assert np.all(flat[sel[-1]] == flat[sel[c1:]])   # the cut lies inside one tie class
+    Tsp = np.zeros_like(flat); Tsp[sel] = flat[sel]; Tsp = Tsp.reshape(n, m - 1)
+    H = np.diag(np.concatenate((T.sum(1), T.sum(0)[:-1])) + shift)
+    H[n:, :n] = Tsp.T
+    Hc = csr_matrix(H)
+    np.testing.assert_array_equal(r["csr_rowptr"].cpu().numpy(), Hc.indptr)
+    np.testing.assert_array_equal(r["csr_colind"].cpu().numpy(), Hc.indices)
+    np.testing.assert_allclose(r["csr_val"].cpu().numpy(), Hc.data, rtol=1e-13)
+    Hs = _dense.csr_dense_sym(Hc.data, Hc.indices, Hc.indptr, n + m - 1)
+    assert rel(r["sol"], np.linalg.solve(Hs, rhs)) < 1e-10
+
+
 @pytest.mark.parametrize("n,m,K", [(200, 150, 600), (150, 200, 600), (300, 300, 3000), (70, 130, 1)])
 def test_newton_solve_vs_dense(cg, n, m, K):
     # multi-block dense Cholesky (P > 64) on the Schur complement vs numpy.linalg.solve
