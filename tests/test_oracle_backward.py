@@ -86,3 +86,36 @@ def test_product_plan_closed_form():
     Q[:n, n:] = np.outer(a, b[:-1]); Q[n:, :n] = Q[:n, n:].T
     mu = np.concatenate([(G * np.outer(a, b)).sum(1), (G * np.outer(a, b)).sum(0)[:-1]])
     np.testing.assert_allclose(res["lambda"], np.linalg.solve(Q, mu), rtol=1e-9)
+
+
+def test_embedding_chain_rule_finite_differences():
+    # The embedding workload of the reference's examples/differentiable_splr_ot.py:
+    # M = |X_i - Y_j|^2 / d, loss = -trace(T).  dL/dX_i = (2/d) sum_j dM_ij (X_i - Y_j)
+    # with dM the implicit backward for G = -I, against central differences of
+    # the loss through the oracle's solve (plans polished by BCD sweeps).
+    n = m = 48
+    d, eta = 8, 0.05
+    rng = np.random.default_rng(11)
+    Y, X = rng.random((m, d)), rng.random((n, d))
+    a, b = np.full(n, 1.0 / n), np.full(m, 1.0 / m)
+
+    def cost(Xv):
+        return ((Xv[:, None, :] - Y[None, :, :]) ** 2).sum(-1) / d
+
+    def loss(Xv):
+        Mv = cost(Xv)
+        fw = oracle.splr_bwd(Mv, a, b, eta, tol=1e-8, max_iter=1000, density=0.1)
+        pol = oracle.sinkhorn_bcd(Mv, a, b, eta, tol=1e-15, max_iter=3000, x0=fw["dual"])
+        return -np.trace(pol["plan"])
+
+    M = cost(X)
+    fw = oracle.splr_bwd(M, a, b, eta, tol=1e-8, max_iter=1000, density=0.1)
+    dM = oracle.backward(M, fw["dual"], -np.eye(n), eta, ctx=fw, preconditioner="diag")["grad"]
+    dX = (2.0 / d) * (dM.sum(axis=1)[:, None] * X - dM @ Y)
+    h = 1e-5
+    for i, k in [(0, 0), (5, 3), (20, 1)]:
+        Xp, Xm = X.copy(), X.copy()
+        Xp[i, k] += h
+        Xm[i, k] -= h
+        fd = (loss(Xp) - loss(Xm)) / (2 * h)
+        assert abs(fd - dX[i, k]) < 1e-6 * np.abs(dX).max(), (i, k, fd, dX[i, k])
