@@ -116,3 +116,33 @@ def test_batched_adjoint_equals_single_uneven_iterations(cg):
         assert r["iterations"] == s["iterations"]
         assert torch.equal(r["grad"], s["grad"])
         assert torch.equal(r["lambda"], s["lambda"])
+
+
+def test_full_size_ragged_batch_properties(cg):
+    # The benchmarked workload at its full size: 128 instances, n, m log-uniform
+    # in [128, 4096], one batched call.  Size-independent checks per instance:
+    # convergence, the stop rule recomputed from the returned plan
+    # (||[T1 - a; (T^T 1 - b)[:-1]]|| < tol, sinkhorn_splr.cu:1375-1382), the
+    # last column's marginal through total mass, and the plan equal to
+    # exp((alpha_i + beta_j - M_ij)/eta) of the returned dual.
+    from curegot_backward_amd.workloads import ragged_batch
+    probs = ragged_batch(128)
+    Ms = [p[0].cuda() for p in probs]
+    As = [p[1].cuda() for p in probs]
+    Bs = [p[2].cuda() for p in probs]
+    etas = [p[3] for p in probs]
+    tol = 1e-8
+    outs = cg.sinkhorn_splr_bwd_batch(Ms, As, Bs, etas, tol, 1000)
+    print("full batch: niter max", max(o["niter"] for o in outs), flush=True)
+    for k, (M, a, b, eta, o) in enumerate(zip(Ms, As, Bs, etas, outs)):
+        n, m = M.shape
+        assert o["converged"], k
+        T, dual = o["plan"], o["dual"]
+        assert torch.isfinite(T).all() and (T >= 0).all(), k
+        g = torch.cat([T.sum(1) - a, (T.sum(0) - b)[:-1]])
+        # the recomputed sums differ from the solver's own by summation rounding
+        assert float(g.norm()) < tol * (1 + 1e-3) + 1e-14 * (n + m), (k, float(g.norm()))
+        # |sum_i (T1 - a)_i - sum_{j<m} (T^T 1 - b)_j| <= sqrt(n + m) ||g|| (Cauchy-Schwarz)
+        assert abs(float(T.sum(0)[-1] - b[-1])) < tol * (n + m) ** 0.5 + 1e-13, k
+        Tx = torch.exp((dual[:n, None] + dual[None, n:] - M) / eta)
+        assert float((T - Tx).abs().max() / Tx.abs().max()) < 1e-12, k
