@@ -178,6 +178,32 @@ def test_lse_update_vs_oracle(cg, n, m, which):
         assert rel(d[n:], ref) < 1e-12
 
 
+@pytest.mark.parametrize("which", [0, 1])
+def test_lse_update_fixed_point_full_size(cg, which):
+    # The isolated LSE workload at its full size (n = m = 16384, 2 GiB of M):
+    # after an alpha update the plan's row sums are a, after a beta update its
+    # column sums are b (the fixed points of sinkhorn_bcd_kernel.cu:35-192),
+    # checked against the plan formed in torch from the updated dual.
+    from curegot_backward_amd import _native
+    n = m = 16384
+    eta = 0.05
+    g = torch.Generator(device="cpu").manual_seed(5)
+    X = torch.randn(n, 16, generator=g, dtype=torch.float64).cuda()
+    Y = torch.randn(m, 16, generator=g, dtype=torch.float64).cuda()
+    M = torch.cdist(X, Y) ** 2
+    M /= M.max()
+    a = torch.rand(n, generator=g, dtype=torch.float64).cuda() + 0.5; a /= a.sum()
+    b = torch.rand(m, generator=g, dtype=torch.float64).cuda() + 0.5; b /= b.sum()
+    d = (0.05 * torch.randn(n + m, generator=g, dtype=torch.float64)).cuda()
+    _native.lse_update(M, a, b, d, eta, which)
+    T = torch.exp((d[:n, None] + d[None, n:] - M) / eta)
+    s, t = (T.sum(1), a) if which == 0 else (T.sum(0), b)
+    del T
+    err = float(((s - t) / t).abs().max())
+    print(f"lse fixed point n=m={n} which={which}: max rel {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
 @pytest.mark.parametrize("n,m,eta", [(80, 60, 0.01), (60, 80, 0.05)])
 def test_sinkhorn_bcd_vs_oracle(cg, n, m, eta):
     from curegot_backward_amd.workloads import make_problem
