@@ -204,6 +204,26 @@ def test_lse_update_fixed_point_full_size(cg, which):
     assert err < 1e-12
 
 
+def test_adjoint_full_size_invariants(cg):
+    # The adjoint workload at its full size (n = m = 4096, loss sum(T^2) + <T, R>)
+    # through the autograd path: the plan does not change under
+    # M -> M + u 1^T + 1 v^T, so dL/dM has zero row and column sums.
+    from curegot_backward_amd.workloads import adjoint_problem
+    M, a, b, eta, R = adjoint_problem()
+    Mg = M.cuda().requires_grad_(True)
+    a, b, R = a.cuda(), b.cuda(), R.cuda()
+    T = cg.torch.splr_ot(Mg, a, b, eta=eta, tol=1e-8)
+    ((T ** 2).sum() + (T * R).sum()).backward()
+    g = Mg.grad
+    assert torch.isfinite(g).all()
+    n, m = g.shape
+    scale = float(g.abs().max())
+    rs, cs = float(g.sum(1).abs().max()), float(g.sum(0).abs().max())
+    print(f"adjoint n=m={n}: max|dM| {scale:.2e}, max row sum {rs:.2e}, max column sum {cs:.2e}", flush=True)
+    assert rs < 1e-9 * scale * m
+    assert cs < 1e-9 * scale * n
+
+
 @pytest.mark.parametrize("n,m,eta", [(80, 60, 0.01), (60, 80, 0.05)])
 def test_sinkhorn_bcd_vs_oracle(cg, n, m, eta):
     from curegot_backward_amd.workloads import make_problem
