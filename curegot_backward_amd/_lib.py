@@ -88,7 +88,7 @@ EXPORTS = (
     "cgot_lse_workspace_size", "cgot_lse_update",
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
     "cgot_profile_enable", "cgot_profile_reset", "cgot_profile_read",
-    "cgot_last_error", "cgot_version",
+    "cgot_shutdown", "cgot_last_error", "cgot_version",
 )
 
 _lib = None
@@ -144,6 +144,8 @@ def load():
     L.cgot_profile_reset.restype = None
     L.cgot_profile_read.argtypes = [i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(i64)]
     L.cgot_profile_read.restype = ctypes.c_int
+    L.cgot_shutdown.argtypes = []
+    L.cgot_shutdown.restype = ctypes.c_int
     L.cgot_last_error.restype = ctypes.c_char_p
     L.cgot_version.restype = ctypes.c_char_p
     _lib = L

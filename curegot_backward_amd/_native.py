@@ -57,9 +57,14 @@ class SPLRBackwardContext:
     is not stored: the adjoint recomputes T from M and the duals.
     """
 
-    def __init__(self, M, dual, reg, info: _lib.SplrInfo, workspace: torch.Tensor):
-        self._M = M
+    def __init__(self, M, dual, reg, info: _lib.SplrInfo, workspace: torch.Tensor, snapshot: bool = False):
+        # The adjoint recomputes T* from M and the duals, so both must stay as
+        # they were at the solve: "snapshot" keeps a private copy of M, and an
+        # in-place change of the tensors the context holds is refused by
+        # check_unmodified() (the reference keeps the plan T itself instead).
+        self._M = M.clone() if snapshot else M
         self._dual = dual
+        self._versions = (self._M._version, dual._version)
         self._ws = workspace
         self._reg = float(reg)
         self._n = int(info.n)
@@ -112,6 +117,15 @@ class SPLRBackwardContext:
     @property
     def M(self) -> torch.Tensor:
         return self._M
+
+    def check_unmodified(self):
+        """Raise if the cost matrix or the duals this context recomputes the
+        plan from were modified in place after the forward solve."""
+        if (self._M._version, self._dual._version) != self._versions:
+            raise RuntimeError(
+                "SPLRBackwardContext: the cost matrix M or the dual returned by the forward solve was "
+                "modified in place after the solve; the backward recomputes the plan from them "
+                "(solve again, or use context_mode='snapshot' to keep a private copy of M)")
 
     @property
     def dual(self) -> torch.Tensor:
@@ -386,7 +400,8 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
             raise RuntimeError(
                 "torch_sinkhorn_splr_bwd: forward pass failed (non-finite objective value); "
                 f"no backward context available (instance {i}, shape {info.n}x{info.m})")
-        ctx = SPLRBackwardContext(keep[i][0], dual, regs[i], info, ws) if info.valid else None
+        ctx = (SPLRBackwardContext(keep[i][0], dual, regs[i], info, ws, snapshot=(ctx_mode == "snapshot"))
+               if info.valid else None)
         src = Ms[i]
         res = {
             "niter": int(info.niter),
@@ -420,7 +435,11 @@ def sinkhorn_splr_bwd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs
 
 def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """Forward-only SPLR (reference torch_sinkhorn_splr, src/solvers_torch.cpp:130-288):
-    {niter, plan, dual}."""
+    {niter, plan, dual}.  Like the reference, a non-finite objective is not an
+    error here: the iterates are returned as they are (only the
+    backward-capable entry needs a usable context)."""
+    kwargs = dict(kwargs)
+    kwargs["allow_invalid"] = True
     out = sinkhorn_splr_bwd(M, a, b, reg, tol, max_iter, verbose, **kwargs)
     return {"niter": out["niter"], "plan": out["plan"], "dual": out["dual"]}
 

@@ -64,6 +64,7 @@ def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=20
     for G, ctx, eta in items:
         if G.shape != ctx.M.shape:
             raise ValueError("G must have the same shape as T")
+        ctx.check_unmodified()
         inv, factor = select_preconditioner(ctx, preconditioner)
         jobs.append((ctx.M, ctx.dual, G, float(eta), inv, factor))
     lams, grads, infos = _native.adjoint_batch(jobs, tol, max_iter)

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -48,20 +49,44 @@ thread_local std::string g_last_error;
     } while (0)
 
 // ---------------------------------------------------------------------------
-// Pinned host staging (per thread, grown on demand, kept for the process).
+// Process-lifetime resources: pinned staging buffers, the size-class streams
+// and the worker threads.  They are registered here and released together by
+// cgot_shutdown (called explicitly, or at exit while no solve is running);
+// the generation counter tells per-thread caches that their entries are gone.
 // ---------------------------------------------------------------------------
+std::mutex g_res_mu;
+std::vector<void*> g_res_pinned;
+std::vector<hipStream_t> g_res_streams;
+std::atomic<int> g_res_gen{0};
+std::atomic<int> g_inflight{0};   // solves running inside the library
+void register_atexit();
+
+struct InFlight {
+    InFlight() { g_inflight.fetch_add(1); }
+    ~InFlight() { g_inflight.fetch_sub(1); }
+};
+
+// Pinned host staging (per thread, grown on demand).
 struct Pinned {
     double* p = nullptr;
     size_t cap = 0;
+    int gen = -1;
     double* get(size_t n)
     {
+        if (gen != g_res_gen.load()) { p = nullptr; cap = 0; gen = g_res_gen.load(); }
         if (n > cap) {
-            // the old buffer may still be read by an in-flight copy: keep it.
-            // Grow geometrically so that a thread reaches its working size in
-            // a few allocations (pinning host memory is slow).
+            // the old buffer may still be read by an in-flight copy: keep it
+            // (registered, freed at shutdown).  Grow geometrically so that a
+            // thread reaches its working size in a few allocations (pinning
+            // host memory is slow).
             const size_t c = std::max<size_t>({n, 2 * cap, 65536});
             double* q = nullptr;
             CG_HIP(hipHostMalloc((void**)&q, c * sizeof(double), hipHostMallocDefault));
+            {
+                std::lock_guard<std::mutex> lk(g_res_mu);
+                g_res_pinned.push_back(q);
+            }
+            register_atexit();
             p = q;
             cap = c;
         }
@@ -1381,7 +1406,9 @@ void bcd_run(Batch& bt, const cgot_bcd_problem* probs, int32_t* niter)
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-#define CG_API_BEGIN try {
+#define CG_API_BEGIN \
+    InFlight cg_inflight_; \
+    try {
 #define CG_API_END                         \
     }                                      \
     catch (const std::invalid_argument& e) \
@@ -1453,15 +1480,16 @@ GroupPlan plan_splr_groups(int B, const cgot_splr_problem* probs)
     return g;
 }
 
-// A pool of persistent host threads (their pinned staging buffers and streams
-// live as long as the process).  Callers may run concurrently: each call
-// queues its jobs and waits for its own completion count; the pool grows so
-// that every queued job has a thread, up to kMaxWorkers.
+// A pool of persistent host threads.  Callers may run concurrently: each
+// call queues its jobs and waits for its own completion count; the pool grows
+// so that every queued job has a thread, up to kMaxWorkers.  stop() (from
+// cgot_shutdown) lets the idle workers finish and joins them; a later call
+// starts new ones.
 class GroupPool {
 public:
     static GroupPool& get()
     {
-        static GroupPool* p = new GroupPool();   // never destroyed: workers are detached
+        static GroupPool* p = new GroupPool();   // outlives every caller; workers joined by stop()
         return *p;
     }
     void run(std::vector<std::function<void()>>& jobs)
@@ -1469,14 +1497,28 @@ public:
         Call call;
         call.remaining = (int)jobs.size();
         std::unique_lock<std::mutex> lk(mu_);
+        stop_ = false;
         for (auto& j : jobs) q_.push_back({&j, &call});
-        while (idle_ < (int)q_.size() && workers_ < kMaxWorkers) {
-            std::thread(&GroupPool::loop, this).detach();
-            workers_++;
+        while (idle_ < (int)q_.size() && (int)threads_.size() < kMaxWorkers) {
+            threads_.emplace_back(&GroupPool::loop, this);
             idle_++;
         }
+        register_atexit();
         cv_.notify_all();
         call.cv.wait(lk, [&] { return call.remaining == 0; });
+    }
+    void stop()
+    {
+        std::vector<std::thread> ts;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            ts.swap(threads_);
+            cv_.notify_all();
+        }
+        for (auto& t : ts) t.join();
+        std::lock_guard<std::mutex> lk(mu_);
+        idle_ = 0;
     }
 
 private:
@@ -1487,7 +1529,8 @@ private:
     {
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
-            cv_.wait(lk, [&] { return !q_.empty(); });
+            cv_.wait(lk, [&] { return !q_.empty() || stop_; });
+            if (q_.empty()) return;       // stopped and nothing queued
             Item it = q_.front();
             q_.pop_front();
             idle_--;
@@ -1501,7 +1544,9 @@ private:
     std::mutex mu_;
     std::condition_variable cv_;
     std::deque<Item> q_;
-    int workers_ = 0, idle_ = 0;
+    std::vector<std::thread> threads_;
+    int idle_ = 0;
+    bool stop_ = false;
 };
 
 // CUs kept free of size classes 1.. when a batch is split: the largest
@@ -1531,6 +1576,8 @@ int cu_reserve(int ncu)
 hipStream_t group_stream(int device, int k)
 {
     thread_local std::map<int, std::array<hipStream_t, kMaxGroups>> streams;
+    thread_local int gen = -1;
+    if (gen != g_res_gen.load()) { streams.clear(); gen = g_res_gen.load(); }
     auto it = streams.find(device);
     if (it == streams.end()) {
         std::array<hipStream_t, kMaxGroups> a{};
@@ -1546,6 +1593,12 @@ hipStream_t group_stream(int device, int k)
         } else {
             for (int q = 1; q < kMaxGroups; q++) CG_HIP(hipStreamCreateWithFlags(&a[q], hipStreamNonBlocking));
         }
+        {
+            std::lock_guard<std::mutex> lk(g_res_mu);
+            for (hipStream_t x : a)
+                if (x) g_res_streams.push_back(x);
+        }
+        register_atexit();
         it = streams.emplace(device, a).first;
     }
     return it->second[k];
@@ -1568,11 +1621,55 @@ void splr_solve_group(const cgot_splr_problem* probs, int B, void* ws, hipStream
     s.run(probs, info);
 }
 
+// Release the registered resources: join the worker threads, then wait for
+// and destroy the size-class streams and free the pinned staging buffers.
+void shutdown_resources()
+{
+    GroupPool::get().stop();
+    std::vector<hipStream_t> streams;
+    std::vector<void*> pinned;
+    {
+        std::lock_guard<std::mutex> lk(g_res_mu);
+        streams.swap(g_res_streams);
+        pinned.swap(g_res_pinned);
+        g_res_gen.fetch_add(1);
+    }
+    for (hipStream_t s : streams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (void* p : pinned) (void)hipHostFree(p);
+}
+
+// At process exit, release everything before the HIP runtime tears itself
+// down (its exit handlers were registered when it loaded, before this one, so
+// they run after it).  Skipped while a solve is still inside the library.
+void atexit_shutdown()
+{
+    if (g_inflight.load() == 0) shutdown_resources();
+}
+
+void register_atexit()
+{
+    static std::once_flag once;
+    std::call_once(once, [] { std::atexit(atexit_shutdown); });
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* cgot_last_error(void) { return g_last_error.c_str(); }
+
+int cgot_shutdown(void)
+{
+    if (g_inflight.load() != 0) {
+        g_last_error = "cgot_shutdown: a solve is still running";
+        return -1;
+    }
+    shutdown_resources();
+    return 0;
+}
 
 void cgot_profile_enable(int32_t on)
 {
@@ -1654,6 +1751,9 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
                 CG_HIP(hipEventRecord(done[k], st));
             } catch (const std::exception& e) {
                 err[k] = e.what();
+                // the caller frees the workspace and outputs when the error
+                // reaches it: let this class's queued kernels finish first
+                (void)hipStreamSynchronize(gst[k]);
             }
         });
     }
@@ -1667,6 +1767,7 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
         }
     } catch (const std::exception& e) {
         err[0] = e.what();
+        (void)hipStreamSynchronize(st0);
     }
     host.join();
     for (int k = 0; k < G; k++) {

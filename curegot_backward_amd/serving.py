@@ -67,6 +67,7 @@ class _Request:
     footprint: int
     t_submit: float
     ready: torch.cuda.Event          # recorded on the submitter's stream
+    stream: torch.cuda.Stream        # the submitter's stream, which will use the results
     future: Future = field(default_factory=Future)
 
 
@@ -131,9 +132,10 @@ class OTServer:
         if not eta > 0:
             raise ValueError("eta must be positive")
         fp = splr_footprint(n, m, backward=G is not None)
+        sub = torch.cuda.current_stream(self.device)
         ready = torch.cuda.Event()
-        ready.record(torch.cuda.current_stream(self.device))   # the inputs are complete after this point
-        req = _Request(M, a, b, float(eta), G, fp, time.perf_counter(), ready)
+        ready.record(sub)   # the inputs are complete after this point
+        req = _Request(M, a, b, float(eta), G, fp, time.perf_counter(), ready, sub)
         with self._cv:
             if self._closed:
                 raise RuntimeError("server is closed")
@@ -273,6 +275,14 @@ class OTServer:
                                        bwd_iterations=bres[i]["iterations"], bwd_converged=bres[i]["converged"])
                         else:
                             out.update(grad=None, bwd_converged=False)
+                    # the outputs were allocated on this slot's stream: once the
+                    # caller frees them, the caching allocator must not hand
+                    # their blocks to this stream's next batch before the
+                    # caller's stream has finished reading them
+                    for key in ("plan", "dual", "grad", "lambda"):
+                        t = out.get(key)
+                        if torch.is_tensor(t):
+                            t.record_stream(r.stream)
                     r.future.set_result(out)
                 with self._cv:
                     self._batches += 1

@@ -214,6 +214,15 @@ void cgot_profile_enable(int32_t on);
 void cgot_profile_reset(void);
 int cgot_profile_read(int32_t cls, double* ms_total, double* work_total, int64_t* launches);
 
+/* Release the library's process-lifetime resources: join its worker threads,
+ * wait for and destroy its internal size-class streams and free its pinned
+ * host staging buffers.  Call only while no solve is running (returns -1
+ * otherwise); a later solve sets them up again.  It also runs by itself at
+ * process exit, before the HIP runtime's own teardown, when no solve is
+ * running.  No counterpart in the reference, whose solver objects free their
+ * state per call (src/sinkhorn_splr.cu, SPLRSolver destructor). */
+int cgot_shutdown(void);
+
 const char* cgot_last_error(void);
 const char* cgot_version(void);
 

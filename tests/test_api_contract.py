@@ -18,6 +18,29 @@ def test_api_surface():
     assert callable(cg.tests.test_T_computation_sparsify)
 
 
+def test_drop_in_import_name():
+    """`import curegot` as the reference's release check does
+    (validate_stage6a_release.py:41-45; curegot/__init__.py:23-39): the public,
+    advanced and legacy names on curegot.torch, the native namespace's name,
+    the numpy/tests submodules and the top-level API, all the same objects as
+    this package's."""
+    import importlib
+    import curegot
+    import curegot.autograd
+    import curegot.backward
+    for name in ("splr_ot", "splr_ot_with_info", "SPLROT", "splr_ot_batch", "solve_backward",
+                 "sinkhorn_splr_bwd", "SPLRBackwardContext", "sinkhorn_splr", "sinkhorn_bcd"):
+        assert callable(getattr(curegot.torch, name, None)), name
+        assert getattr(curegot.torch, name) is getattr(cg.torch, name)
+    assert curegot.torch.__name__ == "curegot._internal_torch.torch"
+    assert importlib.import_module("curegot._internal_torch").torch is curegot.torch
+    assert curegot.numpy.sinkhorn_bcd is cg.numpy.sinkhorn_bcd
+    assert callable(curegot.tests.test_T_computation_sparsify)
+    assert curegot.splr_ot is curegot.torch.splr_ot is curegot.autograd.splr_ot
+    assert curegot.solve_backward is curegot.backward.solve_backward
+    assert curegot.__version__ == cg.__version__
+
+
 def test_cpu_input_rejected():
     M, a, b = make_problem(5, 4, 0)
     with pytest.raises(RuntimeError, match="CUDA"):
