@@ -71,6 +71,15 @@ class BcdProblem(ctypes.Structure):
     ]
 
 
+class SpHdr(ctypes.Structure):
+    """Header of a sparse Newton-system plan (csrc/cgot_symbolic.h)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ("mode", "N", "n", "m", "t", "R", "H", "nnzP", "ncore", "maxlev")] + \
+               [(k, ctypes.c_int64) for k in ("npairs", "ncpairs", "o_perm", "o_lvptr", "o_lcptr", "o_lcols",
+                                              "o_emeta", "o_erow", "o_ecol", "o_pairs",
+                                              "o_cmeta", "o_cpairs", "o_rptr", "o_rq", "o_lval",
+                                              "o_lcore", "o_yx", "bytes")]
+
+
 class PrecondProfile(ctypes.Structure):
     """cgot_precond_profile (microseconds)."""
     _fields_ = [(name, ctypes.c_double) for name in (
@@ -80,7 +89,7 @@ class PrecondProfile(ctypes.Structure):
 
 
 EXPORTS = (
-    "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups",
+    "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups", "cgot_set_sparse_solve",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
     "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply", "cgot_splr_precond_apply_profiled",
     "cgot_splr_context_csr",
@@ -88,6 +97,7 @@ EXPORTS = (
     "cgot_lse_workspace_size", "cgot_lse_update",
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
     "cgot_profile_enable", "cgot_profile_reset", "cgot_profile_read",
+    "cgot_sparse_plan",
     "cgot_shutdown", "cgot_last_error", "cgot_version",
 )
 
@@ -109,6 +119,8 @@ def load():
     L.cgot_splr_workspace_size.restype = sz
     L.cgot_splr_solve.argtypes = [i32, ctypes.POINTER(SplrProblem), vp, sz, vp, ctypes.POINTER(SplrInfo)]
     L.cgot_splr_solve.restype = ctypes.c_int
+    L.cgot_set_sparse_solve.argtypes = [i32]
+    L.cgot_set_sparse_solve.restype = ctypes.c_int
     L.cgot_set_max_groups.argtypes = [i32]
     L.cgot_set_max_groups.restype = ctypes.c_int
     L.cgot_adjoint_workspace_size.argtypes = [i32, ctypes.POINTER(AdjointProblem)]
@@ -144,6 +156,8 @@ def load():
     L.cgot_profile_reset.restype = None
     L.cgot_profile_read.argtypes = [i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(i64)]
     L.cgot_profile_read.restype = ctypes.c_int
+    L.cgot_sparse_plan.argtypes = [i32, i32, vp, vp, i64, i32, vp, vp, i64]
+    L.cgot_sparse_plan.restype = i64
     L.cgot_shutdown.argtypes = []
     L.cgot_shutdown.restype = ctypes.c_int
     L.cgot_last_error.restype = ctypes.c_char_p
@@ -161,7 +175,7 @@ def check(rc: int, what: str):
 
 
 PROFILE_CLASSES = {"tpass": 0, "row_lse": 1, "col_lse": 2, "qv": 3, "chol_update": 4,
-                   "chol_panel": 5, "trsv": 6}
+                   "chol_panel": 5, "trsv": 6, "sp_newton": 7}
 
 
 def set_max_groups(groups: int):
@@ -169,6 +183,13 @@ def set_max_groups(groups: int):
     from the calling thread is split into (1..4; 1 = one lockstep group on the
     caller's stream)."""
     check(load().cgot_set_max_groups(int(groups)), "set_max_groups")
+
+
+def set_sparse_solve(on: bool) -> bool:
+    """Sparse factorization of the Newton system where the pattern admits it,
+    or the dense Schur Cholesky for every instance (process-wide); returns the
+    previous setting."""
+    return bool(load().cgot_set_sparse_solve(1 if on else 0))
 
 
 def profile_enable(on: bool):

@@ -23,7 +23,9 @@ SOURCES = [
     "cgot_sparse.hip",
     "cgot_dense.hip",
     "cgot_pcg.hip",
+    "cgot_spchol.hip",
     "cgot_driver.cpp",
+    "cgot_symbolic.cpp",
 ]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cgot_symbolic.h"
+
 #define CG_WG 256          // threads per workgroup (4 wave64)
 #define CG_WAVES 4
 #define CG_CMAX 4          // columns per lane in an M-pass strip
@@ -93,8 +95,13 @@
     /* backward with the SPLR preconditioner: device address of the forward */ \
     /* instance's descriptor (0 = none), Schur right-hand side and solution */ \
     unsigned long long fctx;                                                                     \
+    int fsp; /* 1: the forward instance's factor is the sparse plan */                           \
     Q double* prhs; /* [P of the forward instance] */                                            \
     Q double* pltmp; /* [P] */                                                                   \
+    /* sparse Newton solve: plan header (mode 1 = the plan in S is in use), */                   \
+    /* and the solve work vectors of the backward's sparse preconditioner */                     \
+    Q SpHdr* sph; /* [1] */                                                                      \
+    Q double* spx; /* [2N] (backward with a sparse forward factor) */                            \
     /* outputs */                                                                                \
     Q double* plan; /* [n*m] or null */                                                          \
     Q double* dual; /* [n+m] or null */                                                         

@@ -870,6 +870,10 @@ CG_DEV const CgInstD& cg_fwd_of(const CgInstD& I) { return *reinterpret_cast<con
 
 CG_DEV bool cg_pc_live(const CgInstD& I) { return I.fctx != 0 && I.scal[PCG_STATUS] == PCG_RUNNING; }
 
+// the preconditioner of this adjoint instance runs through the dense Schur
+// factor (a sparse forward factor is applied by k_sp_precond instead)
+CG_DEV bool cg_pc_dense_live(const CgInstD& I) { return cg_pc_live(I) && I.fsp == 0; }
+
 // Schur right-hand side of r (the PCG residual, N layout), wave per small index.
 __global__ __launch_bounds__(CG_WG) void k_pc_rhs(const CgInst* __restrict__ insts,
                                                   const int* __restrict__ wave_inst,
@@ -878,7 +882,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_rhs(const CgInst* __restrict__ ins
     int b, s;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, s)) return;
     const CgInstD& I = cg_inst(insts, b);
-    if (!cg_pc_live(I)) return;
+    if (!cg_pc_dense_live(I)) return;
     const CgInstD& F = cg_fwd_of(I);
     const int P = F.P, n = F.n;
     if (s >= P) return;
@@ -967,7 +971,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_fwd(const CgInst* __restrict__ ins
 {
     const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
     const CgInstD& I = cg_inst(insts, order[blockIdx.x - lvl_pref[t]]);
-    if (!cg_pc_live(I)) return;
+    if (!cg_pc_dense_live(I)) return;
     const CgInstD& F = cg_fwd_of(I);
     const CgTriArgs A{F.S, F.Winv, F.P, I.prhs, I.pltmp, I.flags, I.scal};
     cg_fwd_block(A, I.prhs, t, target);
@@ -978,7 +982,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_bwd(const CgInst* __restrict__ ins
 {
     const int t = cg_find_inst(lvl_pref, nlvl, blockIdx.x);
     const CgInstD& I = cg_inst(insts, order[blockIdx.x - lvl_pref[t]]);
-    if (!cg_pc_live(I)) return;
+    if (!cg_pc_dense_live(I)) return;
     const CgInstD& F = cg_fwd_of(I);
     const int nblk = (F.P + CG_NB - 1) / CG_NB;
     const CgTriArgs A{F.S, F.Winv, F.P, I.prhs, I.pltmp, I.flags, I.scal};
@@ -994,7 +998,7 @@ __global__ __launch_bounds__(CG_WG) void k_pc_back(const CgInst* __restrict__ in
     int b, q;
     if (!cg_decode_wave(insts, wave_inst, ctl, b, q)) return;
     const CgInstD& I = cg_inst(insts, b);
-    if (!cg_pc_live(I)) return;
+    if (!cg_pc_dense_live(I)) return;
     const CgInstD& F = cg_fwd_of(I);
     const int n = F.n;
     if (q >= F.N) return;

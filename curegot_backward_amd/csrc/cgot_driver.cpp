@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "cgot_kernels.h"
+#include "cgot_symbolic.h"
 #include "../../include/curegot_amd.h"
 
 namespace {
@@ -121,7 +122,7 @@ constexpr int kCtlSlots = 16;
 // (enabled by cgot_profile_enable; used by the benchmark for roofline figures).
 // ---------------------------------------------------------------------------
 enum { PROF_TPASS = 0, PROF_ROWLSE, PROF_COLLSE, PROF_QV, PROF_CHOL_UPDATE, PROF_CHOL_PANEL,
-       PROF_TRSV, PROF_NCLS = 8 };
+       PROF_TRSV, PROF_SPNEWTON, PROF_NCLS = 8 };
 struct ProfRec { hipEvent_t a, b; int cls; double work; };
 bool g_prof = false;
 thread_local std::vector<ProfRec> g_prof_pending;   // recorded by this thread, not yet read
@@ -220,6 +221,9 @@ struct Batch {
     int* d_pref = nullptr;     // dense prefix tables
     int pref_cap = 0;
     const double** d_ptrs = nullptr;   // 3*B pointers
+    int* d_ids = nullptr;              // B instance ids of a sparse-solve launch
+    std::vector<char> pc_sparse;       // adjoint: the forward factor is a sparse plan
+    std::vector<int> pc_R;             // adjoint: its core size
     unsigned int* zero_base = nullptr; // counters + histograms of all instances
     size_t zero_bytes = 0;
     hipStream_t stream = nullptr;
@@ -239,6 +243,7 @@ struct Batch {
         pref_cap = (4 * nblk_max + 3) * (B + 1) + nblk_max + 1;
         d_pref = bp.take<int>(pref_cap);
         d_ptrs = bp.take<const double*>(3 * (size_t)B);
+        d_ids = bp.take<int>(B);
     }
 
     void assign_maps()
@@ -580,6 +585,13 @@ struct Wolfe {
     }
 };
 
+// Largest dense core of a sparse plan on the current device (what its
+// workgroup LDS holds), and parallel-for over the library's analysis pool;
+// both defined with the worker pools below.
+int sp_rmax();
+std::atomic<int> g_sparse_on{0};   // cgot_set_sparse_solve
+void parallel_for(int count, const std::function<void(int)>& fn);
+
 // ---------------------------------------------------------------------------
 // SPLR batch solver
 // ---------------------------------------------------------------------------
@@ -598,6 +610,12 @@ struct HostInst {
     Wolfe ls;
     bool in_ls = false;
     double* trace = nullptr;
+    // sparse Newton solve: the plan of the current pattern is in use; a new
+    // pattern awaits its analysis (its W pattern is being copied to pinned
+    // memory at pat_off); core size
+    bool sparse = false, pend = false;
+    int spR = 0;
+    size_t pat_off = 0;
 };
 
 void resolve_defaults(const cgot_splr_problem& p, double& dmax, double& shift_max, int& cycle, int& cand)
@@ -682,6 +700,7 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.rcut = bp.take<int>(n);
         I.ccnt = bp.take<int>((size_t)I.nrb * m);
         I.S = bp.take<double>((size_t)I.P * I.P);
+        I.sph = bp.take<SpHdr>(1);
         I.Winv = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
         I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
         I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
@@ -690,12 +709,97 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
     return bp.off + 256;
 }
 
+thread_local Pinned g_pat_host;    // W patterns of new sparsity patterns (ints)
+thread_local Pinned g_plan_host;   // sparse plans and headers on their way to the device
+
 struct SplrSolver {
     Batch& bt;
     std::vector<HostInst> hi;
-
+    hipEvent_t pat_ev = nullptr;   // the pattern copies to g_pat_host are complete
 
     SplrSolver(Batch& b) : bt(b) {}
+    ~SplrSolver()
+    {
+        if (pat_ev) (void)hipEventDestroy(pat_ev);
+    }
+
+    // After a new top-K pattern (instances in newpat): copy W's row pattern
+    // (rowptr [n+1], colj [K]) to pinned host memory; the analysis waits for
+    // the copies just before the next Newton step, so the launches queued in
+    // between (the Sinkhorn candidate) run while the host analyses.
+    void request_patterns(const std::vector<char>& newpat)
+    {
+        const int B = bt.B;
+        size_t total = 0;
+        for (int b = 0; b < B; b++)
+            if (newpat[b]) {
+                hi[b].pat_off = total;
+                total += (size_t)hi[b].n + 1 + (size_t)hi[b].K;
+            }
+        if (total == 0) return;
+        int* hp = (int*)g_pat_host.get((total + 1) / 2 + 1);
+        for (int b = 0; b < B; b++) {
+            if (!newpat[b]) continue;
+            HostInst& H = hi[b];
+            CG_HIP(hipMemcpyAsync(hp + H.pat_off, bt.h[b].w_rowptr, sizeof(int) * (H.n + 1), hipMemcpyDeviceToHost,
+                                  bt.stream));
+            if (H.K > 0)
+                CG_HIP(hipMemcpyAsync(hp + H.pat_off + H.n + 1, bt.h[b].w_colj, sizeof(int) * H.K,
+                                      hipMemcpyDeviceToHost, bt.stream));
+            H.pend = true;
+        }
+        if (!pat_ev) CG_HIP(hipEventCreateWithFlags(&pat_ev, hipEventDisableTiming));
+        CG_HIP(hipEventRecord(pat_ev, bt.stream));
+    }
+
+    // Analyse the pending patterns (host, in parallel) and upload each plan
+    // into its instance's Schur buffer with its header (mode 0: the instance
+    // keeps the dense Schur path for this pattern).
+    void prepare_plans()
+    {
+        const int B = bt.B;
+        std::vector<int> todo;
+        for (int b = 0; b < B; b++)
+            if (hi[b].pend) todo.push_back(b);
+        if (todo.empty()) return;
+        CG_HIP(hipEventSynchronize(pat_ev));
+        const int* hp = (const int*)g_pat_host.p;
+        const int rmax = sp_rmax();
+        std::vector<SpHdr> hdr(todo.size());
+        std::vector<std::vector<char>> img(todo.size());
+        std::vector<char> ok(todo.size(), 0);
+        parallel_for((int)todo.size(), [&](int k) {
+            const int b = todo[k];
+            const HostInst& H = hi[b];
+            const int* rp = hp + H.pat_off;
+            const int64_t cap = (int64_t)sizeof(double) * bt.h[b].P * bt.h[b].P;
+            ok[k] = g_sparse_on.load() && sp_analyze(H.n, H.m, rp, rp + H.n + 1, cap, rmax, hdr[k], img[k]) ? 1 : 0;
+            if (!ok[k]) { hdr[k] = SpHdr{}; img[k].clear(); }
+        });
+        // previous uploads from g_plan_host have completed: every iteration
+        // synchronises the stream (line-search read-backs) before the next pattern
+        size_t total = 0;
+        std::vector<size_t> off(todo.size());
+        for (size_t k = 0; k < todo.size(); k++) {
+            off[k] = total;
+            total += (sizeof(SpHdr) + img[k].size() + 255) & ~(size_t)255;
+        }
+        char* up = (char*)g_plan_host.get(total / sizeof(double) + 1);
+        for (size_t k = 0; k < todo.size(); k++) {
+            const int b = todo[k];
+            char* dst = up + off[k];
+            std::memcpy(dst, &hdr[k], sizeof(SpHdr));
+            CG_HIP(hipMemcpyAsync(bt.h[b].sph, dst, sizeof(SpHdr), hipMemcpyHostToDevice, bt.stream));
+            if (ok[k]) {
+                std::memcpy(dst + sizeof(SpHdr), img[k].data(), img[k].size());
+                CG_HIP(hipMemcpyAsync(bt.h[b].S, dst + sizeof(SpHdr), img[k].size(), hipMemcpyHostToDevice,
+                                      bt.stream));
+            }
+            hi[b].sparse = ok[k] != 0;
+            hi[b].spR = ok[k] ? hdr[k].R : 0;
+            hi[b].pend = false;
+        }
+    }
 
     std::vector<double> mask(const std::vector<char>& on)
     {
@@ -765,12 +869,43 @@ struct SplrSolver {
             r[(size_t)b * CG_NCTL + CTL_SHIFT] = shift * hi[b].reg;
         }
         if (any) bt.rows0(k_refresh_values, bt.ctl(r));
+        request_patterns(newpat);
     }
 
-    // Search direction for the instances in `on`.
-    void search_direction(const std::vector<char>& on, const double* dc)
+    // Search direction for the instances in `on`: the sparse factor/solve for
+    // instances whose current pattern has a plan, the dense Schur path for the
+    // others.  Both write dir = A^{-1} grad and uv = A^{-1} yv.
+    void search_direction(const std::vector<char>& on_all, const double*)
     {
         const int B = bt.B;
+        prepare_plans();
+        std::vector<char> on(B, 0);
+        std::vector<int> ids;
+        int rmax_used = 0;
+        for (int b = 0; b < B; b++) {
+            if (!on_all[b]) continue;
+            if (hi[b].sparse) {
+                ids.push_back(b);
+                rmax_used = std::max(rmax_used, hi[b].spR);
+            } else {
+                on[b] = 1;
+            }
+        }
+        if (!ids.empty()) {
+            // ids staged through a pinned buffer: the previous copy out of it
+            // completed at the last iteration's read-back
+            static thread_local Pinned pinned_ids;
+            int* hid = (int*)pinned_ids.get(ids.size() / 2 + 1);
+            std::memcpy(hid, ids.data(), sizeof(int) * ids.size());
+            CG_HIP(hipMemcpyAsync(bt.d_ids, hid, sizeof(int) * ids.size(), hipMemcpyHostToDevice, bt.stream));
+            ProfScope ps(bt.stream, PROF_SPNEWTON, 0.0);
+            hipLaunchKernelGGL(k_sp_newton, dim3((unsigned)ids.size()), dim3(SP_WG), sp_lds_bytes(rmax_used),
+                               bt.stream, bt.d_insts, bt.d_ids);
+        }
+        bool anydense = false;
+        for (int b = 0; b < B; b++) anydense = anydense || on[b];
+        if (!anydense) return;
+        const double* dc = bt.ctl(mask(on));
         int nblk_max = 0;
         for (int b = 0; b < B; b++)
             if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
@@ -1124,6 +1259,8 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
 {
     const int B = bt.B;
     bt.h.assign(B, CgInst{});
+    bt.pc_sparse.assign(B, 0);
+    bt.pc_R.assign(B, 0);
     int nblk_max = 0;
     for (int b = 0; b < B; b++) {
         const auto& p = probs[b];
@@ -1136,13 +1273,23 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
         I.fctx = (unsigned long long)(uintptr_t)p.factor;
         I.P = 0;
         if (p.factor) {
-            // the forward descriptor: Schur size and orientation for the launch geometry
+            // the forward descriptor (Schur size and orientation for the launch
+            // geometry) and its plan header: a sparse factor is applied by
+            // k_sp_precond, a dense one by the Schur kernels
             CgInst F;
             CG_HIP(hipMemcpy(&F, p.factor, sizeof(CgInst), hipMemcpyDeviceToHost));
             if (F.n != p.n || F.m != p.m) throw std::invalid_argument("factor does not match the problem shape");
-            I.P = F.P;
-            I.beta_side = F.beta_side;
-            nblk_max = std::max(nblk_max, cdiv(F.P, CG_NB));
+            SpHdr fh{};
+            if (F.sph) CG_HIP(hipMemcpy(&fh, F.sph, sizeof(SpHdr), hipMemcpyDeviceToHost));
+            if (fh.mode == 1) {
+                I.fsp = 1;
+                bt.pc_sparse[b] = 1;
+                bt.pc_R[b] = fh.R;
+            } else {
+                I.P = F.P;
+                I.beta_side = F.beta_side;
+                nblk_max = std::max(nblk_max, cdiv(F.P, CG_NB));
+            }
         }
     }
     bt.assign_maps();
@@ -1156,7 +1303,9 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
         const int n = I.n, m = I.m, N = I.N;
         I.counter = bt.zero_base ? bt.zero_base + (size_t)b * zstride : nullptr;
         I.flags = bt.zero_base ? (int*)(I.counter + 4) : nullptr;
-        if (I.fctx) {
+        if (I.fctx && I.fsp) {
+            I.spx = bp.take<double>(2 * (size_t)N);
+        } else if (I.fctx) {
             I.prhs = bp.take<double>(I.P);
             I.pltmp = bp.take<double>(I.P);
         }
@@ -1182,6 +1331,8 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
 struct PcSolver {
     Batch& bt;
     std::vector<int> ord, lvl;
+    std::vector<int> sp_ids;   // instances whose forward factor is a sparse plan
+    size_t sp_lds = 0;
     int nlvl = 0;
     int* d_ord = nullptr;
     int* d_lvl = nullptr;
@@ -1189,8 +1340,20 @@ struct PcSolver {
 
     explicit PcSolver(Batch& b) : bt(b)
     {
-        for (int i = 0; i < bt.B; i++)
-            if (bt.h[i].fctx) ord.push_back(i);
+        for (int i = 0; i < bt.B; i++) {
+            if (!bt.h[i].fctx) continue;
+            if (bt.pc_sparse[i]) {
+                sp_ids.push_back(i);
+                sp_lds = std::max(sp_lds, sp_lds_bytes(bt.pc_R[i]));
+            } else {
+                ord.push_back(i);
+            }
+        }
+        if (!sp_ids.empty()) {
+            (void)sp_rmax();   // the kernels' LDS limit is raised once per process
+            CG_HIP(hipMemcpyAsync(bt.d_ids, sp_ids.data(), sizeof(int) * sp_ids.size(), hipMemcpyHostToDevice,
+                                  bt.stream));
+        }
         std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return bt.h[x].P > bt.h[y].P; });
         for (int i : ord) nlvl = std::max(nlvl, cdiv(bt.h[i].P, CG_NB));
         lvl.assign(nlvl + 1, 0);
@@ -1212,7 +1375,7 @@ struct PcSolver {
     // backward blocks are handed over through pltmp).
     void apply(const double* ctlp, int init)
     {
-        if (ord.empty()) return;
+        if (ord.empty() && sp_ids.empty()) return;
         solve(ctlp);
         lowrank_dots(ctlp);
         lowrank_apply(ctlp);
@@ -1223,6 +1386,10 @@ struct PcSolver {
     // (its dot products, then the correction of z)
     void solve(const double* ctlp)
     {
+        if (!sp_ids.empty())
+            hipLaunchKernelGGL(k_sp_precond, dim3((unsigned)sp_ids.size()), dim3(SP_WG), sp_lds, bt.stream,
+                               bt.d_insts, bt.d_ids);
+        if (ord.empty()) return;
         epoch++;
         bt.rows0(k_pc_rhs, ctlp);
         hipLaunchKernelGGL(k_pc_fwd, dim3(lvl[nlvl]), dim3(CG_WG), 0, bt.stream, bt.d_insts, d_ord, d_lvl, nlvl,
@@ -1487,9 +1654,17 @@ GroupPlan plan_splr_groups(int B, const cgot_splr_problem* probs)
 // starts new ones.
 class GroupPool {
 public:
+    // the size-class workers
     static GroupPool& get()
     {
         static GroupPool* p = new GroupPool();   // outlives every caller; workers joined by stop()
+        return *p;
+    }
+    // the sparse-plan analysis workers (separate, so that an analysis never
+    // waits behind a size-class job that is waiting for it)
+    static GroupPool& analysis()
+    {
+        static GroupPool* p = new GroupPool();
         return *p;
     }
     void run(std::vector<std::function<void()>>& jobs)
@@ -1548,6 +1723,50 @@ private:
     int idle_ = 0;
     bool stop_ = false;
 };
+
+void parallel_for(int count, const std::function<void(int)>& fn)
+{
+    if (count <= 0) return;
+    const int hw = std::max(1, (int)std::thread::hardware_concurrency());
+    const int nj = std::min({count, 8, hw});
+    if (nj == 1) {
+        for (int k = 0; k < count; k++) fn(k);
+        return;
+    }
+    std::vector<std::function<void()>> jobs;
+    std::vector<std::string> err(nj);
+    for (int j = 0; j < nj; j++)
+        jobs.push_back([&, j]() {
+            try {
+                for (int k = j; k < count; k += nj) fn(k);
+            } catch (const std::exception& e) {
+                err[j] = e.what();
+            }
+        });
+    GroupPool::analysis().run(jobs);
+    for (auto& e : err)
+        if (!e.empty()) throw std::runtime_error(e);
+}
+
+int sp_rmax()
+{
+    // per process: the device's LDS per workgroup bounds the packed core
+    static const int r = [] {
+        int dev = 0, lds = 0;
+        CG_HIP(hipGetDevice(&dev));
+        CG_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        int R = SP_RMAX;
+        while (R > 0 && sp_lds_bytes(R) > (size_t)lds) R--;
+        const int bytes = (int)sp_lds_bytes(R);
+        // raise the kernels' dynamic-LDS limit where the runtime asks for it;
+        // a launch above what the device allows fails at the launch check
+        (void)hipFuncSetAttribute((const void*)k_sp_newton, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_sp_precond, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipGetLastError();
+        return R;
+    }();
+    return r;
+}
 
 // CUs kept free of size classes 1.. when a batch is split: the largest
 // class, whose factorization chain (one small launch per block step) sets the
@@ -1626,6 +1845,7 @@ void splr_solve_group(const cgot_splr_problem* probs, int B, void* ws, hipStream
 void shutdown_resources()
 {
     GroupPool::get().stop();
+    GroupPool::analysis().stop();
     std::vector<hipStream_t> streams;
     std::vector<void*> pinned;
     {
@@ -1660,6 +1880,24 @@ void register_atexit()
 extern "C" {
 
 const char* cgot_last_error(void) { return g_last_error.c_str(); }
+
+int64_t cgot_sparse_plan(int32_t n, int32_t m, const int32_t* rowptr, const int32_t* colj, int64_t cap_bytes,
+                         int32_t rmax, void* hdr_out, void* plan_out, int64_t plan_cap)
+{
+    try {
+        if (n < 1 || m < 1 || !rowptr || (!colj && rowptr[n] > 0) || !hdr_out)
+            throw std::invalid_argument("bad arguments");
+        SpHdr h{};
+        std::vector<char> plan;
+        if (!sp_analyze(n, m, rowptr, colj, cap_bytes, rmax, h, plan)) return -1;
+        std::memcpy(hdr_out, &h, sizeof(SpHdr));
+        if (plan_out && plan_cap >= h.bytes) std::memcpy(plan_out, plan.data(), (size_t)h.bytes);
+        return h.bytes;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -2;
+    }
+}
 
 int cgot_shutdown(void)
 {
@@ -1783,6 +2021,11 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
         for (size_t q = 0; q < g.idx[k].size(); q++) info[g.idx[k][q]] = sub[k][q];
     return 0;
     CG_API_END
+}
+
+int cgot_set_sparse_solve(int32_t on)
+{
+    return g_sparse_on.exchange(on ? 1 : 0);
 }
 
 int cgot_set_max_groups(int32_t groups)

@@ -75,6 +75,10 @@ __global__ void k_pc_back(const CgInst*, const int* wave_inst, const double* ctl
 __global__ void k_pc_wdots(const CgInst*, const int* wg_inst, const double* ctl);
 __global__ void k_pc_wapply(const CgInst*, const int* wg_inst, const double* ctl);
 __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, int init);
+// sparse Newton solve (one workgroup of SP_WG threads per listed instance,
+// dynamic LDS sp_lds_bytes(R) of the largest core)
+__global__ void k_sp_newton(const CgInst*, const int* ids);
+__global__ void k_sp_precond(const CgInst*, const int* ids);
 __global__ void k_trsv_flow(const CgInst*, const int* order, const int* lvl_pref, int nlvl);
 
 // ---- backward PCG (vec grid) ----

@@ -84,6 +84,14 @@ size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
  * calling thread; 1 runs the whole batch in one lockstep group on `stream`.  No counterpart in the reference, whose
  * splr_ot_batch loops over instances (curegot/autograd.py:340-367). */
 int cgot_set_max_groups(int32_t groups);
+/* Newton step of the SPLR iteration.  With on != 0 every instance whose
+ * sparsity pattern admits it uses the sparse factorization of A (host
+ * minimum-degree analysis once per pattern, one-workgroup numeric factor and
+ * solves per iteration); the others, and all instances with on == 0 (the
+ * default), use the dense Cholesky of the Schur complement.  Process-wide;
+ * returns the previous setting.  Results agree between the two to rounding.
+ * The reference always factors A with cuDSS (src/linsolve.cu:27-268). */
+int cgot_set_sparse_solve(int32_t on);
 int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
                     void* stream, cgot_splr_info* info);
 
@@ -203,6 +211,19 @@ int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const
                     double* csr_values, int32_t* csr_colind, int32_t* csr_rowptr,
                     const double* rhs, double* sol,
                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Test hook (host only, no device work): the symbolic analysis of the sparse
+ * Newton solve for a sparsity pattern of W (rowptr [n+1], colj [K], columns
+ * j < m-1): minimum-degree ordering of the bipartite graph, the dense core,
+ * the elimination-tree levels and the gather lists of the factor and solves.
+ * hdr_out receives the plan header (64-bit-aligned struct of 10 int32 and
+ * 21 int64 fields, layout in csrc/cgot_symbolic.h); the plan image is copied
+ * to plan_out when plan_cap is large enough.  Returns the plan size in bytes,
+ * -1 when the instance keeps the dense Schur path (core above rmax or plan
+ * above cap_bytes), -2 on bad arguments.  Replaces the reordering and
+ * symbolic analysis of SparseCholeskySolver, src/linsolve.cu:171-176. */
+int64_t cgot_sparse_plan(int32_t n, int32_t m, const int32_t* rowptr, const int32_t* colj, int64_t cap_bytes,
+                         int32_t rmax, void* hdr_out, void* plan_out, int64_t plan_cap);
 
 /* Kernel timing (HIP events around each launch on the stream it runs on), for
  * roofline figures.  Classes: 0 T pass (bytes), 1 row LSE (bytes), 2 column LSE

@@ -188,10 +188,21 @@ def test_sparsify_ties_row_major(cg, n, m):
     assert rel(r["sol"], np.linalg.solve(Hs, rhs)) < 1e-10
 
 
-@pytest.mark.parametrize("n,m,K", [(200, 150, 600), (150, 200, 600), (300, 300, 3000), (70, 130, 1)])
-def test_newton_solve_vs_dense(cg, n, m, K):
-    # multi-block dense Cholesky (P > 64) on the Schur complement vs numpy.linalg.solve
-    print(f"newton n={n} m={m} K={K}", flush=True)
+@pytest.fixture(params=[True, False], ids=["sparse", "dense"])
+def newton_path(request):
+    from curegot_backward_amd import _lib
+    prev = _lib.set_sparse_solve(request.param)
+    yield request.param
+    _lib.set_sparse_solve(prev)
+
+
+@pytest.mark.parametrize("n,m,K", [(200, 150, 600), (150, 200, 600), (300, 300, 3000), (70, 130, 1),
+                                   (300, 300, 20000)])
+def test_newton_solve_vs_dense(cg, newton_path, n, m, K):
+    # the sparse factorization (minimum-degree periphery + dense LDS core), and
+    # the multi-block dense Cholesky (P > 64) of the Schur complement, vs
+    # numpy.linalg.solve on the assembled A
+    print(f"newton n={n} m={m} K={K} sparse={newton_path}", flush=True)
     rng = np.random.default_rng(n + m)
     M = rng.random((n, m))
     alpha = rng.normal(scale=0.01, size=n)
@@ -511,3 +522,27 @@ def test_backward_splr_batch_equals_single(cg):
     for r, s in zip(res, singles):
         assert r["iterations"] == s["iterations"]
         assert torch.equal(r["grad"], s["grad"])
+
+
+@pytest.mark.parametrize("n,m,eta,seed", [(300, 257, 0.01, 8), (129, 65, 0.02, 4), (65, 200, 0.1, 5)])
+def test_sparse_and_dense_newton_paths_agree(cg, n, m, eta, seed):
+    """The same SPLR solve with the sparse Newton factorization and with the
+    dense Schur Cholesky: both converge, and where they take the same number
+    of iterations the plans agree to 1e-10 (the two factorizations differ only
+    in rounding)."""
+    from curegot_backward_amd import _lib
+    from curegot_backward_amd.workloads import make_problem
+    M, a, b = [x.cuda() for x in make_problem(n, m, seed)]
+    prev = _lib.set_sparse_solve(False)
+    try:
+        dn = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 500)
+        _lib.set_sparse_solve(True)
+        sp = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 500)
+    finally:
+        _lib.set_sparse_solve(prev)
+    print(f"niter sparse {sp['niter']} dense {dn['niter']}", flush=True)
+    assert sp["converged"] and dn["converged"]
+    if sp["niter"] == dn["niter"]:
+        assert rel(sp["plan"], dn["plan"].cpu().numpy()) < 1e-10
+    else:
+        check_split(M.cpu().numpy(), a.cpu().numpy(), b.cpu().numpy(), eta, sp, 1e-8)
