@@ -73,10 +73,13 @@ class BcdProblem(ctypes.Structure):
 
 class SpHdr(ctypes.Structure):
     """Header of a sparse Newton-system plan (csrc/cgot_symbolic.h)."""
-    _fields_ = [(k, ctypes.c_int32) for k in ("mode", "N", "n", "m", "t", "R", "H", "nnzP", "ncore", "maxlev")] + \
-               [(k, ctypes.c_int64) for k in ("npairs", "ncpairs", "o_perm", "o_lvptr", "o_lcptr", "o_lcols",
-                                              "o_emeta", "o_erow", "o_ecol", "o_pairs",
-                                              "o_cmeta", "o_cpairs", "o_rptr", "o_rq", "o_lval",
+    _fields_ = [(k, ctypes.c_int32) for k in ("mode", "N", "n", "m", "t", "R", "H", "nnzP", "ncore", "T")] + \
+               [(k, ctypes.c_int64) for k in ("npairs", "ncpairs", "nrowpairs")] + \
+               [(k, ctypes.c_int32) for k in ("nlong", "nclong", "nylong", "nblong")] + \
+               [(k, ctypes.c_int64) for k in ("o_perm", "o_lvptr", "o_lcptr",
+                                              "o_lcols", "o_emeta", "o_erow", "o_pairs", "o_ymeta", "o_ycore",
+                                              "o_rowpairs", "o_cmeta", "o_cpairs", "o_lgptr", "o_long", "o_clong",
+                                              "o_ylong", "o_blptr", "o_blong", "o_cflags", "o_meta_end", "o_lval",
                                               "o_lcore", "o_yx", "bytes")]
 
 

@@ -93,30 +93,47 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
         std::vector<int>().swap(nb);
         remaining--;
     }
-    const int t = (int)order.size();
-    const int R = N - t;
-    if (R > rmax) return false;
+    const int t0 = (int)order.size();
+    if (N - t0 > rmax) return false;
     // core vertices in bucket order
     for (int d = 0; d <= N && (int)order.size() < N; d++)
         for (int v = head[d]; v >= 0; v = nxt[v]) order.push_back(v);
     std::vector<int> pos(N);
     for (int p = 0; p < N; p++) pos[order[p]] = p;
-    // ---- periphery columns: structure by position, levels of the elimination tree ----
-    std::vector<std::vector<int>> colrows(t);
-    std::vector<int> level(t, 0);
-    for (int j = 0; j < t; j++) {
+    // ---- elimination-tree levels of the eliminated vertices ----
+    std::vector<std::vector<int>> colrows(t0);
+    std::vector<int> level(t0, 0);
+    for (int j = 0; j < t0; j++) {
         std::vector<int>& r = colrows[j];
         for (int w : structv[order[j]]) r.push_back(pos[w]);
         std::sort(r.begin(), r.end());
     }
-    int H = 0;
-    for (int j = 0; j < t; j++) {
-        H = std::max(H, level[j] + 1);
-        if (!colrows[j].empty() && colrows[j][0] < t) {
+    for (int j = 0; j < t0; j++)
+        if (!colrows[j].empty() && colrows[j][0] < t0) {
             const int par = colrows[j][0];
             level[par] = std::max(level[par], level[j] + 1);
         }
+    // ---- split: the last eliminated vertices (the top of the tree, where its
+    // levels hold one or two columns each) join the dense core while that is
+    // cheaper: a periphery level costs about as much as SP_LEVEL_COST core
+    // columns (a level is two dependent gather phases and barriers, a core
+    // column one broadcast step of the register-tile elimination) ----
+    int t = t0;
+    {
+        std::vector<int> hpre(t0 + 1, 0);   // levels of the periphery [0, t')
+        for (int j = 0; j < t0; j++) hpre[j + 1] = std::max(hpre[j], level[j] + 1);
+        const int lo = std::max(0, N - rmax);
+        double best = 1e300;
+        for (int tc = t0; tc >= lo; tc--) {
+            const double cost = SP_LEVEL_COST * hpre[tc] + (double)(N - tc);
+            if (cost < best) { best = cost; t = tc; }
+        }
+        colrows.resize(t);
+        level.resize(t);
     }
+    const int R = N - t;
+    int H = 0;
+    for (int j = 0; j < t; j++) H = std::max(H, level[j] + 1);
     // columns in (level, position) order; entry indices: diagonal, then rows ascending
     std::vector<int> corder(t);
     for (int j = 0; j < t; j++) corder[j] = j;
@@ -146,12 +163,12 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     const int ncore = R * (R + 1) / 2;
     auto core_of = [&](int a, int b) { return (a - t) * (a - t + 1) / 2 + (b - t); };   // a >= b >= t
     // per-entry data
-    std::vector<int> edpos(nnzP), erow(nnzP), ecol(nnzP), easrc(nnzP, -1);
+    std::vector<int> edpos(nnzP), erow(nnzP), easrc(nnzP, -1);
     for (int j = 0; j < t; j++) {
         const int d = colstart[j];
-        edpos[d] = d; erow[d] = j; ecol[d] = j;
+        edpos[d] = d; erow[d] = j;
         for (size_t k = 0; k < colrows[j].size(); k++) {
-            edpos[d + 1 + k] = d; erow[d + 1 + k] = colrows[j][k]; ecol[d + 1 + k] = j;
+            edpos[d + 1 + k] = d; erow[d + 1 + k] = colrows[j][k];
         }
     }
     std::vector<int> casrc(ncore, -1);
@@ -182,12 +199,36 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     for (int64_t q = 0; q < nnzP; q++) ecnt[q + 1] += ecnt[q];
     for (int c = 0; c < ncore; c++) ccnt[c + 1] += ccnt[c];
     const int64_t npairs = ecnt[nnzP], ncpairs = ccnt[ncore];
-    // ---- row lists of the off-diagonal periphery entries (forward solve) ----
+    // ---- row-pair lists of the off-diagonal periphery entries (forward solve) ----
     std::vector<int> rptr(N + 1, 0);
     for (int64_t q = 0; q < nnzP; q++)
         if (edpos[q] != q) rptr[erow[q] + 1]++;
     for (int p = 0; p < N; p++) rptr[p + 1] += rptr[p];
-    // ---- layout ----
+    const int64_t nrowpairs = rptr[N];
+    // ---- lists long enough for a whole wave ----
+    std::vector<int> lgptr(H + 1, 0), blptr(H + 1, 0), longs, blong, clong, ylong, cflags(t, 0);
+    for (int l = 0; l < H; l++) {
+        lgptr[l] = (int)longs.size();
+        for (int q = lvptr[l]; q < lvptr[l + 1]; q++)
+            if (ecnt[q + 1] - ecnt[q] > SP_LONG) longs.push_back(q);
+        for (int c = lcptr[l]; c < lcptr[l + 1]; c++) {
+            const int j = corder[c];
+            if (rptr[j + 1] - rptr[j] > SP_LONG) longs.push_back(-(c + 1));
+        }
+        blptr[l] = (int)blong.size();
+        for (int c = lcptr[l]; c < lcptr[l + 1]; c++)
+            if ((int)colrows[corder[c]].size() > SP_LONG) {
+                blong.push_back(c);
+                cflags[c] = 1;
+            }
+    }
+    lgptr[H] = (int)longs.size();
+    blptr[H] = (int)blong.size();
+    for (int c = 0; c < ncore; c++)
+        if (ccnt[c + 1] - ccnt[c] > SP_LONG) clong.push_back(c);
+    for (int i = 0; i < R; i++)
+        if (rptr[t + i + 1] - rptr[t + i] > SP_LONG) ylong.push_back(i);
+    // ---- layout: integer arrays first (prefetched together), then values ----
     Layout L;
     SpHdr h{};
     h.mode = 1;
@@ -195,29 +236,40 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     h.t = t; h.R = R; h.H = H;
     h.nnzP = (int)nnzP;
     h.ncore = ncore;
+    h.T = (R + 3) / 4;
     h.npairs = npairs;
     h.ncpairs = ncpairs;
-    int maxlev = 0;
-    for (int l = 0; l < H; l++) maxlev = std::max(maxlev, lvptr[l + 1] - lvptr[l]);
-    h.maxlev = maxlev;
+    h.nrowpairs = nrowpairs;
+    h.nlong = (int)longs.size();
+    h.nclong = (int)clong.size();
+    h.nylong = (int)ylong.size();
+    h.nblong = (int)blong.size();
     h.o_perm = L.take(4LL * N);
     h.o_lvptr = L.take(4LL * (H + 1));
     h.o_lcptr = L.take(4LL * (H + 1));
     h.o_lcols = L.take(4LL * (t + 1));
     h.o_emeta = L.take(16 * nnzP);
     h.o_erow = L.take(4 * nnzP);
-    h.o_ecol = L.take(4 * nnzP);
     h.o_pairs = L.take(8 * npairs);
+    h.o_ymeta = L.take(16LL * t);
+    h.o_ycore = L.take(16LL * R);
+    h.o_rowpairs = L.take(8 * nrowpairs);
     h.o_cmeta = L.take(16LL * ncore);
     h.o_cpairs = L.take(8 * ncpairs);
-    h.o_rptr = L.take(4LL * (N + 1));
-    h.o_rq = L.take(4LL * std::max<int64_t>(rptr[N], 1));
+    h.o_lgptr = L.take(4LL * (H + 1));
+    h.o_long = L.take(4LL * h.nlong);
+    h.o_clong = L.take(4LL * h.nclong);
+    h.o_ylong = L.take(4LL * h.nylong);
+    h.o_blptr = L.take(4LL * (H + 1));
+    h.o_blong = L.take(4LL * h.nblong);
+    h.o_cflags = L.take(4LL * t);
+    h.o_meta_end = L.take(0, 16);
     h.o_lval = L.take(8 * nnzP);
     h.o_lcore = L.take(8LL * ncore);
     h.o_yx = L.take(16LL * N);
     h.bytes = L.take(0, 256);
     if (h.bytes > cap_bytes || npairs >= ((int64_t)1 << 31) || ncpairs >= ((int64_t)1 << 31)) return false;
-    // ---- fill the image (the value arrays Lval, Lcore and yx are left for the device) ----
+    // ---- fill the image (the value arrays lval, lcore and yx are left for the device) ----
     plan.assign((size_t)h.bytes, 0);
     char* base = plan.data();
     auto I32 = [&](int64_t off) { return reinterpret_cast<int*>(base + off); };
@@ -226,18 +278,39 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     std::memcpy(I32(h.o_lcptr), lcptr.data(), 4LL * (H + 1));
     std::memcpy(I32(h.o_lcols), lcols.data(), 4LL * (t + 1));
     std::memcpy(I32(h.o_erow), erow.data(), 4 * nnzP);
-    std::memcpy(I32(h.o_ecol), ecol.data(), 4 * nnzP);
+    auto put = [&](int64_t off, const std::vector<int>& v) {
+        if (!v.empty()) std::memcpy(I32(off), v.data(), 4 * v.size());
+    };
+    put(h.o_lgptr, lgptr);
+    put(h.o_long, longs);
+    put(h.o_clong, clong);
+    put(h.o_ylong, ylong);
+    put(h.o_blptr, blptr);
+    put(h.o_blong, blong);
+    put(h.o_cflags, cflags);
     {
         int* em = I32(h.o_emeta);
         for (int64_t q = 0; q < nnzP; q++) {
-            em[4 * q] = easrc[q]; em[4 * q + 1] = ecnt[q]; em[4 * q + 2] = ecnt[q + 1]; em[4 * q + 3] = edpos[q];
+            em[4 * q] = easrc[q]; em[4 * q + 1] = ecnt[q]; em[4 * q + 2] = ecnt[q + 1];
+            em[4 * q + 3] = edpos[q] | ((ecnt[q + 1] - ecnt[q] > SP_LONG) ? SP_LONG_BIT : 0);
         }
         int* cm = I32(h.o_cmeta);
         for (int c = 0; c < ncore; c++) {
-            cm[4 * c] = casrc[c]; cm[4 * c + 1] = ccnt[c]; cm[4 * c + 2] = ccnt[c + 1]; cm[4 * c + 3] = 0;
+            cm[4 * c] = casrc[c]; cm[4 * c + 1] = ccnt[c]; cm[4 * c + 2] = ccnt[c + 1];
+            cm[4 * c + 3] = (ccnt[c + 1] - ccnt[c] > SP_LONG) ? 1 : 0;
+        }
+        int* ym = I32(h.o_ymeta);
+        for (int c = 0; c < t; c++) {
+            const int j = corder[c];
+            ym[4 * c] = order[j]; ym[4 * c + 1] = rptr[j]; ym[4 * c + 2] = rptr[j + 1];
+            ym[4 * c + 3] = colstart[j] | ((rptr[j + 1] - rptr[j] > SP_LONG) ? SP_LONG_BIT : 0);
+        }
+        int* yc = I32(h.o_ycore);
+        for (int i = 0; i < R; i++) {
+            yc[4 * i] = order[t + i]; yc[4 * i + 1] = rptr[t + i]; yc[4 * i + 2] = rptr[t + i + 1];
+            yc[4 * i + 3] = (rptr[t + i + 1] - rptr[t + i] > SP_LONG) ? 1 : 0;
         }
     }
-    std::memcpy(I32(h.o_rptr), rptr.data(), 4LL * (N + 1));
     {
         // pairs in source-column order within each target (fixed summation order)
         int* pr = I32(h.o_pairs);
@@ -259,12 +332,16 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
                     }
                 }
         }
-        int* rq = I32(h.o_rq);
+        // row pairs (L(j, k), k) by row j, in increasing column position k
+        int* rpp = I32(h.o_rowpairs);
         std::vector<int> rf(rptr.begin(), rptr.end() - 1);
-        // by row, in increasing column position (entries are visited column by column in position order)
-        for (int j = 0; j < t; j++) {
-            const int d = colstart[j];
-            for (size_t k = 0; k < colrows[j].size(); k++) rq[rf[colrows[j][k]]++] = d + 1 + (int)k;
+        for (int k = 0; k < t; k++) {
+            const int d = colstart[k];
+            for (size_t x = 0; x < colrows[k].size(); x++) {
+                const int64_t s = rf[colrows[k][x]]++;
+                rpp[2 * s] = d + 1 + (int)x;
+                rpp[2 * s + 1] = k;
+            }
         }
     }
     hdr = h;

@@ -5,18 +5,27 @@
 // The vertices of A are v in [0, N), N = n + m - 1: v < n is alpha_v, v >= n is
 // beta_{v-n}.  A minimum-degree elimination of the bipartite graph of W orders
 // them; it stops once every remaining vertex is adjacent to at least half of
-// the others, and those R remaining vertices form the dense core.  Positions
+// the others.  Those, and the last eliminated vertices where moving them is
+// cheaper (the top of the elimination tree), form the dense core of R
+// vertices.  Positions
 // [0, t) are the periphery (sparse columns of L, in elimination order),
 // [t, N) the core (t = N - R).
 //
 // Periphery columns are stored in level order of the elimination tree
 // (leaves first; a column's level is one more than its deepest child's), each
 // column as its diagonal followed by its sub-diagonal entries in increasing
-// row order.  Every entry q of L is computed by a gather:
+// row order.  Every entry q of L is one gather
 //     L[q] = A(q) - sum over its pair list of L[x] * L[y]
-// then the square root (diagonal) or the division by its column's diagonal.
-// The core's Schur complement C = A22 - L21 L21^T is gathered the same way
-// (packed lower triangle, C(i, j) at i(i+1)/2 + j) and factored densely.
+// followed by the square root (diagonal) or the division by its column's
+// diagonal.  The forward substitution is the same gather
+// on two extra rows (the right-hand sides): y_j = (b_j - sum L(j,k) y_k) / L_jj
+// over the row-pair list of j, computed with the entries of j's level.  The
+// core's Schur complement C = A22 - L21 L21^T and its right-hand sides
+// b_core - L21 y are gathered the same way (C packed lower, C(i, j) at
+// i(i+1)/2 + j) and factored densely in registers (4 x 4 tiles per thread).
+// Each level has two phases: the gathers (diagonal entries take their square
+// root), then the divisions by the column diagonals (off-diagonal entries and
+// the right-hand sides).
 #pragma once
 
 #include <stdint.h>
@@ -34,31 +43,51 @@ struct SpHdr {
     int32_t t, R, H;     // periphery size, core size, periphery levels
     int32_t nnzP;        // entries of the periphery columns of L
     int32_t ncore;       // R (R + 1) / 2
-    int32_t maxlev;      // largest number of entries of one level
-    int64_t npairs, ncpairs;
-    // byte offsets from the plan base (the instance's Schur buffer)
-    int64_t o_perm;      // int [N]       vertex at each position
-    int64_t o_lvptr;     // int [H + 1]   entry range of each level
-    int64_t o_lcptr;     // int [H + 1]   column range (in lcols) of each level
-    int64_t o_lcols;     // int [t + 1]   first entry (the diagonal) of each column, level order; [t] = nnzP
-    int64_t o_emeta;     // int4 [nnzP]   {A-value source, pair range begin, end, its column's diagonal entry}
-    int64_t o_erow;      // int [nnzP]    row position
-    int64_t o_ecol;      // int [nnzP]    column position
-    int64_t o_pairs;     // int2 [npairs] (entry, entry)
-    int64_t o_cmeta;     // int4 [ncore]  {A-value source, pair range begin, end, 0}
+    int32_t T;           // core tiles per dimension, ceil(R / 4)
+    int64_t npairs, ncpairs, nrowpairs;
+    int32_t nlong, nclong, nylong, nblong;   // lists gathered by a whole wave (longer than SP_LONG)
+    // byte offsets from the plan base (the instance's Schur buffer); the
+    // integer arrays come first, in [0, o_meta_end)
+    int64_t o_perm;      // int [N]        vertex at each position
+    int64_t o_lvptr;     // int [H + 1]    entry range of each level
+    int64_t o_lcptr;     // int [H + 1]    column range (level order) of each level
+    int64_t o_lcols;     // int [t + 1]    first entry (the diagonal) of each column, level order; [t] = nnzP
+    int64_t o_emeta;     // int4 [nnzP]    {A-value source, pair range begin, end, its column's diagonal entry
+                         //                 (| SP_LONG_BIT when the pair list is long)}
+    int64_t o_erow;      // int [nnzP]     row position
+    int64_t o_pairs;     // int2 [npairs]  (entry, entry)
+    int64_t o_ymeta;     // int4 [t]       per column (level order): {vertex, row-pair range begin, end,
+                         //                 diagonal (| SP_LONG_BIT when long)}
+    int64_t o_ycore;     // int4 [R]       per core row: {vertex, row-pair range begin, end, 1 when long}
+    int64_t o_rowpairs;  // int2 [nrowpairs] (entry L(j, k), position k) of the periphery entries of each row
+    int64_t o_cmeta;     // int4 [ncore]   {A-value source, pair range begin, end, 1 when long}
     int64_t o_cpairs;    // int2 [ncpairs]
-    int64_t o_rptr;      // int [N + 1]   per row position: its off-diagonal periphery entries
-    int64_t o_rq;        // int [nnzP - t]
-    int64_t o_lval;      // double [nnzP] periphery columns of L
+    int64_t o_lgptr;     // int [H + 1]    range in `long` of each level
+    int64_t o_long;      // int [nlong]    long items of the factor levels: entry q >= 0, column c as -(c + 1)
+    int64_t o_clong;     // int [nclong]   long core entries (packed index)
+    int64_t o_ylong;     // int [nylong]   long core rows
+    int64_t o_blptr;     // int [H + 1]    range in `blong` of each level
+    int64_t o_blong;     // int [nblong]   long columns of the backward substitution (level order index)
+    int64_t o_cflags;    // int [t]        per column (level order): 1 when its backward sum is long
+    int64_t o_meta_end;
+    int64_t o_lval;      // double [nnzP]  periphery columns of L
     int64_t o_lcore;     // double [ncore] core factor (packed lower)
-    int64_t o_yx;        // double [2N]   solve work vectors (two right-hand sides), by position
+    int64_t o_yx;        // double [2N]    solve work vectors (two right-hand sides), by position
     int64_t bytes;       // plan size
 };
 
-// Largest core factored in one workgroup's LDS (packed lower triangle:
-// 196 * 197 / 2 doubles = 151 KiB of the 160 KiB of a gfx950 workgroup).
-#define SP_RMAX 196
+// Largest dense core: its lower triangle in 4 x 4 tiles, at most two tiles
+// per thread of the SP_WG-thread workgroup (T (T + 1) / 2 <= 2 SP_WG).
 #define SP_WG 1024
+#define SP_TILES_PER_THREAD 2
+#define SP_RMAX 252
+// Lists longer than this are summed by all 64 lanes of a wave (fixed-order
+// reduction); the item then carries SP_LONG_BIT in its record.
+#define SP_LONG 8
+#define SP_LONG_BIT 0x40000000
+// Cost of one periphery level in units of one dense core column, for the
+// choice of the periphery / core split.
+#define SP_LEVEL_COST 4.0
 
 #ifdef __cplusplus
 // Host analysis.  Returns true and fills `plan` (a byte image whose offsets
@@ -69,6 +98,8 @@ struct SpHdr {
 bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_bytes, int rmax, SpHdr& hdr,
                 std::vector<char>& plan);
 
-// LDS bytes the factor/solve kernel needs for a core of R.
-inline size_t sp_lds_bytes(int R) { return sizeof(double) * ((size_t)R * (R + 1) / 2 + 2 * (size_t)R + 8); }
+// LDS bytes the factor/solve kernel needs for a core of R: two column
+// broadcast buffers (4 ceil(R/4) + 2 <= R + 5 each), the pivots and two
+// right-hand sides.
+inline size_t sp_lds_bytes(int R) { return sizeof(double) * (5 * (size_t)R + 16); }
 #endif

@@ -216,8 +216,8 @@ int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const
  * Newton solve for a sparsity pattern of W (rowptr [n+1], colj [K], columns
  * j < m-1): minimum-degree ordering of the bipartite graph, the dense core,
  * the elimination-tree levels and the gather lists of the factor and solves.
- * hdr_out receives the plan header (64-bit-aligned struct of 10 int32 and
- * 21 int64 fields, layout in csrc/cgot_symbolic.h); the plan image is copied
+ * hdr_out receives the plan header (64-bit-aligned struct of 10 int32, 3 int64,
+ * 4 int32 and 24 int64 fields, layout in csrc/cgot_symbolic.h); the plan image is copied
  * to plan_out when plan_cap is large enough.  Returns the plan size in bytes,
  * -1 when the instance keeps the dense Schur path (core above rmax or plan
  * above cap_bytes), -2 on bad arguments.  Replaces the reordering and

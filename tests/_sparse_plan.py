@@ -54,16 +54,17 @@ def execute(h, buf, Da, Db, wval, rhs):
     lcptr = arr(buf, h.o_lcptr, H + 1)
     lcols = arr(buf, h.o_lcols, t + 1)
     em = arr(buf, h.o_emeta, 4 * h.nnzP).reshape(-1, 4)
-    easrc, edpos = em[:, 0], em[:, 3]
+    easrc, edpos = em[:, 0], em[:, 3] & ~0x40000000
     epp = np.append(em[:, 1], em[-1, 2] if h.nnzP else 0)
-    erow, ecol = arr(buf, h.o_erow, h.nnzP), arr(buf, h.o_ecol, h.nnzP)
+    erow = arr(buf, h.o_erow, h.nnzP)
     pairs = arr(buf, h.o_pairs, 2 * h.npairs).reshape(-1, 2)
+    ym = arr(buf, h.o_ymeta, 4 * t).reshape(-1, 4)
+    yc = arr(buf, h.o_ycore, 4 * R).reshape(-1, 4)
+    rowpairs = arr(buf, h.o_rowpairs, 2 * h.nrowpairs).reshape(-1, 2)
     cm = arr(buf, h.o_cmeta, 4 * h.ncore).reshape(-1, 4)
     casrc = cm[:, 0]
     cpp = np.append(cm[:, 1], cm[-1, 2] if h.ncore else 0)
     cpairs = arr(buf, h.o_cpairs, 2 * h.ncpairs).reshape(-1, 2)
-    rptr = arr(buf, h.o_rptr, N + 1)
-    rq = arr(buf, h.o_rq, int(rptr[-1]))
     Acat = np.concatenate([Da, Db, wval, [0.0]])      # index -1 -> the trailing 0
 
     def aval(src):
@@ -77,32 +78,34 @@ def execute(h, buf, Da, Db, wval, rhs):
                 s[k] = np.sum(Lv[prs[p0:p1, 0]] * Lv[prs[p0:p1, 1]])
         return s
 
+    k = rhs.shape[0]
+    b = rhs.copy()
+    y = np.zeros((k, N))
     Lv = np.zeros(h.nnzP)
     for lev in range(H):
+        # the level's entries and its columns' forward substitutions (one phase)
         q = np.arange(lvptr[lev], lvptr[lev + 1])
         s = aval(easrc[q]) - gathered(q, epp, pairs, Lv)
         dg = edpos[q] == q
         assert np.all(s[dg] > 0), "periphery pivot not positive"
         Lv[q[dg]] = np.sqrt(s[dg])
         Lv[q[~dg]] = s[~dg] / Lv[edpos[q[~dg]]]
+        for c in range(lcptr[lev], lcptr[lev + 1]):
+            vtx, r0, r1, d = ym[c]
+            d &= ~0x40000000
+            j = erow[d]
+            rp = rowpairs[r0:r1]
+            y[:, j] = (b[:, vtx] - (Lv[rp[:, 0]] * y[:, rp[:, 1]]).sum(1)) / Lv[d]
     C = aval(casrc) - gathered(np.arange(h.ncore), cpp, cpairs, Lv)
     Cd = np.zeros((R, R))
     Cd[np.tril_indices(R)] = C
-    Lc = np.linalg.cholesky(Cd + np.tril(Cd, -1).T) if R else Cd
-    k = rhs.shape[0]
-    y = np.zeros((k, N))
-    b = rhs[:, perm]
-    for lev in range(H):
-        for c in range(lcptr[lev], lcptr[lev + 1]):
-            d = lcols[c]
-            j = erow[d]
-            qs = rq[rptr[j]:rptr[j + 1]]
-            y[:, j] = (b[:, j] - (Lv[qs] * y[:, ecol[qs]]).sum(1)) / Lv[d]
-    for i in range(t, N):
-        qs = rq[rptr[i]:rptr[i + 1]]
-        y[:, i] = b[:, i] - (Lv[qs] * y[:, ecol[qs]]).sum(1)
+    for i in range(R):
+        vtx, r0, r1, _ = yc[i]
+        rp = rowpairs[r0:r1]
+        y[:, t + i] = b[:, vtx] - (Lv[rp[:, 0]] * y[:, rp[:, 1]]).sum(1)
     if R:
         import scipy.linalg as sla
+        Lc = np.linalg.cholesky(Cd + np.tril(Cd, -1).T)
         z = sla.solve_triangular(Lc, y[:, t:].T, lower=True)
         y[:, t:] = sla.solve_triangular(Lc.T, z, lower=False).T
     for lev in range(H - 1, -1, -1):
