@@ -188,11 +188,21 @@ def set_max_groups(groups: int):
     check(load().cgot_set_max_groups(int(groups)), "set_max_groups")
 
 
-def set_sparse_solve(on: bool) -> bool:
-    """Sparse factorization of the Newton system where the pattern admits it,
-    or the dense Schur Cholesky for every instance (process-wide); returns the
-    previous setting."""
-    return bool(load().cgot_set_sparse_solve(1 if on else 0))
+SPARSE_MODES = {"dense": 0, "auto": 1, "sparse": 2}
+
+
+def set_sparse_solve(mode) -> str:
+    """How the Newton system is factored (process-wide): "dense" (Schur
+    Cholesky for every instance), "sparse" (the sparse factorization wherever
+    the pattern admits one) or "auto" (sparse where its latency estimate is
+    the lower); True / False mean "sparse" / "dense".  Returns the previous
+    mode's name."""
+    if mode is True or mode is False:
+        mode = "sparse" if mode else "dense"
+    if mode not in SPARSE_MODES:
+        raise ValueError(f"sparse solve mode {mode!r} is not one of {sorted(SPARSE_MODES)}")
+    prev = load().cgot_set_sparse_solve(SPARSE_MODES[mode])
+    return {v: k for k, v in SPARSE_MODES.items()}[prev]
 
 
 def profile_enable(on: bool):

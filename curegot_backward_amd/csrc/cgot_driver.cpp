@@ -589,7 +589,7 @@ struct Wolfe {
 // workgroup LDS holds), and parallel-for over the library's analysis pool;
 // both defined with the worker pools below.
 int sp_rmax();
-std::atomic<int> g_sparse_on{0};   // cgot_set_sparse_solve
+std::atomic<int> g_sparse_on{1};   // cgot_set_sparse_solve: 0 dense, 1 by cost estimate, 2 sparse where a plan fits
 void parallel_for(int count, const std::function<void(int)>& fn);
 
 // ---------------------------------------------------------------------------
@@ -614,7 +614,7 @@ struct HostInst {
     // pattern awaits its analysis (its W pattern is being copied to pinned
     // memory at pat_off); core size
     bool sparse = false, pend = false;
-    int spR = 0;
+    int spR = 0, spH = 0;   // sparse plan: core size, periphery levels
     size_t pat_off = 0;
 };
 
@@ -765,6 +765,7 @@ struct SplrSolver {
         CG_HIP(hipEventSynchronize(pat_ev));
         const int* hp = (const int*)g_pat_host.p;
         const int rmax = sp_rmax();
+        const int mode = g_sparse_on.load();
         std::vector<SpHdr> hdr(todo.size());
         std::vector<std::vector<char>> img(todo.size());
         std::vector<char> ok(todo.size(), 0);
@@ -773,9 +774,16 @@ struct SplrSolver {
             const HostInst& H = hi[b];
             const int* rp = hp + H.pat_off;
             const int64_t cap = (int64_t)sizeof(double) * bt.h[b].P * bt.h[b].P;
-            ok[k] = g_sparse_on.load() && sp_analyze(H.n, H.m, rp, rp + H.n + 1, cap, rmax, hdr[k], img[k]) ? 1 : 0;
-            if (!ok[k]) { hdr[k] = SpHdr{}; img[k].clear(); }
+            ok[k] = mode != 0 && sp_analyze(H.n, H.m, rp, rp + H.n + 1, cap, rmax, hdr[k], img[k]) ? 1 : 0;
         });
+        // mode 1: the sparse plan where its estimate beats the instance's own
+        // dense chain; the choice depends on the instance alone, so results
+        // stay independent of the rest of the batch
+        if (mode == 1)
+            for (size_t k = 0; k < todo.size(); k++)
+                if (ok[k] && sp_cost_us(hdr[k]) > sp_dense_cost_us(hi[todo[k]].nblk)) ok[k] = 0;
+        for (size_t k = 0; k < todo.size(); k++)
+            if (!ok[k]) { hdr[k] = SpHdr{}; img[k].clear(); }
         // previous uploads from g_plan_host have completed: every iteration
         // synchronises the stream (line-search read-backs) before the next pattern
         size_t total = 0;
@@ -797,6 +805,7 @@ struct SplrSolver {
             }
             hi[b].sparse = ok[k] != 0;
             hi[b].spR = ok[k] ? hdr[k].R : 0;
+            hi[b].spH = ok[k] ? hdr[k].H : 0;
             hi[b].pend = false;
         }
     }
@@ -2023,9 +2032,10 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
     CG_API_END
 }
 
-int cgot_set_sparse_solve(int32_t on)
+int cgot_set_sparse_solve(int32_t mode)
 {
-    return g_sparse_on.exchange(on ? 1 : 0);
+    if (mode < 0 || mode > 2) return -1;
+    return g_sparse_on.exchange(mode);
 }
 
 int cgot_set_max_groups(int32_t groups)

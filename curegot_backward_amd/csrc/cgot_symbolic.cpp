@@ -205,7 +205,7 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
         if (edpos[q] != q) rptr[erow[q] + 1]++;
     for (int p = 0; p < N; p++) rptr[p + 1] += rptr[p];
     const int64_t nrowpairs = rptr[N];
-    // ---- lists long enough for a whole wave ----
+    // ---- lists summed by eight-lane groups ----
     std::vector<int> lgptr(H + 1, 0), blptr(H + 1, 0), longs, blong, clong, ylong, cflags(t, 0);
     for (int l = 0; l < H; l++) {
         lgptr[l] = (int)longs.size();
@@ -217,7 +217,7 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
         }
         blptr[l] = (int)blong.size();
         for (int c = lcptr[l]; c < lcptr[l + 1]; c++)
-            if ((int)colrows[corder[c]].size() > SP_LONG) {
+            if ((int)colrows[corder[c]].size() > SP_BLONG) {
                 blong.push_back(c);
                 cflags[c] = 1;
             }
@@ -225,7 +225,7 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     lgptr[H] = (int)longs.size();
     blptr[H] = (int)blong.size();
     for (int c = 0; c < ncore; c++)
-        if (ccnt[c + 1] - ccnt[c] > SP_LONG) clong.push_back(c);
+        if (ccnt[c + 1] - ccnt[c] > SP_CLONG) clong.push_back(c);
     for (int i = 0; i < R; i++)
         if (rptr[t + i + 1] - rptr[t + i] > SP_LONG) ylong.push_back(i);
     // ---- layout: integer arrays first (prefetched together), then values ----
@@ -297,7 +297,7 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
         int* cm = I32(h.o_cmeta);
         for (int c = 0; c < ncore; c++) {
             cm[4 * c] = casrc[c]; cm[4 * c + 1] = ccnt[c]; cm[4 * c + 2] = ccnt[c + 1];
-            cm[4 * c + 3] = (ccnt[c + 1] - ccnt[c] > SP_LONG) ? 1 : 0;
+            cm[4 * c + 3] = (ccnt[c + 1] - ccnt[c] > SP_CLONG) ? 1 : 0;
         }
         int* ym = I32(h.o_ymeta);
         for (int c = 0; c < t; c++) {

@@ -45,7 +45,7 @@ struct SpHdr {
     int32_t ncore;       // R (R + 1) / 2
     int32_t T;           // core tiles per dimension, ceil(R / 4)
     int64_t npairs, ncpairs, nrowpairs;
-    int32_t nlong, nclong, nylong, nblong;   // lists gathered by a whole wave (longer than SP_LONG)
+    int32_t nlong, nclong, nylong, nblong;   // lists summed by eight-lane groups
     // byte offsets from the plan base (the instance's Schur buffer); the
     // integer arrays come first, in [0, o_meta_end)
     int64_t o_perm;      // int [N]        vertex at each position
@@ -81,13 +81,32 @@ struct SpHdr {
 #define SP_WG 1024
 #define SP_TILES_PER_THREAD 2
 #define SP_RMAX 252
-// Lists longer than this are summed by all 64 lanes of a wave (fixed-order
-// reduction); the item then carries SP_LONG_BIT in its record.
-#define SP_LONG 8
+// Pair lists longer than these are summed by eight lanes (fixed-order
+// reduction) instead of one thread: SP_LONG for the periphery entries and the
+// right-hand-side rows (few items per level, so latency decides), SP_CLONG
+// for the core's Schur-complement entries (R (R + 1) / 2 items, throughput
+// decides) and SP_BLONG for the backward substitution's columns.  A long
+// periphery item carries SP_LONG_BIT in its record.
+#define SP_LONG 1
+#define SP_CLONG 8
+#define SP_BLONG 8
 #define SP_LONG_BIT 0x40000000
 // Cost of one periphery level in units of one dense core column, for the
 // choice of the periphery / core split.
 #define SP_LEVEL_COST 4.0
+
+// Latency estimates (microseconds per Newton solve of one instance), fitted
+// to measured MI355X timings, that decide whether an instance takes its sparse
+// plan.  Sparse: a fixed part, one barrier pair per periphery level, the dense
+// core's columns (linear and quadratic parts) and the prefetch of the plan
+// (about n + m).  Dense Schur chain: a fixed part (Schur build, right-hand
+// sides, triangular solves, back-substitution) and one dependent update
+// launch per CG_NB-block of the order P.
+inline double sp_cost_us(const SpHdr& h)
+{
+    return 10.7 + 6.44 * h.H + 0.34 * h.R + 8.6e-3 * (double)h.R * h.R + 20.4e-3 * (double)(h.n + h.m);
+}
+inline double sp_dense_cost_us(int nblk) { return 150.0 + 32.0 * nblk; }
 
 #ifdef __cplusplus
 // Host analysis.  Returns true and fills `plan` (a byte image whose offsets

@@ -84,14 +84,16 @@ size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
  * calling thread; 1 runs the whole batch in one lockstep group on `stream`.  No counterpart in the reference, whose
  * splr_ot_batch loops over instances (curegot/autograd.py:340-367). */
 int cgot_set_max_groups(int32_t groups);
-/* Newton step of the SPLR iteration.  With on != 0 every instance whose
- * sparsity pattern admits it uses the sparse factorization of A (host
- * minimum-degree analysis once per pattern, one-workgroup numeric factor and
- * solves per iteration); the others, and all instances with on == 0 (the
- * default), use the dense Cholesky of the Schur complement.  Process-wide;
- * returns the previous setting.  Results agree between the two to rounding.
- * The reference always factors A with cuDSS (src/linsolve.cu:27-268). */
-int cgot_set_sparse_solve(int32_t on);
+/* Newton step of the SPLR iteration: how A is factored.  mode 0: the dense
+ * Cholesky of the Schur complement for every instance; mode 2: the sparse
+ * factorization (host minimum-degree analysis once per pattern,
+ * one-workgroup numeric factor and solves per iteration) for every instance
+ * whose pattern admits one; mode 1 (the default): the sparse factorization
+ * only where the plan's latency estimate is below the dense one's.  Process-wide;
+ * returns the previous mode, or -1 for a mode outside 0..2.  Results agree
+ * between the two factorizations to rounding.  The reference always factors
+ * A with cuDSS (src/linsolve.cu:27-268). */
+int cgot_set_sparse_solve(int32_t mode);
 int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
                     void* stream, cgot_splr_info* info);
 
