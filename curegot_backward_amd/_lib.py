@@ -95,6 +95,7 @@ EXPORTS = (
     "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups", "cgot_set_sparse_solve",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
     "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply", "cgot_splr_precond_apply_profiled",
+    "cgot_splr_precond_apply_noalloc",
     "cgot_splr_context_csr",
     "cgot_bcd_workspace_size", "cgot_bcd_solve",
     "cgot_lse_workspace_size", "cgot_lse_update",
@@ -135,6 +136,8 @@ def load():
     L.cgot_splr_precond_workspace_size.restype = sz
     L.cgot_splr_precond_apply.argtypes = [vp, i32, i32, vp, vp, vp, sz, vp]
     L.cgot_splr_precond_apply.restype = ctypes.c_int
+    L.cgot_splr_precond_apply_noalloc.argtypes = [vp, i32, i32, vp, vp, vp]
+    L.cgot_splr_precond_apply_noalloc.restype = ctypes.c_int
     L.cgot_splr_precond_apply_profiled.argtypes = [vp, i32, i32, vp, vp, i32, i32, ctypes.POINTER(PrecondProfile),
                                                    vp, sz, vp]
     L.cgot_splr_precond_apply_profiled.restype = ctypes.c_int

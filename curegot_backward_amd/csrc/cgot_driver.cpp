@@ -230,6 +230,12 @@ struct Batch {
     int slot = 0;              // next ctl slot
     double* hctl = nullptr;    // pinned, kCtlSlots * B * CG_NCTL
     double* hscal = nullptr;   // pinned, B * CG_NSCAL
+    // Host staging in pageable memory instead of the pinned buffers (the
+    // entry point that allocates no GPU or pinned memory).  The ring of
+    // control slots and the synchronisations are the same either way.
+    bool pageable = false;
+    std::vector<int> pg_maps;
+    std::vector<double> pg_ctl, pg_scal;
     std::map<const double*, const double*> hmap;   // device ctl slot -> host copy
 
     void layout_common(Bump& bp, int nblk_max)
@@ -262,7 +268,8 @@ struct Batch {
     void write_maps(int* dst, const std::vector<char>& keep, int& nt, int& nv, int& nr)
     {
         const size_t cap = 2 * ((size_t)ntiles_full + nvec_full + nrow_full);
-        int* hm = (int*)g_map_host.get((cap + 1) / 2);
+        if (pageable) pg_maps.resize(cap + 1);
+        int* hm = pageable ? pg_maps.data() : (int*)g_map_host.get((cap + 1) / 2);
         nt = nv = nr = 0;
         for (int b = 0; b < B; b++) if (keep[b]) { nt += h[b].ntiles; nv += h[b].nvec; nr += h[b].nrow; }
         int* pt = hm;
@@ -319,8 +326,15 @@ struct Batch {
             CG_HIP(hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(void*) * ptrs.size(), hipMemcpyHostToDevice, stream));
         // pageable sources: make sure the copies are done before the vectors die
         CG_HIP(hipStreamSynchronize(stream));
-        hctl = g_ctl_host.get((size_t)kCtlSlots * B * CG_NCTL);
-        hscal = g_scal_host.get((size_t)B * CG_NSCAL);
+        if (pageable) {
+            pg_ctl.assign((size_t)kCtlSlots * B * CG_NCTL, 0.0);
+            pg_scal.assign((size_t)B * CG_NSCAL, 0.0);
+            hctl = pg_ctl.data();
+            hscal = pg_scal.data();
+        } else {
+            hctl = g_ctl_host.get((size_t)kCtlSlots * B * CG_NCTL);
+            hscal = g_scal_host.get((size_t)B * CG_NSCAL);
+        }
     }
 
     // Upload a control block (ctl[b*CG_NCTL + field]) and return its device copy.
@@ -1264,7 +1278,9 @@ void check_problem(const cgot_splr_problem& p)
 // ---------------------------------------------------------------------------
 // Adjoint (backward) batch: PCG on Q(T*) lambda = mu, T* recomputed from duals.
 // ---------------------------------------------------------------------------
-size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
+// precond_only: the buffers of a preconditioner application alone (no
+// M-pass partial sums).
+size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base, bool precond_only = false)
 {
     const int B = bt.B;
     bt.h.assign(B, CgInst{});
@@ -1324,10 +1340,12 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base)
         I.grad = bp.take<double>(N);
         I.rowsum = bp.take<double>(n);
         I.colsum = bp.take<double>(m);
-        I.rp0 = bp.take<double>((size_t)n * I.nstrips);
-        I.rp1 = bp.take<double>((size_t)n * I.nstrips);
-        I.cp0 = bp.take<double>((size_t)I.nrb * m);
-        I.cp1 = bp.take<double>((size_t)I.nrb * m);
+        if (!precond_only) {
+            I.rp0 = bp.take<double>((size_t)n * I.nstrips);
+            I.rp1 = bp.take<double>((size_t)n * I.nstrips);
+            I.cp0 = bp.take<double>((size_t)I.nrb * m);
+            I.cp1 = bp.take<double>((size_t)I.nrb * m);
+        }
         I.wgpart = bp.take<double>((size_t)8 * I.nvec);
     }
     return bp.off + 256;
@@ -2201,6 +2219,104 @@ int cgot_splr_precond_apply_profiled(const void* factor, int32_t n, int32_t m, c
         }
         *prof = pr;
     }
+    return 0;
+    CG_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cgot_splr_precond_apply_noalloc: the preconditioner application with its
+// work buffers in a fixed device array instead of a caller's workspace, for
+// callers that cannot allocate (code under graph capture, serving loops).
+// The array lives in the library's code object: every GPU holds its own copy
+// at its own address, taken once per device with hipGetSymbolAddress; the
+// host only ever passes that address (the array's name is a host-side
+// stand-in, not a device pointer).  Calls on one GPU share the array, so
+// calls that overlap on one GPU race; each call clears the counters and
+// scalars it relies on.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken on the first call
+// there).
+void* scratch_on_current_device()
+{
+    int dev = 0;
+    CG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        CG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Upper bound of the precond-only layout for (n, m) whatever the factor: the
+// layout without a factor, plus the factor's vectors (2N doubles for a sparse
+// plan, 2P <= 2N for a dense one), its block tables and the alignment of the
+// extra buffers.
+size_t noalloc_bound(int n, int m)
+{
+    Batch bt;
+    bt.B = 1;
+    cgot_adjoint_problem p{};
+    p.n = n; p.m = m; p.reg = 1.0;
+    const size_t N = (size_t)n + m - 1;
+    const size_t nblk = (N + CG_NB - 1) / CG_NB;
+    return adjoint_layout(bt, &p, nullptr, true) + 16 * N + 4 * (9 * nblk + 8) + 3 * 256;
+}
+}  // namespace
+
+extern "C" {
+
+int cgot_splr_precond_apply_noalloc(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                                    void* stream)
+{
+    CG_API_BEGIN
+    if (!factor || !r || !z) throw std::invalid_argument("factor, r and z must be non-null");
+    if (n < 1 || m < 2) throw std::invalid_argument("n >= 1 and m >= 2 required");
+    const size_t bound = noalloc_bound(n, m);
+    if (bound > kNoallocScratchBytes)
+        throw std::invalid_argument("problem too large for the fixed scratch: n=" + std::to_string(n) + ", m=" +
+                                    std::to_string(m) + " needs up to " + std::to_string(bound) +
+                                    " bytes, the limit is " + std::to_string(kNoallocScratchBytes) + " bytes");
+    char* base = (char*)scratch_on_current_device();
+    Batch bt;
+    bt.B = 1;
+    bt.stream = (hipStream_t)stream;
+    bt.pageable = true;
+    cgot_adjoint_problem p{};
+    p.n = n; p.m = m; p.reg = 1.0; p.factor = factor;
+    const size_t need = adjoint_layout(bt, &p, nullptr, true);
+    if (need > kNoallocScratchBytes)
+        throw std::invalid_argument("problem too large for the fixed scratch: needs " + std::to_string(need) +
+                                    " bytes, the limit is " + std::to_string(kNoallocScratchBytes) + " bytes");
+    adjoint_layout(bt, &p, base, true);
+    bt.upload_maps(std::vector<const double*>(3, nullptr));
+    CG_HIP(hipMemsetAsync(bt.zero_base, 0, bt.zero_bytes, bt.stream));
+    CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * CG_NSCAL, bt.stream));   // PCG status: running
+    const CgInst& I = bt.h[0];
+    const size_t N = (size_t)n + m - 1;
+    auto c = ctl_vec(1);
+    c[CTL_ACTIVE] = 1.0;
+    PcSolver pc(bt);
+    const double* dc = bt.ctl(c);
+    CG_HIP(hipMemcpyAsync(I.yv, r, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+    pc.apply(dc, 1);
+    CG_HIP(hipMemcpyAsync(z, I.sv, sizeof(double) * N, hipMemcpyDeviceToDevice, bt.stream));
+    bt.check_launch();
+    bt.read_scal();   // stream sync
+    if (bt.sc(0, CS_PIVOT) >= 2.0)
+        throw std::runtime_error("internal: a block-flag wait of the preconditioner exceeded its bound");
     return 0;
     CG_API_END
 }

@@ -137,6 +137,18 @@ size_t cgot_splr_precond_workspace_size(const void* factor, int32_t n, int32_t m
 int cgot_splr_precond_apply(const void* factor, int32_t n, int32_t m, const double* r, double* z,
                             void* workspace, size_t workspace_bytes, void* stream);
 
+/* cgot_splr_precond_apply without a workspace, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): no counterpart in the reference.  Its work buffers live
+ * in a fixed 4 MiB device array of the library (one per GPU, held until the
+ * process ends); it calls no hipMalloc, hipFree or pinned allocation.
+ * Problems whose buffers exceed the array are rejected with -2 and a message
+ * naming the sizes; every problem with n, m <= 24576 fits.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race.  Same result
+ * as cgot_splr_precond_apply, bit for bit. */
+int cgot_splr_precond_apply_noalloc(const void* factor, int32_t n, int32_t m, const double* r, double* z,
+                                    void* stream);
+
 /* Timings of cgot_splr_precond_apply_profiled, in microseconds.  host_*: time
  * in the calling thread (enqueueing, and the final wait); gpu_*: HIP-event
  * intervals on the stream.  There is no host copy in this design, so the
