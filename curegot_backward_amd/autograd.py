@@ -124,7 +124,10 @@ class SPLROTFunction(torch.autograd.Function):
                                    preconditioner=cfg["backward_preconditioner"],
                                    tol=cfg["backward_tol"], max_iter=cfg["backward_max_iter"])[0]
         _raise_backward(res, cfg)
-        return (res["grad"], None, None, None, None, None, None, None, None, None, None)
+        ga = res["grad_a"] if ctx.needs_input_grad[1] else None
+        gb = res["grad_b"] if ctx.needs_input_grad[2] else None
+        gM = res["grad"] if ctx.needs_input_grad[0] else None
+        return (gM, ga, gb, None, None, None, None, None, None, None, None)
 
 
 class _SPLROTBatchFunction(torch.autograd.Function):
@@ -158,32 +161,42 @@ class _SPLROTBatchFunction(torch.autograd.Function):
                 continue
             items.append((g.contiguous(), ctx.contexts[i], cfg["etas"][i]))
             idx.append(i)
-        gM = [None] * ctx.nb
+        B = ctx.nb
+        gM, gA, gB = [None] * B, [None] * B, [None] * B
         if items:
             res = solve_backward_batch(items, preconditioner=cfg["backward_preconditioner"],
                                        tol=cfg["backward_tol"], max_iter=cfg["backward_max_iter"])
+            need = ctx.needs_input_grad
             for i, r in zip(idx, res):
                 _raise_backward(r, cfg)
-                gM[i] = r["grad"]
-        return (None, *gM, *([None] * (2 * ctx.nb)))
+                gM[i] = r["grad"] if need[1 + i] else None
+                gA[i] = r["grad_a"] if need[1 + B + i] else None
+                gB[i] = r["grad_b"] if need[1 + 2 * B + i] else None
+        return (None, *gM, *gA, *gB)
 
 
 # ---------------------------------------------------------------------------
 # Public API (reference autograd.py:214-423)
 # ---------------------------------------------------------------------------
 def splr_ot(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None, candidate_sinkhorn_iter=3,
-            backward_preconditioner="diag", backward_tol=1e-12, backward_max_iter=2000):
+            backward_preconditioner="diag", backward_tol=1e-12, backward_max_iter=2000,
+            marginal_grads=False):
     """Differentiable entropic-OT plan via the SPLR solver (drop-in for
-    curegot.torch.splr_ot).  M [n, m] CUDA float64 is the only differentiable
-    input; a, b are constants.  Raises RuntimeError on forward or backward
-    non-convergence, NotImplementedError if a or b requires grad."""
+    curegot.torch.splr_ot).  M [n, m] CUDA float64 is differentiable; a, b are
+    constants unless marginal_grads=True (an extension: the reference has
+    only dL/dM), in which case gradients also flow to a and b:
+    dL/da = lambda_alpha and dL/db = (lambda_beta, 0) from the same adjoint
+    solve, for the system the forward solves (row sums a, the first m-1
+    column sums b; the last column takes up any imbalance).  Raises
+    RuntimeError on forward or backward non-convergence, NotImplementedError
+    if a or b requires grad without marginal_grads (as the reference does)."""
     if torch.is_autocast_enabled():
         raise RuntimeError("splr_ot: AMP/autocast is unsupported (the solver is float64-only); "
                            "disable autocast for this call")
     _check_inputs(M, a, b)
     _validate_scalars(eta, tol, max_iter, density, candidate_sinkhorn_iter, backward_tol,
                       backward_max_iter)
-    if a.requires_grad or b.requires_grad:
+    if (a.requires_grad or b.requires_grad) and not marginal_grads:
         raise NotImplementedError("splr_ot: gradients w.r.t. a/b are not supported (only dL/dM); "
                                   "detach them or use the lower-level API")
     _check_precond(backward_preconditioner)
@@ -207,13 +220,14 @@ def splr_ot_with_info(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None,
 
 def splr_ot_batch(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None,
                   candidate_sinkhorn_iter=3, backward_preconditioner="diag", backward_tol=1e-12,
-                  backward_max_iter=2000):
+                  backward_max_iter=2000, marginal_grads=False):
     """Batched splr_ot in one native call.
 
     M: [B, n, m] tensor (returns a stacked [B, n, m] plan, like the reference,
     autograd.py:340-367), or a list of [n_i, m_i] tensors with lists a, b
     (returns a list of plans).  eta may be a float or one value per instance.
-    Gradients flow to every M.
+    Gradients flow to every M, and with marginal_grads=True to a and b too
+    (as in splr_ot).
     """
     if torch.is_autocast_enabled():
         raise RuntimeError("splr_ot_batch: AMP/autocast is unsupported")
@@ -236,7 +250,7 @@ def splr_ot_batch(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None,
         raise ValueError("splr_ot_batch: empty batch")
     for i in range(B):
         _check_inputs(Ms[i], As[i], Bs[i], "splr_ot_batch")
-        if As[i].requires_grad or Bs[i].requires_grad:
+        if (As[i].requires_grad or Bs[i].requires_grad) and not marginal_grads:
             raise NotImplementedError("splr_ot_batch: gradients w.r.t. a/b are not supported")
     etas = [float(e) for e in eta] if isinstance(eta, (list, tuple)) else [float(eta)] * B
     if len(etas) != B:
@@ -258,7 +272,8 @@ class SPLROT(torch.nn.Module):
     (reference autograd.py:370-423)."""
 
     def __init__(self, eta, *, tol=1e-6, max_iter=1000, density=None, candidate_sinkhorn_iter=3,
-                 backward_preconditioner="diag", backward_tol=1e-12, backward_max_iter=2000):
+                 backward_preconditioner="diag", backward_tol=1e-12, backward_max_iter=2000,
+                 marginal_grads=False):
         super().__init__()
         _validate_scalars(float(eta), float(tol), int(max_iter), density,
                           int(candidate_sinkhorn_iter), float(backward_tol), int(backward_max_iter))
@@ -271,12 +286,14 @@ class SPLROT(torch.nn.Module):
         self.backward_preconditioner = backward_preconditioner
         self.backward_tol = float(backward_tol)
         self.backward_max_iter = int(backward_max_iter)
+        self.marginal_grads = bool(marginal_grads)
 
     def forward(self, M, a, b):
         return splr_ot(M, a, b, eta=self.eta, tol=self.tol, max_iter=self.max_iter,
                        density=self.density, candidate_sinkhorn_iter=self.candidate_sinkhorn_iter,
                        backward_preconditioner=self.backward_preconditioner,
-                       backward_tol=self.backward_tol, backward_max_iter=self.backward_max_iter)
+                       backward_tol=self.backward_tol, backward_max_iter=self.backward_max_iter,
+                       marginal_grads=self.marginal_grads)
 
     def extra_repr(self):
         return (f"eta={self.eta}, tol={self.tol}, density={self.density}, "

@@ -72,9 +72,16 @@ def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=20
     for lam, grad, info, (G, ctx, eta) in zip(lams, grads, infos, items):
         if info.status == 2:
             raise RuntimeError("PCG breakdown: p^T A p <= 0 (matrix not SPD?)")
+        n = ctx.M.shape[0]
         out.append({
             "lambda": lam,
             "grad": grad.to(G.dtype) if G.dtype != torch.float64 else grad,
+            # gradients w.r.t. the marginals from the same lambda: the forward
+            # solves row sums = a and the first m-1 column sums = b (beta_m = 0,
+            # the last column takes up any imbalance), so dL/da = lambda_alpha,
+            # dL/db = (lambda_beta, 0)
+            "grad_a": lam[:n],
+            "grad_b": torch.cat([lam[n:], lam.new_zeros(1)]),
             "iterations": int(info.iterations),
             "converged": bool(info.converged),
             "preconditioner": preconditioner,
@@ -89,7 +96,9 @@ def solve_backward(T, G, ctx, eta, *, preconditioner="diag", tol=1e-12, max_iter
     shape only: the plan is recomputed inside the kernels from the context's
     cost matrix and converged duals (it equals the forward's plan).
 
-    Returns dict(lambda [n+m-1], grad [n, m], iterations, converged, preconditioner).
+    Returns dict(lambda [n+m-1], grad [n, m], grad_a [n], grad_b [m],
+    iterations, converged, preconditioner); grad_a / grad_b are dL/da and
+    dL/db (the reference has no counterpart for these two).
     """
     _check(T, G)
     return solve_backward_batch([(G, ctx, eta)], preconditioner=preconditioner, tol=tol,

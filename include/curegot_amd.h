@@ -99,7 +99,9 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
 
 /* Implicit backward of the SPLR plan w.r.t. M for one instance of a batch:
  * solve Q(T*) lambda = mu by PCG (T* recomputed from M and the duals),
- * then dL/dM = T*.(lambda_a 1^T + 1 lambda_b^T - G)/reg. */
+ * then dL/dM = T*.(lambda_a 1^T + 1 lambda_b^T - G)/reg.  The same lambda
+ * gives the gradients w.r.t. the marginals (no counterpart in the reference,
+ * which raises for them): dL/da = lambda[0:n], dL/db = (lambda[n:n+m-1], 0). */
 typedef struct cgot_adjoint_problem {
     const double* M;        /* [n*m] */
     const double* dual;     /* [n+m] converged duals, beta[m-1] = 0 */

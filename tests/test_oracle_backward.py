@@ -119,3 +119,31 @@ def test_embedding_chain_rule_finite_differences():
         Xm[i, k] -= h
         fd = (loss(Xp) - loss(Xm)) / (2 * h)
         assert abs(fd - dX[i, k]) < 1e-6 * np.abs(dX).max(), (i, k, fd, dX[i, k])
+
+
+def test_marginal_gradients_central_finite_differences():
+    """dL/da = lambda_alpha and dL/db = (lambda_beta, 0): the gradients w.r.t.
+    the marginals of the reduced system the forward solves (row sums a, the
+    first m-1 column sums b, beta_m = 0; the last column's sum takes up any
+    imbalance), from the same lambda as dL/dM.  Central differences of
+    L = sum(T^2) + <T, R> through the brute-force dense Newton plan, every
+    entry of a and b (the last entry of b has no effect on the plan)."""
+    n, m, eta = 6, 5, 0.1
+    M, a, b = [x.numpy() for x in make_problem(n, m, 13)]
+    fw = oracle.splr_bwd(M, a, b, eta, tol=1e-10, max_iter=5000)
+    R = np.random.default_rng(14).normal(size=M.shape)
+    loss = lambda T: (T ** 2).sum() + (T * R).sum()
+    T0, dual0, _ = _dense.newton(M, a, b, eta)
+    lam = oracle.backward(M, dual0, 2 * T0 + R, eta, ctx=fw, tol=1e-14)["lambda"]
+    ga, gb = lam[:n], np.concatenate([lam[n:], [0.0]])
+    h = 1e-7
+    for vec, grad, name in ((a, ga, "a"), (b, gb, "b")):
+        for k in range(len(vec)):
+            vp, vm = vec.copy(), vec.copy()
+            vp[k] += h
+            vm[k] -= h
+            args_p = (vp, b) if name == "a" else (a, vp)
+            args_m = (vm, b) if name == "a" else (a, vm)
+            fd = (loss(_dense.newton(M, *args_p, eta)[0]) - loss(_dense.newton(M, *args_m, eta)[0])) / (2 * h)
+            assert abs(fd - grad[k]) <= 1e-6 * max(1.0, abs(grad[k])), (name, k, fd, grad[k])
+    assert abs(gb[-1]) == 0.0 and np.abs(ga).max() > 1e-3
