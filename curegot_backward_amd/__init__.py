@@ -28,7 +28,8 @@ from ._native import (  # noqa: E402
     sinkhorn_splr_bwd,
     sinkhorn_splr_bwd_batch,
 )
-from .autograd import SPLROT, SPLROTFunction, splr_ot, splr_ot_batch, splr_ot_with_info  # noqa: E402
+from .autograd import SPLROT, SPLROTFunction, splr_ot, splr_ot_batch, splr_ot_with_info, splr_ot_xy  # noqa: E402
+from .cost import GeometricCost  # noqa: E402
 from .backward import solve_backward, solve_backward_batch  # noqa: E402
 
 __version__ = "0.1.0"
@@ -37,7 +38,7 @@ __version__ = "0.1.0"
 torch = _types.ModuleType(__name__ + ".torch")
 for _name in ("splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT", "solve_backward",
               "sinkhorn_splr_bwd", "sinkhorn_splr_bwd_batch", "sinkhorn_splr", "sinkhorn_bcd",
-              "SPLRBackwardContext", "solve_backward_batch"):
+              "SPLRBackwardContext", "solve_backward_batch", "splr_ot_xy", "GeometricCost"):
     setattr(torch, _name, globals()[_name])
 _sys.modules[torch.__name__] = torch
 
@@ -55,4 +56,5 @@ _sys.modules[tests.__name__] = tests
 
 __all__ = ["torch", "numpy", "tests", "splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT",
            "SPLROTFunction", "solve_backward", "solve_backward_batch", "sinkhorn_splr_bwd",
-           "sinkhorn_splr_bwd_batch", "sinkhorn_splr", "sinkhorn_bcd", "SPLRBackwardContext"]
+           "sinkhorn_splr_bwd_batch", "sinkhorn_splr", "sinkhorn_bcd", "SPLRBackwardContext", "splr_ot_xy",
+           "GeometricCost"]

@@ -20,6 +20,11 @@ LIB_PATH = os.path.join(_HERE, "libcuregot_amd.so")
 _dp = ctypes.c_void_p   # device pointers are passed as integers
 
 
+class CostXY(ctypes.Structure):
+    """Geometric cost M_ij = scale * sum_k (X_ik - Y_jk)^2 (X = NULL: stored M)."""
+    _fields_ = [("X", _dp), ("Y", _dp), ("d", ctypes.c_int32), ("scale", ctypes.c_double)]
+
+
 class SplrProblem(ctypes.Structure):
     _fields_ = [
         ("M", _dp), ("a", _dp), ("b", _dp),
@@ -29,6 +34,7 @@ class SplrProblem(ctypes.Structure):
         ("density_max", ctypes.c_double), ("shift_max", ctypes.c_double),
         ("sparsity_pattern_cycle", ctypes.c_int32), ("candidate_sinkhorn_iter", ctypes.c_int32),
         ("x0", _dp), ("plan", _dp), ("dual", _dp), ("trace", _dp),
+        ("xy", CostXY),
     ]
 
 
@@ -53,6 +59,7 @@ class AdjointProblem(ctypes.Structure):
         ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
         ("lam", _dp), ("grad", _dp),
         ("factor", _dp),
+        ("xy", CostXY),
     ]
 
 
@@ -68,6 +75,7 @@ class BcdProblem(ctypes.Structure):
         ("reg", ctypes.c_double), ("tol", ctypes.c_double),
         ("max_iter", ctypes.c_int32),
         ("x0", _dp), ("plan", _dp), ("dual", _dp),
+        ("xy", CostXY),
     ]
 
 

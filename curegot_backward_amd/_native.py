@@ -16,6 +16,7 @@ from typing import Sequence
 import torch
 
 from . import _lib
+from .cost import GeometricCost
 
 __all__ = [
     "sinkhorn_splr_bwd",
@@ -31,6 +32,8 @@ def _stream(device) -> int:
 
 
 def _to_dev(t: torch.Tensor, device) -> torch.Tensor:
+    if isinstance(t, GeometricCost):
+        return t.to(device)
     return t.to(device=device, dtype=torch.float64).contiguous()
 
 
@@ -303,10 +306,28 @@ def _kwargs_density(kwargs, n, m):
     return dmax
 
 
+def _set_cost(p, M):
+    """The cost source of a problem struct: a stored M, or a GeometricCost's
+    points (then M is NULL)."""
+    if isinstance(M, GeometricCost):
+        p.M = None
+        p.xy.X, p.xy.Y, p.xy.d, p.xy.scale = M.X.data_ptr(), M.Y.data_ptr(), M.d, M.scale
+    else:
+        p.M = M.data_ptr()
+        p.xy.X = None
+
+
+def _check_one_cost_kind(Ms, what):
+    kinds = {isinstance(M, GeometricCost) for M in Ms}
+    if len(kinds) > 1:
+        raise ValueError(f"{what}: a batch must not mix geometric (X, Y) and stored (M) costs")
+
+
 def _make_problem(M, a, b, reg, tol, max_iter, kwargs, plan, dual, x0):
     n, m = M.shape
     p = _lib.SplrProblem()
-    p.M, p.a, p.b = M.data_ptr(), a.data_ptr(), b.data_ptr()
+    _set_cost(p, M)
+    p.a, p.b = a.data_ptr(), b.data_ptr()
     p.n, p.m = n, m
     p.reg, p.tol, p.max_iter = float(reg), float(tol), int(max_iter)
     p.density_max = _kwargs_density(kwargs, n, m)
@@ -350,6 +371,7 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
         return []
     if len(as_) != B or len(bs) != B:
         raise RuntimeError("Ms, as_ and bs must have the same length")
+    _check_one_cost_kind(Ms, "sinkhorn_splr_bwd_batch")
     regs = [float(r) for r in reg] if isinstance(reg, (list, tuple)) else [float(reg)] * B
     if len(regs) != B:
         raise RuntimeError("reg must be a float or have one value per instance")
@@ -459,7 +481,8 @@ def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     plan = torch.empty((n, m), dtype=torch.float64, device=device)
     dual = torch.empty(n + m, dtype=torch.float64, device=device)
     p = _lib.BcdProblem()
-    p.M, p.a, p.b = Md.data_ptr(), ad.data_ptr(), bd.data_ptr()
+    _set_cost(p, Md)
+    p.a, p.b = ad.data_ptr(), bd.data_ptr()
     p.n, p.m, p.reg, p.tol, p.max_iter = n, m, float(reg), float(tol), int(max_iter)
     p.x0 = x0.data_ptr() if x0 is not None else None
     p.plan, p.dual = plan.data_ptr(), dual.data_ptr()
@@ -482,6 +505,7 @@ def adjoint_batch(items, tol, max_iter):
     None[, factor pointer or None]).  Returns (lambdas, grads, infos)."""
     B = len(items)
     device = items[0][0].device
+    _check_one_cost_kind([it[0] for it in items], "solve_backward_batch")
     probs = (_lib.AdjointProblem * B)()
     lams, grads, keep = [], [], []
     for i, item in enumerate(items):
@@ -492,7 +516,8 @@ def adjoint_batch(items, tol, max_iter):
         lam = torch.empty(n + m - 1, dtype=torch.float64, device=device)
         grad = torch.empty((n, m), dtype=torch.float64, device=device)
         p = probs[i]
-        p.M, p.dual, p.G = M.data_ptr(), dual.data_ptr(), Gd.data_ptr()
+        _set_cost(p, M)
+        p.dual, p.G = dual.data_ptr(), Gd.data_ptr()
         p.n, p.m, p.reg = n, m, float(reg)
         p.inv_diag = inv.data_ptr() if inv is not None else None
         p.factor = factor

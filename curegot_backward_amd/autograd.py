@@ -19,6 +19,7 @@ import torch
 from torch.autograd.function import once_differentiable
 
 from . import _native
+from .cost import GeometricCost
 from .backward import _VALID_PRECONDITIONERS, solve_backward_batch
 
 __all__ = ["splr_ot", "splr_ot_with_info", "splr_ot_batch", "SPLROT", "SPLROTFunction"]
@@ -203,6 +204,79 @@ def splr_ot(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None, candidate_si
     return SPLROTFunction.apply(M, a, b, float(eta), float(tol), int(max_iter), density,
                                 int(candidate_sinkhorn_iter), backward_preconditioner,
                                 float(backward_tol), int(backward_max_iter))
+
+
+class _SPLROTXYFunction(torch.autograd.Function):
+    """Forward on a geometric cost (M formed from X, Y in the kernels).
+    Backward: the same adjoint as splr_ot, dL/dM carried to the points by
+    dM_ij/dX_i = 2 scale (X_i - Y_j), dM_ij/dY_j = -2 scale (X_i - Y_j)."""
+
+    @staticmethod
+    def forward(ctx, X, Y, a, b, scale, eta, tol, max_iter, density, candidate_sinkhorn_iter,
+                backward_preconditioner, backward_tol, backward_max_iter):
+        cost = GeometricCost(X, Y, scale)
+        out = _native.sinkhorn_splr_bwd(cost, a.detach(), b.detach(), eta, tol, max_iter, 0,
+                                        **_native_kwargs(density, candidate_sinkhorn_iter))
+        _raise_forward(out, max_iter, tol, "splr_ot_xy")
+        ctx.backward_context = out["context"]
+        ctx.cost = cost
+        ctx.eta = eta
+        ctx.cfg = dict(backward_preconditioner=backward_preconditioner, backward_tol=backward_tol,
+                       backward_max_iter=backward_max_iter)
+        ctx.save_for_backward(X, Y)
+        return out["plan"]
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_output):
+        ctx.saved_tensors   # version check: X, Y must not have been modified in place
+        G = grad_output if grad_output.is_contiguous() else grad_output.contiguous()
+        cfg = ctx.cfg
+        res = solve_backward_batch([(G, ctx.backward_context, ctx.eta)],
+                                   preconditioner=cfg["backward_preconditioner"],
+                                   tol=cfg["backward_tol"], max_iter=cfg["backward_max_iter"])[0]
+        _raise_backward(res, cfg)
+        gX = gY = None
+        if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+            gX, gY = ctx.cost.grad_points(res["grad"])
+        ga = res["grad_a"] if ctx.needs_input_grad[2] else None
+        gb = res["grad_b"] if ctx.needs_input_grad[3] else None
+        return (gX if ctx.needs_input_grad[0] else None, gY if ctx.needs_input_grad[1] else None, ga, gb,
+                None, None, None, None, None, None, None, None, None)
+
+
+def splr_ot_xy(X, Y, a, b, *, eta, scale=None, tol=1e-6, max_iter=1000, density=None,
+               candidate_sinkhorn_iter=3, backward_preconditioner="diag", backward_tol=1e-12,
+               backward_max_iter=2000, marginal_grads=False):
+    """splr_ot on the cost M_ij = scale * |X_i - Y_j|^2 (scale = 1/d by
+    default: the mean squared distance of the reference's workloads,
+    examples/differentiable_splr_ot.py:55) without forming M: every kernel
+    computes the entries it needs from the points.  X [n, d] and Y [m, d]
+    (CUDA float64) are differentiable; a, b as in splr_ot.  Same plan as
+    splr_ot on the stored matrix of the same formula (GeometricCost)."""
+    if torch.is_autocast_enabled():
+        raise RuntimeError("splr_ot_xy: AMP/autocast is unsupported (the solver is float64-only); "
+                           "disable autocast for this call")
+    for name, t in (("X", X), ("Y", Y)):
+        if not isinstance(t, torch.Tensor) or t.dim() != 2:
+            raise ValueError(f"splr_ot_xy: {name} must be a 2D tensor")
+        if t.dtype != torch.float64:
+            raise RuntimeError(f"splr_ot_xy: {name} must be float64 (got {t.dtype})")
+        if not t.is_cuda:
+            raise ValueError(f"splr_ot_xy: {name} must be a CUDA tensor")
+    if X.shape[1] != Y.shape[1]:
+        raise ValueError("splr_ot_xy: X and Y must have the same number of columns")
+    n, m = X.shape[0], Y.shape[0]
+    if a.dim() != 1 or b.dim() != 1 or a.shape[0] != n or b.shape[0] != m:
+        raise ValueError(f"splr_ot_xy: a must be [{n}] and b must be [{m}]")
+    _validate_scalars(eta, tol, max_iter, density, candidate_sinkhorn_iter, backward_tol,
+                      backward_max_iter)
+    if (a.requires_grad or b.requires_grad) and not marginal_grads:
+        raise NotImplementedError("splr_ot_xy: gradients w.r.t. a/b need marginal_grads=True")
+    _check_precond(backward_preconditioner, "splr_ot_xy")
+    return _SPLROTXYFunction.apply(X, Y, a, b, None if scale is None else float(scale), float(eta), float(tol),
+                                   int(max_iter), density, int(candidate_sinkhorn_iter), backward_preconditioner,
+                                   float(backward_tol), int(backward_max_iter))
 
 
 def splr_ot_with_info(M, a, b, *, eta, tol=1e-6, max_iter=1000, density=None,

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "cgot_symbolic.h"
 
 #define CG_WG 256          // threads per workgroup (4 wave64)
@@ -32,7 +34,12 @@
 // have the same layout.
 #define CG_INST_FIELDS(Q) \
                                                                                                  \
-    const Q double* M; /* [n*m] */                                                               \
+    const Q double* M; /* [n*m], unused for a geometric instance */                              \
+    /* geometric cost (X != null): M_ij = cscale * sum_k (X_ik - Y_jk)^2 formed on the fly */    \
+    const Q double* X; /* [n*d] */                                                               \
+    const Q double* Y; /* [m*d] */                                                               \
+    double cscale;                                                                               \
+    int d;                                                                                       \
     int n, m, N; /* N = n+m-1 */                                                                 \
     int c; /* columns per lane (1..CG_CMAX) */                                                   \
     int sw; /* strip width 64*c */                                                               \
@@ -122,6 +129,46 @@ CG_DEV const CgInstD& cg_inst(const CgInst* insts, int b)
     return reinterpret_cast<const CgInstD*>(insts)[b];
 }
 
+
+// Cost entry M_ij of a geometric instance: cscale * sum_k (X_ik - Y_jk)^2,
+// the terms added in k order by fused multiply-adds, then one rounded
+// multiply.  Contraction is off here, so the final product is never fused
+// into the caller's arithmetic and the value is the one a stored M of the
+// same formula holds.
+CG_DEV double cg_cost_xy(const CgInstD& I, int i, int j)
+{
+#pragma clang fp contract(off)
+    const CG_AS1 double* x = I.X + (size_t)i * I.d;
+    const CG_AS1 double* y = I.Y + (size_t)j * I.d;
+    double s = 0.0;
+    for (int k = 0; k < I.d; k++) {
+        const double t = x[k] - y[k];
+        s = fma(t, t, s);
+    }
+    return s * I.cscale;
+}
+
+// M_ij of any instance (stored or geometric); for the kernels that read a
+// few entries.
+CG_DEV double cg_cost(const CgInstD& I, int i, int j)
+{
+    return I.X ? cg_cost_xy(I, i, j) : I.M[(size_t)i * I.m + j];
+}
+
+// Row i of the cost at the lane's strip columns j = s0 + lane + 64 k (0 where
+// !ok[k]): loads of the stored M, or the geometric formula (XY).
+template <int C, bool XY>
+CG_DEV void cg_cost_row(const CgInstD& I, int i, int s0, int lane, const bool (&ok)[C], double (&mv)[C])
+{
+    if constexpr (XY) {
+#pragma unroll
+        for (int k = 0; k < C; k++) mv[k] = ok[k] ? cg_cost_xy(I, i, s0 + lane + 64 * k) : 0.0;
+    } else {
+        const auto Mi = I.M + (size_t)i * I.m + s0 + lane;
+#pragma unroll
+        for (int k = 0; k < C; k++) mv[k] = ok[k] ? Mi[64 * k] : 0.0;
+    }
+}
 
 // Scalar slots in CgInst::scal
 enum {
@@ -291,25 +338,31 @@ CG_DEV int cg_strip_c(const CgInstD& I, int st, int mlim)
 // when all 64*C columns of the strip lie below mlim (every strip but, at
 // most, an instance's last): the tile bodies then drop their per-column
 // bounds tests, which otherwise put every load under its own exec-mask branch.
-template <int C_, bool FULL_>
+// tag::XY: the cost is formed from X, Y (geometric instances).  The M-pass
+// kernels take XY as a template parameter and a launch covers instances of
+// one kind, so the stored-M instantiations are unchanged by the geometric
+// path.
+template <int C_, bool FULL_, bool XY_ = false>
 struct CgStrip {
     static constexpr int C = C_;
     static constexpr bool FULL = FULL_;
+    static constexpr bool XY = XY_;
 };
 
-template <class Fn>
+template <bool XY, class Fn>
 CG_DEV void cg_strip_dispatch(const CgInstD& I, int st, int mlim, Fn&& fn)
 {
     const int c = cg_strip_c(I, st, mlim);
     const bool full = mlim - st * I.sw >= 64 * c;
     if (c == 1) {
-        if (full) fn(CgStrip<1, true>{}); else fn(CgStrip<1, false>{});
+        if (full) fn(CgStrip<1, true, XY>{}); else fn(CgStrip<1, false, XY>{});
     } else if (c == 2) {
-        if (full) fn(CgStrip<2, true>{}); else fn(CgStrip<2, false>{});
+        if (full) fn(CgStrip<2, true, XY>{}); else fn(CgStrip<2, false, XY>{});
     } else {
-        if (full) fn(CgStrip<CG_CMAX, true>{}); else fn(CgStrip<CG_CMAX, false>{});
+        if (full) fn(CgStrip<CG_CMAX, true, XY>{}); else fn(CgStrip<CG_CMAX, false, XY>{});
     }
 }
+
 
 // Merge of two (max, sum-of-exp) log-sum-exp partials.
 CG_DEV void cg_lse_merge(double& mx, double& s, double mx2, double s2)

@@ -200,6 +200,27 @@ void set_geometry(CgInst& I, int n, int m, double reg)
     I.inv_reg = 1.0 / reg;
 }
 
+// The cost source of a descriptor: a stored M, or the geometric cost (X, Y).
+void set_cost(CgInst& I, const double* M, const cgot_cost_xy& xy)
+{
+    I.M = xy.X ? nullptr : M;
+    I.X = xy.X;
+    I.Y = xy.X ? xy.Y : nullptr;
+    I.d = xy.X ? xy.d : 0;
+    I.cscale = xy.X ? xy.scale : 0.0;
+}
+
+void check_cost(const double* M, const cgot_cost_xy& xy)
+{
+    if (xy.X) {
+        if (!xy.Y) throw std::invalid_argument("geometric cost: X and Y must both be non-null device pointers");
+        if (xy.d < 1) throw std::invalid_argument("geometric cost: d must be >= 1");
+        if (!std::isfinite(xy.scale)) throw std::invalid_argument("geometric cost: scale must be finite");
+    } else if (!M) {
+        throw std::invalid_argument("M must be a non-null device pointer (or give the geometric cost X, Y)");
+    }
+}
+
 // Maps and per-launch tables shared by all solvers.
 struct Batch {
     int B = 0;
@@ -235,6 +256,7 @@ struct Batch {
     // control slots and the synchronisations are the same either way.
     bool pageable = false;
     std::vector<int> pg_maps;
+    bool xy = false;           // every instance has the geometric cost (else none has)
     std::vector<double> pg_ctl, pg_scal;
     std::map<const double*, const double*> hmap;   // device ctl slot -> host copy
 
@@ -254,6 +276,11 @@ struct Batch {
 
     void assign_maps()
     {
+        // the M passes are launched for one cost kind
+        xy = !h.empty() && h[0].X != nullptr;
+        for (auto& I : h)
+            if ((I.X != nullptr) != xy)
+                throw std::invalid_argument("a batch must not mix geometric (X, Y) and stored (M) costs");
         ntiles = nvec = nrow = 0;
         for (auto& I : h) {
             I.tile0 = ntiles; ntiles += I.ntiles;
@@ -422,19 +449,19 @@ std::vector<double> ctl_vec(int B) { return std::vector<double>((size_t)B * CG_N
 // T pass (M pass + finalize) for the instances marked active in c.
 void tpass(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_tpass, c, mode, PROF_TPASS, mode == CG_PT_TRIAL ? 24.0 : 16.0);
+    bt.mpass(bt.xy ? k_tpass<true> : k_tpass<false>, c, mode, PROF_TPASS, mode == CG_PT_TRIAL ? 24.0 : 16.0);
     bt.vec(k_tpass_fin, c, mode);
 }
 
 void row_lse(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_row_lse, c, mode, PROF_ROWLSE, 16.0);
+    bt.mpass(bt.xy ? k_row_lse<true> : k_row_lse<false>, c, mode, PROF_ROWLSE, 16.0);
     bt.vec(k_rowlse_fin, c, mode);
 }
 
 void col_lse(Batch& bt, const double* c, int mode)
 {
-    bt.mpass(k_col_lse, c, mode, PROF_COLLSE, 16.0);
+    bt.mpass(bt.xy ? k_col_lse<true> : k_col_lse<false>, c, mode, PROF_COLLSE, 16.0);
     bt.vec(k_collse_fin, c, mode);
 }
 
@@ -664,7 +691,7 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.beta_side = (p.m - 1 <= p.n) ? 1 : 0;
         I.P = I.beta_side ? (p.m - 1) : p.n;
         nblk_max = std::max(nblk_max, cdiv(I.P, CG_NB));
-        I.M = p.M;
+        set_cost(I, p.M, p.xy);
         I.plan = p.plan;
         I.dual = p.dual;
     }
@@ -863,16 +890,16 @@ struct SplrSolver {
         if (anynew) {
             const double* dc = bt.ctl(c);
             for (int pass = 0; pass < 5; pass++) {
-                bt.mpass(k_hist, dc, pass);
+                bt.mpass(bt.xy ? k_hist<true> : k_hist<false>, dc, pass);
                 bt.per_inst(k_hist_scan, dc, pass);
             }
             bt.rows0(k_sel_rows_count, dc);
             bt.per_inst0(k_sel_rows_scan, dc);
             bt.rows0(k_sel_rows_write, dc);
-            bt.tiles0(k_sel_cols_count, dc);
+            bt.tiles0(bt.xy ? k_sel_cols_count<true> : k_sel_cols_count<false>, dc);
             bt.vec0(k_sel_cols_prefix, dc);
             bt.per_inst0(k_sel_cols_scan, dc);
-            bt.tiles0(k_sel_cols_write, dc);
+            bt.tiles0(bt.xy ? k_sel_cols_write<true> : k_sel_cols_write<false>, dc);
         }
         // instances without columns (m == 1): empty pattern
         for (int b = 0; b < B; b++) {
@@ -1268,7 +1295,8 @@ struct SplrSolver {
 void check_problem(const cgot_splr_problem& p)
 {
     if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
-    if (!p.M || !p.a || !p.b) throw std::invalid_argument("M, a and b must be non-null device pointers");
+    if (!p.a || !p.b) throw std::invalid_argument("a and b must be non-null device pointers");
+    check_cost(p.M, p.xy);
     if (!(p.reg > 0.0) || !std::isfinite(p.reg)) throw std::invalid_argument("reg must be positive and finite");
     if ((long long)p.n * (long long)p.m >= (1LL << 31) || (long long)p.n + p.m >= (1LL << 28))
         throw std::invalid_argument("instance too large for 32-bit indices");
@@ -1291,7 +1319,7 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base, 
         const auto& p = probs[b];
         CgInst& I = bt.h[b];
         set_geometry(I, p.n, p.m, p.reg);
-        I.M = p.M;
+        set_cost(I, p.M, p.xy);
         I.gamma = const_cast<double*>(p.dual);
         I.dir = p.lambda;
         I.Da = const_cast<double*>(p.inv_diag);
@@ -1445,7 +1473,8 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
     double* const* outs = (double* const*)(bt.d_ptrs + 2 * B);
     PcSolver pc(bt);
     auto precondition = [&](const double* ctlp, int init) { pc.apply(ctlp, init); };
-    hipLaunchKernelGGL(k_qv_rhs, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, dc, Gs);
+    hipLaunchKernelGGL(bt.xy ? k_qv_rhs<true> : k_qv_rhs<false>, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream,
+                       bt.d_insts, bt.d_tile, dc, Gs);
     bt.vec0(k_pcg_init, dc);
     precondition(dc, 1);
     bt.check_launch();
@@ -1464,7 +1493,8 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         const double* d2 = bt.ctl(c2);
         for (int k = 0; k < chunk; k++) {
             ProfScope ps(bt.stream, PROF_QV, bt.mpass_bytes(bt.hmap[d2], 24.0));
-            hipLaunchKernelGGL(k_qv_pcg, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_tile, d2);
+            hipLaunchKernelGGL(bt.xy ? k_qv_pcg<true> : k_qv_pcg<false>, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream,
+                               bt.d_insts, bt.d_tile, d2);
             bt.vec0(k_pcg_q_fin, d2);
             bt.vec0(k_pcg_update, d2);
             precondition(d2, 0);
@@ -1509,7 +1539,7 @@ size_t bcd_layout(Batch& bt, const cgot_bcd_problem* probs, void* base)
         const auto& p = probs[b];
         CgInst& I = bt.h[b];
         set_geometry(I, p.n, p.m, p.reg);
-        I.M = p.M;
+        set_cost(I, p.M, p.xy);
         I.plan = p.plan;
         I.dual = p.dual;
     }
@@ -2087,7 +2117,8 @@ int cgot_adjoint_solve(int32_t B, const cgot_adjoint_problem* probs, void* works
     for (int b = 0; b < B; b++) {
         const auto& p = probs[b];
         if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
-        if (!p.M || !p.dual || !p.G || !p.lambda || !p.grad) throw std::invalid_argument("null pointer");
+        if (!p.dual || !p.G || !p.lambda || !p.grad) throw std::invalid_argument("null pointer");
+        check_cost(p.M, p.xy);
         if (!(p.reg > 0.0)) throw std::invalid_argument("reg must be positive");
     }
     Batch bt;
@@ -2363,7 +2394,8 @@ int cgot_bcd_solve(int32_t B, const cgot_bcd_problem* probs, void* workspace, si
     for (int b = 0; b < B; b++) {
         const auto& p = probs[b];
         if (p.n < 1 || p.m < 1) throw std::invalid_argument("n and m must be >= 1");
-        if (!p.M || !p.a || !p.b) throw std::invalid_argument("null pointer");
+        if (!p.a || !p.b) throw std::invalid_argument("null pointer");
+        check_cost(p.M, p.xy);
         if (!(p.reg > 0.0)) throw std::invalid_argument("reg must be positive");
     }
     Batch bt;

@@ -18,10 +18,15 @@ enum {
 enum { CG_PT_GAMMA = 0, CG_PT_TRIAL = 1, CG_PT_BCD = 2 };
 
 // ---- M passes (grid = total tiles; tile_inst maps tile -> instance) ----
+template <bool XY>
 __global__ void k_tpass(const CgInst*, const int*, const double*, int mode);
+template <bool XY>
 __global__ void k_row_lse(const CgInst*, const int*, const double*, int mode);
+template <bool XY>
 __global__ void k_col_lse(const CgInst*, const int*, const double*, int mode);
+template <bool XY>
 __global__ void k_qv_rhs(const CgInst*, const int*, const double*, const double* const* Gs);
+template <bool XY>
 __global__ void k_qv_pcg(const CgInst*, const int*, const double*);
 __global__ void k_plan(const CgInst*, const int*, const double*);
 __global__ void k_grad_assemble(const CgInst*, const int*, const double*, const double* const*,
@@ -46,15 +51,18 @@ __global__ void k_bcd_diff(const CgInst*, const int*, const double*);
 __global__ void k_hsl_csr(const CgInst*, const int*, const double*, double*, int*, int*);
 
 // ---- sparsified Hessian (top-K pattern and values) ----
+template <bool XY>
 __global__ void k_hist(const CgInst*, const int*, const double*, int pass);
 __global__ void k_hist_scan(const CgInst*, const double*, int pass);              // grid = B
 __global__ void k_sel_rows_count(const CgInst*, const int* row_inst, const double*); // row grid
 __global__ void k_sel_rows_scan(const CgInst*, const double*);                       // grid = B
 __global__ void k_sel_rows_write(const CgInst*, const int* row_inst, const double*); // row grid
+template <bool XY>
 __global__ void k_sel_cols_count(const CgInst*, const int*, const double*);
 __global__ void k_ctx_csr(const CgInst* fwd, double* vals, int* cols, int* rowptr);   // stored context
 __global__ void k_sel_cols_prefix(const CgInst*, const int*, const double*);      // vector map
 __global__ void k_sel_cols_scan(const CgInst*, const double*);                    // grid = B
+template <bool XY>
 __global__ void k_sel_cols_write(const CgInst*, const int*, const double*);
 __global__ void k_refresh_values(const CgInst*, const int*, const double*);       // wave grid
 

@@ -78,7 +78,7 @@ CG_DEV double cg_rows_reduce(const double (&v)[R], double* red)
 // Each wave takes its CG_RPW rows in groups of R and issues the loads of a
 // whole group before the exponentials, so R*C loads per lane are in flight.
 // ---------------------------------------------------------------------------
-template <int C, bool FULL>
+template <int C, bool FULL, bool XY>
 CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, double (*lds)[CG_CMAX * 64],
                           const double* tab)
 {
@@ -111,9 +111,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = cg_point(I, mode, t, i);
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -149,6 +147,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
     }
 }
 
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_tpass(const CgInst* __restrict__ insts,
                                                  const int* __restrict__ tile_inst,
                                                  const double* __restrict__ ctl, int mode)
@@ -159,9 +158,9 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     const double t = ctl[b * CG_NCTL + CTL_STEP];
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];   // one array for every instantiation
-    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
-        cg_tpass_tile<S::C, S::FULL>(I, mode, t, rb, st, lds, tab);
+        cg_tpass_tile<S::C, S::FULL, S::XY>(I, mode, t, rb, st, lds, tab);
     });
 }
 
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // of one 64-lane DPP/readlane reduction per row and quantity.
 // ---------------------------------------------------------------------------
 
-template <int C, bool FULL>
+template <int C, bool FULL, bool XY>
 CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab, double* red,
                            double* rmx)
 {
@@ -201,9 +200,7 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
         }
         // x = (beta_j - M_ij)/reg, -inf past m; formed again for the sums
         // instead of being held
@@ -238,6 +235,7 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
     }
 }
 
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_row_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
@@ -249,9 +247,9 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     __shared__ double red[CG_WAVES][8 * 64];   // row partials of a group, per wave
     __shared__ double rmx[CG_WAVES][8];        // row maxima of a group, per wave
     const int w = threadIdx.x >> 6;
-    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
-        cg_rowlse_tile<S::C, S::FULL>(I, mode, rb, st, tab, red[w], rmx[w]);
+        cg_rowlse_tile<S::C, S::FULL, S::XY>(I, mode, rb, st, tab, red[w], rmx[w]);
     });
 }
 
@@ -262,7 +260,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // with an online max/sum, reads stay row-major and coalesced, R rows' loads
 // in flight at a time).
 // ---------------------------------------------------------------------------
-template <int C, bool FULL>
+template <int C, bool FULL, bool XY>
 CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const double* tab,
                            double (*lmx)[CG_CMAX * 64], double (*lsm)[CG_CMAX * 64])
 {
@@ -288,9 +286,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);
             ai[r] = alpha[i] * ir;
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
         }
         // online update without lane branches: e = exp(-|x - mx|) serves
         // both cases (a new maximum rescales the sum by e and adds 1, any
@@ -325,6 +321,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
     }
 }
 
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_col_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
@@ -335,9 +332,9 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     __shared__ double lmx[CG_WAVES][CG_CMAX * 64];
     __shared__ double lsm[CG_WAVES][CG_CMAX * 64];
-    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
-        cg_collse_tile<S::C, S::FULL>(I, mode, rb, st, tab, lmx, lsm);
+        cg_collse_tile<S::C, S::FULL, S::XY>(I, mode, rb, st, tab, lmx, lsm);
     });
 }
 
@@ -350,7 +347,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 // Reference: q_matvec / rhs_Q_scale, curegot/_backward_core.py:31-45 (3-4 torch
 // passes over a stored T there, one pass over M here).
 // ---------------------------------------------------------------------------
-template <int C, bool FULL, bool RHS>
+template <int C, bool FULL, bool XY, bool RHS>
 CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
                        const double* tab)
 {
@@ -385,9 +382,7 @@ CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* co
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = I.gamma[i];
             va[r] = MV ? vat(i) : 0.0;
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
             if (RHS) {
                 const auto Gi = G + (size_t)i * m + s0 + lane;
 #pragma unroll
@@ -447,17 +442,18 @@ CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* co
     }
 }
 
-template <bool RHS>
+template <bool XY, bool RHS>
 CG_DEV void cg_qv_strip(const CgInstD& I, int b, int rb, int st, const double* const* Gs, double (*lds)[CG_CMAX * 64],
                         const double* tab)
 {
-    cg_strip_dispatch(I, st, I.m, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, I.m, [&](auto tag) {
         using S = decltype(tag);
-        cg_qv_tile<S::C, S::FULL, RHS>(I, b, rb, st, Gs, lds, tab);
+        cg_qv_tile<S::C, S::FULL, S::XY, RHS>(I, b, rb, st, Gs, lds, tab);
     });
 }
 
 // once per adjoint solve (loads G beside M; no register cap)
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) void k_qv_rhs(const CgInst* __restrict__ insts, const int* __restrict__ tile_inst,
                                                   const double* __restrict__ ctl, const double* const* __restrict__ Gs)
 {
@@ -466,10 +462,11 @@ __global__ __launch_bounds__(CG_WG) void k_qv_rhs(const CgInst* __restrict__ ins
     if (!cg_decode_tile(insts, tile_inst, ctl, b, rb, st)) return;
     const CgInstD& I = cg_inst(insts, b);
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];
-    cg_qv_strip<true>(I, b, rb, st, Gs, lds, tab);
+    cg_qv_strip<XY, true>(I, b, rb, st, Gs, lds, tab);
 }
 
 // every PCG iteration; only instances whose PCG is running
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_qv_pcg(
     const CgInst* __restrict__ insts, const int* __restrict__ tile_inst, const double* __restrict__ ctl)
 {
@@ -479,7 +476,7 @@ __global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void
     const CgInstD& I = cg_inst(insts, b);
     if (I.scal[PCG_STATUS] != 0.0) return;
     __shared__ double lds[CG_WAVES][CG_CMAX * 64];
-    cg_qv_strip<false>(I, b, rb, st, nullptr, lds, tab);
+    cg_qv_strip<XY, false>(I, b, rb, st, nullptr, lds, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,12 +508,11 @@ __global__ __launch_bounds__(CG_WG) void k_plan(const CgInst* __restrict__ insts
         const int i = i0 + r;
         if (i >= n) break;
         const double ai = dual[i];
-        const auto Mi = I.M + (size_t)i * m;
         auto Pi = I.plan + (size_t)i * m;
 #pragma unroll
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
-            if (k < I.c && j < m) Pi[j] = cg_texp((ai + bj[k] - Mi[j]) * I.inv_reg, tab);
+            if (k < I.c && j < m) Pi[j] = cg_texp((ai + bj[k] - cg_cost(I, i, j)) * I.inv_reg, tab);
         }
     }
 }
@@ -559,9 +555,19 @@ __global__ __launch_bounds__(CG_WG) void k_grad_assemble(const CgInst* __restric
         for (int k = 0; k < CG_CMAX; k++) {
             const int j = s0 + lane + 64 * k;
             if (k < I.c && j < m) {
-                const double tv = cg_texp((ai + bj[k] - I.M[row + j]) * I.inv_reg, tab);
+                const double tv = cg_texp((ai + bj[k] - cg_cost(I, i, j)) * I.inv_reg, tab);
                 out[row + j] = (tv * (la + lb[k] - G[row + j])) / I.reg;
             }
         }
     }
 }
+
+// the stored-cost and the geometric-cost instantiations of the M passes
+#define CG_MPASS_INSTANTIATE(XY)                                                                        \
+    template __global__ void k_tpass<XY>(const CgInst*, const int*, const double*, int);               \
+    template __global__ void k_row_lse<XY>(const CgInst*, const int*, const double*, int);             \
+    template __global__ void k_col_lse<XY>(const CgInst*, const int*, const double*, int);             \
+    template __global__ void k_qv_rhs<XY>(const CgInst*, const int*, const double*, const double* const*); \
+    template __global__ void k_qv_pcg<XY>(const CgInst*, const int*, const double*);
+CG_MPASS_INSTANTIATE(false)
+CG_MPASS_INSTANTIATE(true)

@@ -12,7 +12,7 @@
 
 CG_DEV double cg_T(const CgInstD& I, int i, int j, const double* tab)
 {
-    return cg_texp((I.gamma[i] + I.gamma[I.n + j] - I.M[(size_t)i * I.m + j]) * I.inv_reg, tab);
+    return cg_texp((I.gamma[i] + I.gamma[I.n + j] - cg_cost(I, i, j)) * I.inv_reg, tab);
 }
 
 CG_DEV unsigned long long cg_kth(const CgInstD& I) { return (unsigned long long)__double_as_longlong(I.scal[CS_KTH_HI]); }
@@ -35,7 +35,7 @@ CG_DEV bool cg_decode_tile2(const CgInst* insts, const int* tile_inst, const dou
 // CG_RPW rows in groups of R with the loads of a group issued first; the
 // strip's beta and the rows' alpha are read once.  T is formed exactly as
 // cg_T forms it, so the selection kernels see the same keys.
-template <int C, bool FULL>
+template <int C, bool FULL, bool XY>
 CG_DEV void cg_hist_tile(const CgInstD& I, int rb, int st, int pass, unsigned int* h, const double* tab)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -62,9 +62,7 @@ CG_DEV void cg_hist_tile(const CgInstD& I, int rb, int st, int pass, unsigned in
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not counted
             ai[r] = I.gamma[i];
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -79,6 +77,7 @@ CG_DEV void cg_hist_tile(const CgInstD& I, int rb, int st, int pass, unsigned in
     }
 }
 
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts,
                                                 const int* __restrict__ tile_inst,
                                                 const double* __restrict__ ctl, int pass)
@@ -90,9 +89,9 @@ __global__ __launch_bounds__(CG_WG) void k_hist(const CgInst* __restrict__ insts
     __shared__ unsigned int h[CG_HBINS];
     for (int q = threadIdx.x; q < CG_HBINS; q += CG_WG) h[q] = 0u;
     __syncthreads();
-    cg_strip_dispatch(I, st, I.m - 1, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, I.m - 1, [&](auto tag) {
         using S = decltype(tag);
-        cg_hist_tile<S::C, S::FULL>(I, rb, st, pass, h, tab);
+        cg_hist_tile<S::C, S::FULL, S::XY>(I, rb, st, pass, h, tab);
     });
     __syncthreads();
     const int nb = (pass == 0) ? 4096 : CG_HBINS;
@@ -169,13 +168,18 @@ CG_DEV void cg_row_keys(const CgInstD& I, int i, int j0, int mm, double ai, cons
     const auto Mi = I.M + (size_t)i * I.m;
     const auto gb = I.gamma + I.n;
     double mv[U], bv[U];
+    const bool xy = I.X != nullptr;
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int j = j0 + lane + 64 * u;
         ok[u] = FULL || j < mm;
         const int jc = ok[u] ? j : 0;
-        mv[u] = Mi[jc];
+        if (!xy) mv[u] = Mi[jc];
         bv[u] = gb[jc];
+    }
+    if (xy) {
+#pragma unroll
+        for (int u = 0; u < U; u++) mv[u] = cg_cost_xy(I, i, ok[u] ? j0 + lane + 64 * u : 0);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) key[u] = cg_key(cg_texp((ai + bv[u] - mv[u]) * I.inv_reg, tab));
@@ -330,7 +334,7 @@ CG_DEV bool cg_selected(unsigned long long key, unsigned long long kth, int j, i
 // set when (i0 + r, s0 + lane + 64k) is selected.  Rows in groups of R with
 // the loads of a group issued first; beta, alpha and the rows' tie cuts are
 // read once.  T is formed exactly as cg_T forms it.
-template <int C, bool FULL>
+template <int C, bool FULL, bool XY>
 CG_DEV void cg_sel_bits(const CgInstD& I, int rb, int st, const double* tab, unsigned int (&bits)[C])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -359,9 +363,7 @@ CG_DEV void cg_sel_bits(const CgInstD& I, int rb, int st, const double* tab, uns
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not taken
             ai[r] = I.gamma[i];
             cut[r] = I.rcut[i];
-            const auto Mi = I.M + (size_t)i * m + s0 + lane;
-#pragma unroll
-            for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? Mi[64 * k] : 0.0;
+            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -376,6 +378,7 @@ CG_DEV void cg_sel_bits(const CgInstD& I, int rb, int st, const double* tab, uns
 }
 
 // Tile pass: selected count per (row block, column).
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restrict__ insts,
                                                           const int* __restrict__ tile_inst,
                                                           const double* __restrict__ ctl)
@@ -387,10 +390,10 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_count(const CgInst* __restri
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int s0 = st * I.sw, mm = I.m - 1;
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
-    cg_strip_dispatch(I, st, mm, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, mm, [&](auto tag) {
         using S = decltype(tag);
         unsigned int bits[S::C];
-        cg_sel_bits<S::C, S::FULL>(I, rb, st, tab, bits);
+        cg_sel_bits<S::C, S::FULL, S::XY>(I, rb, st, tab, bits);
 #pragma unroll
         for (int k = 0; k < S::C; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
     });
@@ -459,6 +462,7 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_scan(const CgInst* __restric
 }
 
 // Tile pass: write the row indices of the CSC in ascending row order.
+template <bool XY>
 __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restrict__ insts,
                                                           const int* __restrict__ tile_inst,
                                                           const double* __restrict__ ctl)
@@ -471,10 +475,10 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
     const int s0 = st * I.sw, mm = I.m - 1;
     const int i0 = rb * CG_RB + w * CG_RPW;
     __shared__ int lds[CG_WAVES][CG_CMAX * 64];
-    cg_strip_dispatch(I, st, mm, [&](auto tag) {
+    cg_strip_dispatch<XY>(I, st, mm, [&](auto tag) {
         using S = decltype(tag);
         unsigned int bits[S::C];
-        cg_sel_bits<S::C, S::FULL>(I, rb, st, tab, bits);
+        cg_sel_bits<S::C, S::FULL, S::XY>(I, rb, st, tab, bits);
 #pragma unroll
         for (int k = 0; k < S::C; k++) lds[w][lane + 64 * k] = __popc(bits[k]);
         __syncthreads();
@@ -557,3 +561,11 @@ __global__ __launch_bounds__(CG_WG) void k_ctx_csr(const CgInst* __restrict__ fw
 {
     cg_hsl_csr_entry(cg_inst(fwd, 0), blockIdx.x * CG_WG + threadIdx.x, vals, cols, rowptr);
 }
+
+// the stored-cost and the geometric-cost instantiations of the selection passes
+#define CG_SEL_INSTANTIATE(XY)                                                              \
+    template __global__ void k_hist<XY>(const CgInst*, const int*, const double*, int);    \
+    template __global__ void k_sel_cols_count<XY>(const CgInst*, const int*, const double*); \
+    template __global__ void k_sel_cols_write<XY>(const CgInst*, const int*, const double*);
+CG_SEL_INSTANTIATE(false)
+CG_SEL_INSTANTIATE(true)

@@ -31,9 +31,25 @@
 extern "C" {
 #endif
 
+/* Geometric cost in place of a stored M: with X != NULL the instance's M is
+ * never read (it may be NULL) and every kernel forms
+ *     M_ij = scale * sum_{k<d} (X[i*d+k] - Y[j*d+k])^2
+ * on the fly (k in increasing order, each term added by a fused
+ * multiply-add, then one rounded multiply by scale).  No counterpart in the
+ * reference, whose callers form M = |x_i - y_j|^2 / d in torch
+ * (examples/differentiable_splr_ot.py:55, validate_stage5_workload.py:172);
+ * with d a power of two and scale = 1/d the division there is the same
+ * multiply.  X: [n*d], Y: [m*d] row-major device arrays; d >= 1. */
+typedef struct cgot_cost_xy {
+    const double* X;
+    const double* Y;
+    int32_t d;
+    double scale;
+} cgot_cost_xy;
+
 /* One OT instance of a (possibly ragged) SPLR batch. */
 typedef struct cgot_splr_problem {
-    const double* M;        /* [n*m] row-major cost */
+    const double* M;        /* [n*m] row-major cost (NULL when xy.X is set) */
     const double* a;        /* [n] source marginal (> 0) */
     const double* b;        /* [m] target marginal (> 0) */
     int32_t n, m;
@@ -50,6 +66,7 @@ typedef struct cgot_splr_problem {
     double* trace;          /* HOST [8*max_iter] per-iteration record or NULL:
                                f, ||g||, step, low_rank, sinkhorn candidate taken, K,
                                line-search trials, recomputed */
+    cgot_cost_xy xy;        /* geometric cost (xy.X = NULL: M is used) */
 } cgot_splr_problem;
 
 /* Per-instance result and backward-context view (device pointers into the
@@ -103,7 +120,7 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
  * gives the gradients w.r.t. the marginals (no counterpart in the reference,
  * which raises for them): dL/da = lambda[0:n], dL/db = (lambda[n:n+m-1], 0). */
 typedef struct cgot_adjoint_problem {
-    const double* M;        /* [n*m] */
+    const double* M;        /* [n*m] (NULL when xy.X is set) */
     const double* dual;     /* [n+m] converged duals, beta[m-1] = 0 */
     const double* G;        /* [n*m] upstream gradient dL/dT */
     int32_t n, m;
@@ -118,6 +135,7 @@ typedef struct cgot_adjoint_problem {
                                (SPLRBackwardContext::apply_preconditioner,
                                src/backward_context.cu:543-581); takes precedence over inv_diag;
                                NULL for none */
+    cgot_cost_xy xy;        /* geometric cost of the forward solve (xy.X = NULL: M is used) */
 } cgot_adjoint_problem;
 
 typedef struct cgot_adjoint_info {
@@ -200,6 +218,7 @@ typedef struct cgot_bcd_problem {
     const double* x0;       /* [n+m] or NULL */
     double* plan;           /* [n*m] or NULL */
     double* dual;           /* [n+m] or NULL */
+    cgot_cost_xy xy;        /* geometric cost (xy.X = NULL: M is used) */
 } cgot_bcd_problem;
 
 size_t cgot_bcd_workspace_size(int32_t B, const cgot_bcd_problem* probs);

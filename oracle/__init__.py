@@ -97,6 +97,7 @@ def lib():
         L.or_backward.restype = _c_int
         L.or_q_matvec.argtypes = [_dp, _dp, _c_double, _c_int, _c_int, _dp, _dp]
         L.or_rhs_mu.argtypes = [_dp, _dp, _dp, _c_double, _c_int, _c_int, _dp]
+        L.or_cost_xy.argtypes = [_dp, _dp, _c_int, _c_int, _c_int, _c_double, _dp]
         _lib = L
     return _lib
 
@@ -342,3 +343,19 @@ def rhs_mu(M, dual, G, reg):
     out = np.empty(n + m - 1)
     lib().or_rhs_mu(pM, pd, pG, reg, n, m, out.ctypes.data_as(_dp))
     return out
+
+
+def cost_xy(X, Y, scale):
+    """M_ij = scale * sum_k (X_ik - Y_jk)^2 in the library's order (fused
+    multiply-adds in k order, one rounded multiply); mirrors the geometric
+    cost of the kernels (the reference forms M in torch,
+    examples/differentiable_splr_ot.py:55)."""
+    X, pX = _d(X)
+    Y, pY = _d(Y)
+    n, d = X.shape
+    m = Y.shape[0]
+    if Y.shape[1] != d:
+        raise ValueError("X and Y must have the same number of columns")
+    M = np.empty((n, m))
+    lib().or_cost_xy(pX, pY, n, m, d, float(scale), M.ctypes.data_as(_dp))
+    return M

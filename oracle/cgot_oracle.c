@@ -24,6 +24,8 @@
  *                           conditions; invariance under M + u1^T + 1v^T; warm start at optimum
  *   or_backward             dense numpy.linalg.solve(Q, mu) (stage3b_validate.py:431-457);
  *                           zero row/col sums of dL/dM; central finite differences
+ *   or_cost_xy              exact on small-integer points (numpy, exact arithmetic there);
+ *                           an extended-precision numpy sum within a few ulps elsewhere
  */
 #include "cgot_oracle.h"
 
@@ -34,6 +36,30 @@
 
 #define OR_MAX(a, b) ((a) > (b) ? (a) : (b))
 #define OR_MIN(a, b) ((a) < (b) ? (a) : (b))
+
+/* ------------------------------------------------------------------------ */
+/* Geometric cost                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* M_ij = scale * sum_{k<d} (X_ik - Y_jk)^2 (X [n*d], Y [m*d] row-major), the
+ * terms added in k order by fused multiply-adds, then one rounded multiply.
+ * The cost the library forms on the fly in place of a stored M; the
+ * reference's callers form it in torch (examples/differentiable_splr_ot.py:55,
+ * validate_stage5_workload.py:117, 172). */
+void or_cost_xy(const double* X, const double* Y, int n, int m, int d, double scale, double* M)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < m; j++) {
+            double s = 0.0;
+            for (int k = 0; k < d; k++) {
+                const double t = X[(size_t)i * d + k] - Y[(size_t)j * d + k];
+                s = fma(t, t, s);
+            }
+            M[(size_t)i * m + j] = s * scale;
+        }
+    }
+}
 
 /* ------------------------------------------------------------------------ */
 /* BCD / LSE updates                                                         */

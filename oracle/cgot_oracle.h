@@ -123,6 +123,7 @@ int or_backward(const double* M, const double* dual, const double* G,
 /* Matrix-free Q(T) v and mu = [(G.T)1, ((G.T)^T 1)_{:m-1}] (for tests). */
 void or_q_matvec(const double* M, const double* dual, double reg, int n, int m,
                  const double* v, double* out);
+void or_cost_xy(const double* X, const double* Y, int n, int m, int d, double scale, double* M);
 void or_rhs_mu(const double* M, const double* dual, const double* G, double reg,
                int n, int m, double* mu);
 

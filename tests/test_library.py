@@ -51,3 +51,42 @@ def test_serving_footprint_host_only():
     assert splr_footprint(100, 120, backward=True) > small
     # dense Schur block dominates at scale: at least P^2 doubles
     assert big >= 8 * 999 * 999
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """Size and field offsets of every problem/result struct of the C header,
+    as gcc lays them out, against the ctypes mirrors the package passes."""
+    import ctypes
+    import subprocess
+
+    from curegot_backward_amd import _lib
+    structs = {
+        "cgot_cost_xy": (_lib.CostXY, ["X", "Y", "d", "scale"]),
+        "cgot_splr_problem": (_lib.SplrProblem, ["M", "a", "b", "n", "m", "reg", "tol", "max_iter", "density_max",
+                                                 "shift_max", "sparsity_pattern_cycle", "candidate_sinkhorn_iter",
+                                                 "x0", "plan", "dual", "trace", "xy"]),
+        "cgot_splr_info": (_lib.SplrInfo, ["niter", "converged", "valid", "low_rank", "fallback", "n", "m", "ys",
+                                           "f", "gnorm", "K", "inv_diag", "s", "y", "u", "gamma", "factor"]),
+        "cgot_adjoint_problem": (_lib.AdjointProblem, ["M", "dual", "G", "n", "m", "reg", "inv_diag", "tol",
+                                                       "max_iter", "lambda", "grad", "factor", "xy"]),
+        "cgot_adjoint_info": (_lib.AdjointInfo, ["iterations", "converged", "status"]),
+        "cgot_bcd_problem": (_lib.BcdProblem, ["M", "a", "b", "n", "m", "reg", "tol", "max_iter", "x0", "plan",
+                                               "dual", "xy"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{os.path.join(ROOT, "include", "curegot_amd.h")}"',
+             'int main(void) {']
+    for cname, (_, fields) in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                           check=True).stdout.split("\n") if l)
+    for cname, (cls, fields) in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, cf in zip(fields, [x[0] for x in cls._fields_]):
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, cf).offset, (cname, f, cf)
