@@ -155,19 +155,55 @@ CG_DEV double cg_cost(const CgInstD& I, int i, int j)
     return I.X ? cg_cost_xy(I, i, j) : I.M[(size_t)i * I.m + j];
 }
 
-// Row i of the cost at the lane's strip columns j = s0 + lane + 64 k (0 where
-// !ok[k]): loads of the stored M, or the geometric formula (XY).
-template <int C, bool XY>
+// Row i of a stored cost at the lane's strip columns j = s0 + lane + 64 k (0
+// where !ok[k]).
+template <int C>
 CG_DEV void cg_cost_row(const CgInstD& I, int i, int s0, int lane, const bool (&ok)[C], double (&mv)[C])
 {
-    if constexpr (XY) {
+    const auto Mi = I.M + (size_t)i * I.m + s0 + lane;
 #pragma unroll
-        for (int k = 0; k < C; k++) mv[k] = ok[k] ? cg_cost_xy(I, i, s0 + lane + 64 * k) : 0.0;
-    } else {
-        const auto Mi = I.M + (size_t)i * I.m + s0 + lane;
+    for (int k = 0; k < C; k++) mv[k] = ok[k] ? Mi[64 * k] : 0.0;
+}
+
+// Rows min(i0 + r, n - 1), r < R, of a geometric cost at the lane's strip
+// columns, with the rounding of cg_cost_xy.  i0 is the same for every lane of
+// the wave (one wave's row group in every M-pass tile), so each X_i element
+// is one wave-wide read and each Y element read serves the R rows.
+template <int R, int C>
+CG_DEV void cg_cost_rows_xy(const CgInstD& I, int i0, int n, int s0, int lane, const bool (&ok)[C],
+                            double (&mv)[R][C])
+{
+#pragma clang fp contract(off)
+    const int iu = __builtin_amdgcn_readfirstlane(i0);
+    const CG_AS1 double* x[R];
 #pragma unroll
-        for (int k = 0; k < C; k++) mv[k] = ok[k] ? Mi[64 * k] : 0.0;
+    for (int r = 0; r < R; r++) x[r] = I.X + (size_t)min(iu + r, n - 1) * I.d;
+    const CG_AS1 double* y[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) y[k] = I.Y + (size_t)(ok[k] ? s0 + lane + 64 * k : 0) * I.d;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < C; k++) mv[r][k] = 0.0;
+#pragma unroll 4
+    for (int q = 0; q < I.d; q++) {
+        double yq[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) yq[k] = y[k][q];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double xq = x[r][q];
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                const double t = xq - yq[k];
+                mv[r][k] = fma(t, t, mv[r][k]);
+            }
+        }
     }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < C; k++) mv[r][k] = ok[k] ? mv[r][k] * I.cscale : 0.0;
 }
 
 // Scalar slots in CgInst::scal

@@ -107,11 +107,12 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
 #pragma unroll 1
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C], ai[R], part[R];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = cg_point(I, mode, t, i);
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -148,7 +149,7 @@ CG_DEV void cg_tpass_tile(const CgInstD& I, int mode, double t, int rb, int st, 
 }
 
 template <bool XY>
-__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_tpass(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(XY ? 4 : 8))) void k_tpass(const CgInst* __restrict__ insts,
                                                  const int* __restrict__ tile_inst,
                                                  const double* __restrict__ ctl, int mode)
 {
@@ -197,10 +198,11 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
         }
         // x = (beta_j - M_ij)/reg, -inf past m; formed again for the sums
         // instead of being held
@@ -236,7 +238,7 @@ CG_DEV void cg_rowlse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 }
 
 template <bool XY>
-__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_row_lse(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(XY ? 4 : 8))) void k_row_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
 {
@@ -282,11 +284,12 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 #pragma unroll
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C], ai[R];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);
             ai[r] = alpha[i] * ir;
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
         }
         // online update without lane branches: e = exp(-|x - mx|) serves
         // both cases (a new maximum rescales the sum by e and adds 1, any
@@ -322,7 +325,7 @@ CG_DEV void cg_collse_tile(const CgInstD& I, int mode, int rb, int st, const dou
 }
 
 template <bool XY>
-__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_col_lse(const CgInst* __restrict__ insts,
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(XY ? 4 : 8))) void k_col_lse(const CgInst* __restrict__ insts,
                                                    const int* __restrict__ tile_inst,
                                                    const double* __restrict__ ctl, int mode)
 {
@@ -377,12 +380,13 @@ CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* co
 #pragma unroll 1
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         double mv[R][C], gv[RHS ? R : 1][C], ai[R], va[R];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = I.gamma[i];
             va[r] = MV ? vat(i) : 0.0;
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
             if (RHS) {
                 const auto Gi = G + (size_t)i * m + s0 + lane;
 #pragma unroll
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(CG_WG) void k_qv_rhs(const CgInst* __restrict__ ins
 
 // every PCG iteration; only instances whose PCG is running
 template <bool XY>
-__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_qv_pcg(
+__global__ __launch_bounds__(CG_WG) __attribute__((amdgpu_waves_per_eu(XY ? 4 : 8))) void k_qv_pcg(
     const CgInst* __restrict__ insts, const int* __restrict__ tile_inst, const double* __restrict__ ctl)
 {
     CG_EXP_TABLE(tab);

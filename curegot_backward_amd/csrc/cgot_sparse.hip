@@ -58,11 +58,12 @@ CG_DEV void cg_hist_tile(const CgInstD& I, int rb, int st, int pass, unsigned in
     for (int r0 = 0; r0 < CG_RPW; r0 += R) {
         if (i0 + r0 >= n) break;
         double mv[R][C], ai[R];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not counted
             ai[r] = I.gamma[i];
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -358,12 +359,13 @@ CG_DEV void cg_sel_bits(const CgInstD& I, int rb, int st, const double* tab, uns
         if (i0 + r0 >= n) break;
         double mv[R][C], ai[R];
         int cut[R];
+        if constexpr (XY) cg_cost_rows_xy<R, C>(I, i0 + r0, n, s0, lane, ok, mv);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not taken
             ai[r] = I.gamma[i];
             cut[r] = I.rcut[i];
-            cg_cost_row<C, XY>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
