@@ -67,11 +67,16 @@ struct InFlight {
     ~InFlight() { g_inflight.fetch_sub(1); }
 };
 
-// Pinned host staging (per thread, grown on demand).
+// Pinned host staging (per thread, grown on demand).  `coherent` buffers
+// (hipHostMallocCoherent) are read and written by kernels directly: device
+// accesses to them go to host memory uncached.
 struct Pinned {
     double* p = nullptr;
     size_t cap = 0;
     int gen = -1;
+    bool coherent = false;
+    Pinned() = default;
+    explicit Pinned(bool coh) : coherent(coh) {}
     double* get(size_t n)
     {
         if (gen != g_res_gen.load()) { p = nullptr; cap = 0; gen = g_res_gen.load(); }
@@ -82,7 +87,8 @@ struct Pinned {
             // host memory is slow).
             const size_t c = std::max<size_t>({n, 2 * cap, 65536});
             double* q = nullptr;
-            CG_HIP(hipHostMalloc((void**)&q, c * sizeof(double), hipHostMallocDefault));
+            CG_HIP(hipHostMalloc((void**)&q, c * sizeof(double),
+                                 coherent ? hipHostMallocCoherent : hipHostMallocDefault));
             {
                 std::lock_guard<std::mutex> lk(g_res_mu);
                 g_res_pinned.push_back(q);
@@ -93,10 +99,18 @@ struct Pinned {
         }
         return p;
     }
+    // the device address of the buffer (after get)
+    void* dev() const
+    {
+        void* d = nullptr;
+        CG_HIP(hipHostGetDevicePointer(&d, p, 0));
+        return d;
+    }
 };
 thread_local Pinned g_ctl_host;
 thread_local Pinned g_scal_host;
 thread_local Pinned g_map_host;
+thread_local Pinned g_sub_host;
 
 // ---------------------------------------------------------------------------
 // Workspace layout: a bump allocator over the caller's bytes.
@@ -243,6 +257,11 @@ struct Batch {
     int pref_cap = 0;
     const double** d_ptrs = nullptr;   // 3*B pointers
     int* d_ids = nullptr;              // B instance ids of a sparse-solve launch
+    // wave map of a subset of the instances (those on the dense Newton path),
+    // so that the per-row Schur kernels dispatch no workgroups for the others
+    int* d_sub = nullptr;
+    int nsub = 0;
+    std::vector<char> sub_key;
     std::vector<char> pc_sparse;       // adjoint: the forward factor is a sparse plan
     std::vector<int> pc_R;             // adjoint: its core size
     unsigned int* zero_base = nullptr; // counters + histograms of all instances
@@ -255,8 +274,12 @@ struct Batch {
     // entry point that allocates no GPU or pinned memory).  The ring of
     // control slots and the synchronisations are the same either way.
     bool pageable = false;
-    std::vector<int> pg_maps;
+    std::vector<int> pg_maps, pg_sub;
     bool xy = false;           // every instance has the geometric cost (else none has)
+    // the value refresh runs on the wave map (64x the threads of the vector
+    // map) when some instance's pattern may hold more than 32 items per
+    // vector-map thread (N + 2 Kmax items)
+    bool refresh_rows = false;
     std::vector<double> pg_ctl, pg_scal;
     std::map<const double*, const double*> hmap;   // device ctl slot -> host copy
 
@@ -272,6 +295,7 @@ struct Batch {
         d_pref = bp.take<int>(pref_cap);
         d_ptrs = bp.take<const double*>(3 * (size_t)B);
         d_ids = bp.take<int>(B);
+        d_sub = bp.take<int>(2 * (size_t)std::max(nrow, 1));
     }
 
     void assign_maps()
@@ -339,6 +363,31 @@ struct Batch {
         int nt, nv, nr;
         write_maps(d_live, live, nt, nv, nr);
         set_maps(d_live, nt, nv, nr);
+    }
+
+    // Make d_sub the wave map of the instances with on[b] (rewritten only when
+    // the set changes; the pinned staging buffer is reused, so call at most
+    // once between two stream synchronisations).
+    void sub_rows(const std::vector<char>& on)
+    {
+        if (on == sub_key) return;
+        sub_key = on;
+        int cnt = 0;
+        for (int b = 0; b < B; b++) if (on[b]) cnt += h[b].nrow;
+        nsub = cnt;
+        if (!cnt) return;
+        if (pageable) pg_sub.resize(2 * (size_t)cnt);
+        int* hm = pageable ? pg_sub.data() : (int*)g_sub_host.get((size_t)cnt + 1);
+        int* pr = hm;
+        for (int b = 0; b < B; b++) {
+            if (!on[b]) continue;
+            for (int t = 0; t < h[b].nrow; t++) { *pr++ = b; *pr++ = t; }
+        }
+        CG_HIP(hipMemcpyAsync(d_sub, hm, sizeof(int) * 2 * (size_t)cnt, hipMemcpyHostToDevice, stream));
+    }
+    void rows_sub(void (*k)(const CgInst*, const int*, const double*), const double* c, size_t lds = 0)
+    {
+        if (nsub) hipLaunchKernelGGL(k, dim3(nsub), dim3(CG_WG), lds, stream, d_insts, d_sub, c);
     }
 
     void upload_maps(const std::vector<const double*>& ptrs)
@@ -696,6 +745,9 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
         I.dual = p.dual;
     }
     bt.assign_maps();
+    bt.refresh_rows = false;
+    for (const CgInst& I : bt.h)
+        if ((double)I.N + 2.0 * (double)I.Kmax > 32.0 * CG_WG * (double)I.nvec) bt.refresh_rows = true;
     Bump bp(base);
     bt.layout_common(bp, nblk_max);
     // scratch that must start at zero, contiguous for one clear
@@ -750,45 +802,46 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
     return bp.off + 256;
 }
 
-thread_local Pinned g_pat_host;    // W patterns of new sparsity patterns (ints)
-thread_local Pinned g_plan_host;   // sparse plans and headers on their way to the device
+static_assert(sizeof(SpHdr) <= SP_HDR_PAD, "plan header larger than its staging slot");
+thread_local Pinned g_pat_host(true);    // W patterns of new sparsity patterns (ints), written by k_pat_export
+thread_local Pinned g_plan_host(true);   // sparse plans and headers, read by k_plan_import
 
 struct SplrSolver {
     Batch& bt;
     std::vector<HostInst> hi;
     hipEvent_t pat_ev = nullptr;   // the pattern copies to g_pat_host are complete
 
+
     SplrSolver(Batch& b) : bt(b) {}
     ~SplrSolver()
     {
         if (pat_ev) (void)hipEventDestroy(pat_ev);
+
     }
 
-    // After a new top-K pattern (instances in newpat): copy W's row pattern
-    // (rowptr [n+1], colj [K]) to pinned host memory; the analysis waits for
-    // the copies just before the next Newton step, so the launches queued in
-    // between (the Sinkhorn candidate) run while the host analyses.
+    // After a new top-K pattern (instances in newpat): W's row pattern
+    // (rowptr [n+1], colj [K]) goes to pinned host memory (one export launch);
+    // the analysis waits for it just before the next Newton step, so the
+    // launches queued in between (the Sinkhorn candidate) run while the host
+    // analyses.
     void request_patterns(const std::vector<char>& newpat)
     {
         const int B = bt.B;
         size_t total = 0;
+        auto c = ctl_vec(B);
         for (int b = 0; b < B; b++)
             if (newpat[b]) {
                 hi[b].pat_off = total;
                 total += (size_t)hi[b].n + 1 + (size_t)hi[b].K;
+                c[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+                c[(size_t)b * CG_NCTL + CTL_AUX] = (double)hi[b].pat_off;
+                c[(size_t)b * CG_NCTL + CTL_KSEL] = (double)hi[b].K;
+                hi[b].pend = true;
             }
         if (total == 0) return;
-        int* hp = (int*)g_pat_host.get((total + 1) / 2 + 1);
-        for (int b = 0; b < B; b++) {
-            if (!newpat[b]) continue;
-            HostInst& H = hi[b];
-            CG_HIP(hipMemcpyAsync(hp + H.pat_off, bt.h[b].w_rowptr, sizeof(int) * (H.n + 1), hipMemcpyDeviceToHost,
-                                  bt.stream));
-            if (H.K > 0)
-                CG_HIP(hipMemcpyAsync(hp + H.pat_off + H.n + 1, bt.h[b].w_colj, sizeof(int) * H.K,
-                                      hipMemcpyDeviceToHost, bt.stream));
-            H.pend = true;
-        }
+        g_pat_host.get((total + 1) / 2 + 1);
+        hipLaunchKernelGGL(k_pat_export, dim3(B), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.ctl(c),
+                           (int*)g_pat_host.dev());
         if (!pat_ev) CG_HIP(hipEventCreateWithFlags(&pat_ev, hipEventDisableTiming));
         CG_HIP(hipEventRecord(pat_ev, bt.stream));
     }
@@ -825,30 +878,30 @@ struct SplrSolver {
                 if (ok[k] && sp_cost_us(hdr[k]) > sp_dense_cost_us(hi[todo[k]].nblk)) ok[k] = 0;
         for (size_t k = 0; k < todo.size(); k++)
             if (!ok[k]) { hdr[k] = SpHdr{}; img[k].clear(); }
-        // previous uploads from g_plan_host have completed: every iteration
+        // previous imports from g_plan_host have completed: every iteration
         // synchronises the stream (line-search read-backs) before the next pattern
         size_t total = 0;
         std::vector<size_t> off(todo.size());
         for (size_t k = 0; k < todo.size(); k++) {
             off[k] = total;
-            total += (sizeof(SpHdr) + img[k].size() + 255) & ~(size_t)255;
+            total += (SP_HDR_PAD + img[k].size() + 255) & ~(size_t)255;
         }
         char* up = (char*)g_plan_host.get(total / sizeof(double) + 1);
+        auto c = ctl_vec(B);
         for (size_t k = 0; k < todo.size(); k++) {
             const int b = todo[k];
             char* dst = up + off[k];
             std::memcpy(dst, &hdr[k], sizeof(SpHdr));
-            CG_HIP(hipMemcpyAsync(bt.h[b].sph, dst, sizeof(SpHdr), hipMemcpyHostToDevice, bt.stream));
-            if (ok[k]) {
-                std::memcpy(dst + sizeof(SpHdr), img[k].data(), img[k].size());
-                CG_HIP(hipMemcpyAsync(bt.h[b].S, dst + sizeof(SpHdr), img[k].size(), hipMemcpyHostToDevice,
-                                      bt.stream));
-            }
+            if (ok[k]) std::memcpy(dst + SP_HDR_PAD, img[k].data(), img[k].size());
+            c[(size_t)b * CG_NCTL + CTL_ACTIVE] = 1.0;
+            c[(size_t)b * CG_NCTL + CTL_AUX] = (double)off[k];
             hi[b].sparse = ok[k] != 0;
             hi[b].spR = ok[k] ? hdr[k].R : 0;
             hi[b].spH = ok[k] ? hdr[k].H : 0;
             hi[b].pend = false;
         }
+        hipLaunchKernelGGL(k_plan_import, dim3(B * SP_IMPORT_WGS), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.ctl(c),
+                           (const char*)g_plan_host.dev());
     }
 
     std::vector<double> mask(const std::vector<char>& on)
@@ -918,7 +971,16 @@ struct SplrSolver {
             const double shift = std::min(hi[b].gnorm, hi[b].shift_max);
             r[(size_t)b * CG_NCTL + CTL_SHIFT] = shift * hi[b].reg;
         }
-        if (any) bt.rows0(k_refresh_values, bt.ctl(r));
+        if (any) {
+            const double* dr = bt.ctl(r);
+            if (bt.refresh_rows) {
+                if (bt.nrow) hipLaunchKernelGGL(k_refresh_values, dim3(bt.nrow), dim3(CG_WG), 0, bt.stream, bt.d_insts,
+                                                bt.d_row, dr, 1);
+            } else if (bt.nvec) {
+                hipLaunchKernelGGL(k_refresh_values, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec,
+                                   dr, 0);
+            }
+        }
         request_patterns(newpat);
     }
 
@@ -1013,8 +1075,9 @@ struct SplrSolver {
         int pcap = 1;
         for (int b = 0; b < B; b++)
             if (on[b] && bt.h[b].P <= CG_SCHUR_LDS) pcap = std::max(pcap, bt.h[b].P);
-        bt.rows0(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
-        bt.rows0(k_schur_rhs, dc);
+        bt.sub_rows(on);
+        bt.rows_sub(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
+        bt.rows_sub(k_schur_rhs, dc);
         if (nblk_max > 0 && total(1 + 4 * nblk_max) > 0)
             hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 4 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
                                dtab(1 + 4 * nblk_max));
@@ -1049,7 +1112,7 @@ struct SplrSolver {
             hipLaunchKernelGGL(k_trsv_flow, dim3(ntrsv), dim3(CG_WG), 0, bt.stream, bt.d_insts,
                                dtab(2 + 4 * nblk_max), dtab0 + lvl_off, nblk_max);
         }
-        bt.rows0(k_schur_back, dc);
+        bt.rows_sub(k_schur_back, dc);
     }
 
     void woodbury(const double* dc)
@@ -1346,6 +1409,9 @@ size_t adjoint_layout(Batch& bt, const cgot_adjoint_problem* probs, void* base, 
         }
     }
     bt.assign_maps();
+    bt.refresh_rows = false;
+    for (const CgInst& I : bt.h)
+        if ((double)I.N + 2.0 * (double)I.Kmax > 32.0 * CG_WG * (double)I.nvec) bt.refresh_rows = true;
     Bump bp(base);
     bt.layout_common(bp, nblk_max);
     const size_t zstride = 4 + (size_t)nblk_max;
@@ -1825,20 +1891,19 @@ int sp_rmax()
     return r;
 }
 
-// CUs kept free of size classes 1.. when a batch is split: the largest
-// class, whose factorization chain (one small launch per block step) sets the
-// batch time, then finds CUs for its step launches without waiting behind the
-// other classes' wide launches.  5/16 of the CUs (80 of 256: ragged B = 128
-// batch 117 -> 110 ms; 0, 16, 32, 64, 80, 96, 112, 128 reserved gave 117,
-// 115.5, 114, 112, 110, 110.4, 119, 121 ms); CGOT_CU_RESERVE=<count>
-// overrides it, 0 turns it off.
-int cu_reserve(int ncu)
+// CUs kept free of size classes 1.. when a batch is split (CU-masked streams
+// for classes 1..; class 0 on a stream of its own).  It paid while the
+// largest class's dense factorization chain set the batch time; with the
+// sparse Newton solve the smallest class (most iterations) sets it, and on
+// the ragged B = 128 batch no reserve is fastest (63.4 ms against 66.4 ms
+// with 80 CUs kept for class 0).  Default 0; CGOT_CU_RESERVE=<count> sets one.
+int cu_reserve()
 {
     static const int env = [] {
         const char* e = std::getenv("CGOT_CU_RESERVE");
         return e ? std::max(0, std::atoi(e)) : -1;
     }();
-    return (env >= 0) ? env : ncu * 5 / 16;
+    return (env >= 0) ? env : 0;
 }
 
 // Streams of size classes 0..kMaxGroups-1 for batches issued from the calling
@@ -1859,7 +1924,7 @@ hipStream_t group_stream(int device, int k)
         std::array<hipStream_t, kMaxGroups> a{};
         int ncu = 0;
         CG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-        const int res = cu_reserve(ncu);
+        const int res = cu_reserve();
         if (res > 0 && res < ncu) {
             CG_HIP(hipStreamCreateWithFlags(&a[0], hipStreamNonBlocking));
             std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
@@ -1948,7 +2013,11 @@ int64_t cgot_sparse_plan(int32_t n, int32_t m, const int32_t* rowptr, const int3
         std::vector<char> plan;
         if (!sp_analyze(n, m, rowptr, colj, cap_bytes, rmax, h, plan)) return -1;
         std::memcpy(hdr_out, &h, sizeof(SpHdr));
-        if (plan_out && plan_cap >= h.bytes) std::memcpy(plan_out, plan.data(), (size_t)h.bytes);
+        if (plan_out && plan_cap >= h.bytes) {
+            // the integer part; the value arrays are the device's (zero here)
+            std::memcpy(plan_out, plan.data(), plan.size());
+            std::memset((char*)plan_out + plan.size(), 0, (size_t)h.bytes - plan.size());
+        }
         return h.bytes;
     } catch (const std::exception& e) {
         g_last_error = e.what();

@@ -64,7 +64,7 @@ __global__ void k_sel_cols_prefix(const CgInst*, const int*, const double*);    
 __global__ void k_sel_cols_scan(const CgInst*, const double*);                    // grid = B
 template <bool XY>
 __global__ void k_sel_cols_write(const CgInst*, const int*, const double*);
-__global__ void k_refresh_values(const CgInst*, const int*, const double*);       // wave grid
+__global__ void k_refresh_values(const CgInst*, const int*, const double*, int wave_map);  // vector or wave grid
 
 // ---- Newton solve on the Schur complement ----
 __global__ void k_schur_build(const CgInst*, const int*, const double*);          // wave grid
@@ -87,6 +87,12 @@ __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, i
 // dynamic LDS sp_lds_bytes(R) of the largest core)
 __global__ void k_sp_newton(const CgInst*, const int* ids);
 __global__ void k_sp_precond(const CgInst*, const int* ids);
+// pattern export / plan import through pinned host memory (grid = B, and
+// B * SP_IMPORT_WGS)
+#define SP_IMPORT_WGS 8
+#define SP_HDR_PAD 512
+__global__ void k_pat_export(const CgInst*, const double* ctl, int* dst);
+__global__ void k_plan_import(const CgInst*, const double* ctl, const char* src);
 __global__ void k_trsv_flow(const CgInst*, const int* order, const int* lvl_pref, int nlvl);
 
 // ---- backward PCG (vec grid) ----

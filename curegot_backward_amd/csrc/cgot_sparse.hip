@@ -499,28 +499,60 @@ __global__ __launch_bounds__(CG_WG) void k_sel_cols_write(const CgInst* __restri
 }
 
 // Refresh W values on the current pattern and the diagonal blocks
-// Da = T1 + shift, Db = (T^T 1)[:m-1] + shift; one wave per row i (q < n) or
-// column j = q - n.  Reference: recompute_nonzero_values_kernel +
-// write_diagonal_kernel, src/sinkhorn_splr_kernel.cu:554-734.
+// Da = T1 + shift, Db = (T^T 1)[:m-1] + shift.  Reference:
+// recompute_nonzero_values_kernel + write_diagonal_kernel,
+// src/sinkhorn_splr_kernel.cu:554-734.
+//
+// Flat over the instance's items: N diagonal entries, then the K entries of
+// W's row CSR, then the K entries of its column CSC, one thread per item
+// (grid-stride over the instance's workgroups of the launch map: the vector
+// map, ceil((n+m)/CG_WG) workgroups, or for dense patterns the wave map,
+// ceil((n+m)/CG_WAVES); `wave_map` says which).  A row or column holds about
+// K/n entries on average but the top-K concentrates on the heavy rows, so
+// items rather than rows are spread over the lanes; a CSR (CSC) entry finds
+// its row (column) by binary search in the pointer array.
+template <class Ptr>
+CG_DEV int cg_upper_index(Ptr ptr, int count, int p)
+{
+    // largest r in [0, count) with ptr[r] <= p (ptr non-decreasing, ptr[0] = 0)
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ptr[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(CG_WG) void k_refresh_values(const CgInst* __restrict__ insts,
-                                                          const int* __restrict__ wave_inst,
-                                                          const double* __restrict__ ctl)
+                                                          const int* __restrict__ map,
+                                                          const double* __restrict__ ctl, int wave_map)
 {
     CG_EXP_TABLE(tab);
-    const int2 e = cg_map_entry(wave_inst);
+    const int2 e = cg_map_entry(map);
     const int b = e.x;
     if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
     const CgInstD& I = cg_inst(insts, b);
-    const int q = e.y * CG_WAVES + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
     const double sh = ctl[b * CG_NCTL + CTL_SHIFT];
-    if (q < I.n) {
-        for (int p = I.w_rowptr[q] + lane; p < I.w_rowptr[q + 1]; p += 64) I.w_rval[p] = cg_T(I, q, I.w_colj[p], tab);
-        if (lane == 0) I.Da[q] = I.rowsum[q] + sh;
-    } else if (q < I.N) {
-        const int j = q - I.n;
-        for (int p = I.w_colptr[j] + lane; p < I.w_colptr[j + 1]; p += 64) I.w_cval[p] = cg_T(I, I.w_rowi[p], j, tab);
-        if (lane == 0) I.Db[j] = I.colsum[j] + sh;
+    const int n = I.n, mm = I.m - 1, N = I.N;
+    const int K = I.w_rowptr[n];
+    const long long items = (long long)N + 2LL * K;
+    const int stride = (wave_map ? I.nrow : I.nvec) * CG_WG;
+    for (long long x = (long long)e.y * CG_WG + threadIdx.x; x < items; x += stride) {
+        if (x < n) {
+            I.Da[x] = I.rowsum[x] + sh;
+        } else if (x < N) {
+            const int j = (int)x - n;
+            I.Db[j] = I.colsum[j] + sh;
+        } else if (x < (long long)N + K) {
+            const int p = (int)(x - N);
+            const int i = cg_upper_index(I.w_rowptr, n, p);
+            I.w_rval[p] = cg_T(I, i, I.w_colj[p], tab);
+        } else {
+            const int p = (int)(x - N - K);
+            const int j = cg_upper_index(I.w_colptr, mm, p);
+            I.w_cval[p] = cg_T(I, I.w_rowi[p], j, tab);
+        }
     }
 }
 

@@ -816,3 +816,50 @@ __global__ __launch_bounds__(SP_WG) void k_sp_precond(const CgInst* __restrict__
     periphery_bwd<1>(v, I.sv, nullptr, yx);
     if (acc == 0x9e3779b9u && v.N < 0) yx[0] = 0.0;   // never true: keeps the prefetch loads
 }
+
+// ---------------------------------------------------------------------------
+// Pattern and plan transfers through pinned (coherent) host memory, one
+// launch for all instances of a lockstep group instead of one or two copies
+// per instance.
+// ---------------------------------------------------------------------------
+
+// W's row pattern of every instance flagged in ctl (rowptr [n + 1], then colj
+// [K] with K = ctl[CTL_KSEL]) into host memory at int offset ctl[CTL_AUX];
+// one workgroup per instance (grid = B).
+__global__ __launch_bounds__(CG_WG) void k_pat_export(const CgInst* __restrict__ insts,
+                                                      const double* __restrict__ ctl, int* __restrict__ dst)
+{
+    const int b = blockIdx.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInstD& I = cg_inst(insts, b);
+    int* d = dst + (long long)ctl[b * CG_NCTL + CTL_AUX];
+    const int n = I.n, K = (int)ctl[b * CG_NCTL + CTL_KSEL];
+    for (int x = threadIdx.x; x <= n; x += CG_WG) d[x] = I.w_rowptr[x];
+    for (int x = threadIdx.x; x < K; x += CG_WG) d[n + 1 + x] = I.w_colj[x];
+}
+
+// Sparse plans of the instances flagged in ctl, from host memory at byte
+// offset ctl[CTL_AUX]: the header (SP_HDR_PAD bytes reserved) into sph, then
+// the plan's integer part [0, o_meta_end) into the instance's Schur buffer,
+// whose value part [o_meta_end, bytes) is cleared.  A header with mode 0
+// (the instance keeps the dense path) is copied alone.  SP_IMPORT_WGS
+// workgroups per instance (grid = B * SP_IMPORT_WGS), 8-byte words.
+__global__ __launch_bounds__(CG_WG) void k_plan_import(const CgInst* __restrict__ insts,
+                                                       const double* __restrict__ ctl,
+                                                       const char* __restrict__ src)
+{
+    const int b = blockIdx.x / SP_IMPORT_WGS, g = blockIdx.x % SP_IMPORT_WGS;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const char* s = src + (long long)ctl[b * CG_NCTL + CTL_AUX];
+    const int* hs = reinterpret_cast<const int*>(s);
+    if (g == 0)
+        for (int x = threadIdx.x; x < (int)(sizeof(SpHdr) / 4); x += CG_WG) reinterpret_cast<CG_AS1 int*>(I.sph)[x] = hs[x];
+    const SpHdr* hh = reinterpret_cast<const SpHdr*>(s);
+    if (hh->mode == 0) return;
+    const long long meta8 = hh->o_meta_end / 8, all8 = hh->bytes / 8;
+    const long long* s8 = reinterpret_cast<const long long*>(s + SP_HDR_PAD);
+    CG_AS1 long long* d8 = reinterpret_cast<CG_AS1 long long*>(I.S);
+    const long long step = (long long)SP_IMPORT_WGS * CG_WG;
+    for (long long x = (long long)g * CG_WG + threadIdx.x; x < all8; x += step) d8[x] = (x < meta8) ? s8[x] : 0LL;
+}

@@ -269,8 +269,9 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
     h.o_yx = L.take(16LL * N);
     h.bytes = L.take(0, 256);
     if (h.bytes > cap_bytes || npairs >= ((int64_t)1 << 31) || ncpairs >= ((int64_t)1 << 31)) return false;
-    // ---- fill the image (the value arrays lval, lcore and yx are left for the device) ----
-    plan.assign((size_t)h.bytes, 0);
+    // ---- fill the image: its integer part [0, o_meta_end); the value arrays
+    // lval, lcore and yx after it are the device's ----
+    plan.assign((size_t)h.o_meta_end, 0);
     char* base = plan.data();
     auto I32 = [&](int64_t off) { return reinterpret_cast<int*>(base + off); };
     std::memcpy(I32(h.o_perm), order.data(), 4LL * N);

@@ -31,6 +31,7 @@
 #include <stdint.h>
 
 #ifdef __cplusplus
+#include <cstddef>
 #include <vector>
 #endif
 
@@ -109,8 +110,9 @@ inline double sp_cost_us(const SpHdr& h)
 inline double sp_dense_cost_us(int nblk) { return 150.0 + 32.0 * nblk; }
 
 #ifdef __cplusplus
-// Host analysis.  Returns true and fills `plan` (a byte image whose offsets
-// are relative to its start; hdr.bytes long) when a sparse plan fits in
+// Host analysis.  Returns true and fills `plan` (the integer part of the
+// plan image, hdr.o_meta_end bytes, offsets relative to its start; the value
+// arrays up to hdr.bytes follow on the device) when a sparse plan fits in
 // `cap_bytes` with a core of at most `rmax` (<= SP_RMAX); false otherwise
 // (then the instance keeps the dense Schur path).  rowptr/colj: W's pattern
 // by row (columns j < m - 1).
