@@ -7,9 +7,13 @@ FP64, tol 1e-8, default density.  A step is one batched SPLR solve of the whole
 batch (forward, plans written); inputs are resident in HBM before timing.
 
 Also measured: the same instances solved one at a time with the plain entry
-(per-solve latency p50/p99), the T-pass kernel's achieved HBM bandwidth (HIP
-events around each launch in one extra, single-group solve), and the CPU oracle
-on a bounded sample.
+(per-solve latency p50/p99); per-kernel times in one extra, single-group solve
+of the batch (HIP events around each launch), which give the roofline figures
+of the dominant kernel and of the HBM-bound M passes; the serving leg (the head
+of the mixed-size request stream, n, m up to 8192, through the
+continuous-batching server, beside one-at-a-time solves of the same
+requests); and the CPU oracle on a size-stratified sample of the batch, on all
+host cores and on one.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 """
@@ -36,28 +40,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problems, budget_s):
-    """CPU oracle (FP64 C restatement, OpenMP) on instances in batch order, until
-    the time budget is spent.  Only small instances (max(n, m) <= 1024) are
-    sampled so one solve stays within the budget."""
+def _omp_threads(k):
+    """Set the OpenMP thread count of the oracle's parallel regions (libgomp,
+    the runtime liboracle.so links)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(k))
+
+
+def cpu_baseline(problems, budget_s, strata=4, threads=None, max_stratum=None):
+    """CPU oracle (FP64 C restatement, OpenMP) on a size-stratified sample of
+    the batch: the instances are split into `strata` groups of equal count by
+    n*m; within each group (up to max_stratum) instances are solved in batch
+    order until the group's share of the time budget is spent (at least one
+    each).  The batch's CPU time is estimated as sum over groups of (group
+    size x mean solve time in the group); value = B / that time."""
     import oracle
     oracle.lib()
-    t0 = time.perf_counter()
-    done, cells = 0, 0
-    for (M, a, b, eta) in problems:
-        n, m = M.shape
-        if max(n, m) > 1024:
-            continue
-        oracle.splr_bwd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-8, max_iter=1000)
-        done += 1
-        cells += n * m
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-    return {"value": done / el, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"first {done} instances with max(n,m)<=1024 of the same batch "
-                      f"({cells / 1e6:.1f}M cells, {el:.1f} s)"}
+    cores = int(threads or os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    _omp_threads(cores)
+    order = sorted(range(len(problems)), key=lambda i: problems[i][0].numel())
+    groups = [sorted(order[k * len(order) // strata:(k + 1) * len(order) // strata]) for k in range(strata)]
+    if max_stratum is not None:
+        groups = groups[:max_stratum]
+    est, done, cells, t_all = 0.0, 0, 0, time.perf_counter()
+    per = []
+    for g in groups:
+        t0 = time.perf_counter()
+        k = 0
+        for i in g:
+            M, a, b, eta = problems[i]
+            oracle.splr_bwd(M.numpy(), a.numpy(), b.numpy(), eta, tol=1e-8, max_iter=1000)
+            k += 1
+            cells += M.numel()
+            if time.perf_counter() - t0 > budget_s / len(groups):
+                break
+        el = time.perf_counter() - t0
+        est += len(g) * el / k
+        done += k
+        per.append(f"{k}/{len(g)}")
+    n_inst = sum(len(g) for g in groups)
+    return {"value": n_inst / est, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"size-stratified sample of the same batch: {len(groups)} groups of equal count by n*m "
+                      f"(solved/size {', '.join(per)}), {done} instances, {cells / 1e6:.1f}M cells, "
+                      f"{time.perf_counter() - t_all:.1f} s; value = {n_inst} / estimated CPU time of those "
+                      f"{n_inst} instances"}
 
 
 def main():
@@ -80,7 +106,10 @@ def main():
     ap.add_argument("--fd-sweeps", type=int, default=2000, help="BCD polish sweeps per FD solve (--adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB of one device-to-device copy issued first (counter calibration under rocprofv3)")
-    ap.add_argument("--serve-n", type=int, default=300, help="requests replayed from the serving trace")
+    ap.add_argument("--serve-n", type=int, default=1000,
+                    help="requests replayed from the serving trace (default run: the serving leg; 0 skips it)")
+    ap.add_argument("--serve-seq", type=int, default=300,
+                    help="requests of the stream also solved one at a time (--serve)")
     ap.add_argument("--rate", type=float, default=2000.0, help="arrival rate of the serving trace (req/s)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=64)
@@ -169,12 +198,19 @@ def main():
     roof_tpass = hbm_roof("tpass", "k_tpass (T pass / line-search trial over M)")
     roof_row = hbm_roof("row_lse", "k_row_lse (Sinkhorn candidate alpha half-sweep)")
     roof_col = hbm_roof("col_lse", "k_col_lse (Sinkhorn candidate beta half-sweep)")
-    ms_u, fl_u, n_u = prof["chol_update"]
-    ach_u = (fl_u / 1e12) / (ms_u / 1e3) if ms_u > 0 else 0.0
-    roof_chol = {"kernel": "k_chol_update (trailing update, diagonal-block factor, next panel of the Schur Cholesky)",
-                 "bound": "mfma", "achieved": ach_u, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                 "frac": ach_u / FP64_MFMA_PEAK_TFS, "traffic": None, "avg_launch_us": 1000 * ms_u / max(n_u, 1),
-                 "launches": n_u, "flop_per_launch": fl_u / max(n_u, 1)}
+    def mfma_roof(name, kernel, note):
+        ms, fl, n = prof[name]
+        ach = (fl / 1e12) / (ms / 1e3) if ms > 0 else 0.0
+        return {"kernel": kernel, "bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": None, "avg_launch_us": 1000 * ms / max(n, 1),
+                "launches": n, "flop_per_launch": fl / max(n, 1), "note": note}
+
+    roof_sp = mfma_roof("sp_newton", "k_sp_newton (sparse factorization and two-RHS solve of the Newton system, "
+                        "one workgroup per instance)",
+                        "latency-bound: a chain of elimination-tree levels and dense-core columns per instance; "
+                        "flops = periphery pair FMAs, core Schur gathers, R^3/3 core elimination, substitutions")
+    roof_chol = mfma_roof("chol_update", "k_chol_update (trailing update, diagonal-block factor, next panel of the "
+                          "dense Schur Cholesky)", "64x64 blocks on FP64 MFMA; 2*64^3 flops per trailing tile")
     # HBM bytes per launch measured with rocprofv3 counter passes over this
     # workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size);
     # null when the measurement file is absent
@@ -184,12 +220,16 @@ def main():
     except (OSError, ValueError, KeyError):
         measured = {}
     for roof, kname in ((roof_tpass, "k_tpass"), (roof_chol, "k_chol_update"), (roof_row, "k_row_lse"),
-                        (roof_col, "k_col_lse")):
+                        (roof_col, "k_col_lse"), (roof_sp, "k_sp_newton")):
         if kname in measured and args.batch == 128 and args.seed == 0:
             roof["traffic"] = measured[kname]["traffic_bytes_per_launch"]
             roof["traffic_source"] = "profiles/pmc_traffic_splr_ragged_b128.md"
-    # the dominant kernel by device time in the profiled solve
-    dominant = roof_chol if ms_u >= prof["tpass"][0] else roof_tpass
+    # the dominant kernel: the most device time in the profiled solve
+    cand = [(prof["sp_newton"][0], roof_sp), (prof["chol_update"][0], roof_chol), (prof["tpass"][0], roof_tpass)]
+    dominant = max(cand, key=lambda x: x[0])[1]
+    dominant = dict(dominant, source="one extra solve of the same batch as a single lockstep group, HIP events "
+                                     "around every launch on the solve stream (the timed steps run four "
+                                     "concurrent size classes, whose kernels overlap)")
 
     value = len(Ms) * args.steps / el
     batch_ms = 1000 * el / args.steps
@@ -218,7 +258,9 @@ def main():
             "row_lse": roof_row,
             "col_lse": roof_col,
         },
-        "batched_latency_ms": {"p50": batch_ms, "p99": batch_ms},
+        "batched_latency_ms": {"p50": batch_ms, "p99": batch_ms,
+                               "note": "one synchronous call returns every instance of the batch together"},
+        "kernel_times_profiled_solve_ms": {k: v[0] for k, v in prof.items() if v[2]},
     }
 
     if not args.no_sequential:
@@ -238,18 +280,28 @@ def main():
         log(f"sequential: {seq:.2f} solves/s, p50 {np.percentile(lat, 50):.1f} ms, "
             f"p99 {np.percentile(lat, 99):.1f} ms")
 
-    result["cpu_baseline"] = cpu_baseline(probs, args.cpu_budget)
+    if args.serve_n > 0:
+        log(f"serving leg: {args.serve_n} requests of the mixed-size stream")
+        result["served"] = serve_leg(args, seq_n=min(args.serve_n, 300))
+    if args.cpu_budget > 0:
+        result["cpu_baseline"] = cpu_baseline(probs, args.cpu_budget)
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.2f} solves/s on {result['cpu_baseline']['cores']} cores")
+        one = cpu_baseline(probs, 0.5 * args.cpu_budget, threads=1, max_stratum=2)
+        one["sample"] = "one thread; lower half of the batch by n*m only (two of the four groups): " + one["sample"]
+        result["cpu_baseline_1core"] = one
+        log(f"cpu baseline, 1 thread, lower half: {one['value']:.2f} solves/s")
     print(json.dumps(result), flush=True)
 
 
 SERVE_METRIC = "served SPLR-OT solves/s + p50/p99 latency vs sequential, mixed-size Poisson stream, FP64 tol 1e-8"
 
 
-def serve_main(args):
+def serve_leg(args, seq_n=None):
     """Replay the head of the serving trace (Poisson arrivals at args.rate
     req/s, n, m log-uniform in [64, 8192], d in {4,16,64}, Dirichlet(0.5..2)
     marginals, eta in {0.01,0.05,0.1}, 30% with an upstream gradient G) through
-    the continuous-batching server, then solve the same requests one at a time."""
+    the continuous-batching server, then solve the first seq_n of the same
+    requests one at a time (plain entry, plus the backward where G is given)."""
     import curegot_backward_amd as cg
     from curegot_backward_amd.serving import OTServer
     from curegot_backward_amd.workloads import serving_problem, serving_trace
@@ -281,15 +333,15 @@ def serve_main(args):
         outs = [f.result() for f in futs]
         el = time.perf_counter() - t0
         st = srv.stats()
-        for (k, c, nb, a, b) in srv.batch_log():
-            log(f"  batch slot {k} class {c} requests {nb:3d}  {1000 * a:8.1f} -> {1000 * b:8.1f} ms")
+        blog = srv.batch_log()
     conv = sum(1 for o in outs if o["converged"])
     log(f"served: {len(outs)} in {el:.3f} s, batches {st['batches']}, p50 {st['p50_ms']:.1f} ms, "
         f"p99 {st['p99_ms']:.1f} ms, converged {conv}")
 
+    seq_n = len(reqs) if seq_n is None else min(seq_n, len(reqs))
     lat = []
     t1 = time.perf_counter()
-    for (M, a, b, eta, G) in reqs:
+    for (M, a, b, eta, G) in reqs[:seq_n]:
         ts = time.perf_counter()
         o = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 1000, allow_invalid=True)
         if G is not None and o["context"] is not None:
@@ -297,33 +349,38 @@ def serve_main(args):
         torch.cuda.synchronize()
         lat.append(1000 * (time.perf_counter() - ts))
     seq_el = time.perf_counter() - t1
-    order = np.argsort(lat)[::-1][:8]
-    log("slowest solo: " + ", ".join(f"{tuple(reqs[i][0].shape)} {lat[i]:.1f} ms" for i in order))
     value = len(outs) / el
-    seq = len(reqs) / seq_el
-    log(f"sequential: {seq:.2f} solves/s")
-    result = {
-        "metric": SERVE_METRIC,
+    seq = seq_n / seq_el
+    log(f"sequential (first {seq_n}): {seq:.2f} solves/s; served/sequential {value / seq:.2f}")
+    del reqs
+    torch.cuda.empty_cache()
+    return {
         "value": value,
         "unit": "solves/s",
-        "n_gpus": 1,
-        "higher_is_better": True,
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"head of the serving trace: {len(reqs)} requests, Poisson {args.rate:g} req/s, "
-                        "n,m log-uniform in [64,8192], d in {4,16,64}, Dirichlet(0.5..2), eta in "
-                        "{0.01,0.05,0.1}, 30% forward+backward (G ~ N(0,1)), tol 1e-8",
-            "requests": len(reqs), "cells": cells, "slots": args.slots, "max_batch": args.max_batch,
-            "seed": args.seed, "converged": conv,
-        },
-        "served_latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "max": st["max_ms"],
-                              "queue_p50": st["queue_p50_ms"], "queue_p99": st["queue_p99_ms"]},
-        "batches": st["batches"],
-        "sequential": {"solves_per_s": seq, "p50_ms": float(np.percentile(lat, 50)),
-                       "p99_ms": float(np.percentile(lat, 99)), "total_s": seq_el},
+        "workload": f"head of the serving trace: {len(outs)} requests, Poisson {args.rate:g} req/s, "
+                    "n,m log-uniform in [64,8192], d in {4,16,64}, Dirichlet(0.5..2), eta in "
+                    "{0.01,0.05,0.1}, 30% forward+backward (G ~ N(0,1)), tol 1e-8",
+        "requests": len(outs), "arrival_span_s": trace[-1]["t"], "makespan_s": el, "cells": cells,
+        "slots": args.slots, "max_batch": args.max_batch, "converged": conv, "batches": st["batches"],
+        "latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "max": st["max_ms"],
+                       "queue_p50": st["queue_p50_ms"], "queue_p99": st["queue_p99_ms"]},
+        "sequential": {"solves_per_s": seq, "requests": seq_n, "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99)), "total_s": seq_el,
+                       "note": f"the first {seq_n} requests of the same stream, one at a time"},
         "speedup_vs_sequential": value / seq,
+        "batch_log_tail": blog[-5:],
+    }
+
+
+def serve_main(args):
+    r = serve_leg(args, seq_n=args.serve_seq)
+    result = {
+        "metric": SERVE_METRIC, "value": r["value"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": r["workload"], "requests": r["requests"], "slots": r["slots"],
+                   "max_batch": r["max_batch"], "seed": args.seed, "converged": r["converged"]},
+        "served_latency_ms": r["latency_ms"], "batches": r["batches"], "sequential": r["sequential"],
+        "speedup_vs_sequential": r["speedup_vs_sequential"],
     }
     print(json.dumps(result), flush=True)
 

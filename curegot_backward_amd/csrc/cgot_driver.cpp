@@ -705,6 +705,7 @@ struct HostInst {
     // memory at pat_off); core size
     bool sparse = false, pend = false;
     int spR = 0, spH = 0;   // sparse plan: core size, periphery levels
+    double sp_flops = 0.0;  // sparse plan: flops of one factorization and solve (sp_solve_flops)
     size_t pat_off = 0;
 };
 
@@ -898,6 +899,7 @@ struct SplrSolver {
             hi[b].sparse = ok[k] != 0;
             hi[b].spR = ok[k] ? hdr[k].R : 0;
             hi[b].spH = ok[k] ? hdr[k].H : 0;
+            hi[b].sp_flops = ok[k] ? sp_solve_flops(hdr[k]) : 0.0;
             hi[b].pend = false;
         }
         hipLaunchKernelGGL(k_plan_import, dim3(B * SP_IMPORT_WGS), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.ctl(c),
@@ -994,10 +996,12 @@ struct SplrSolver {
         std::vector<char> on(B, 0);
         std::vector<int> ids;
         int rmax_used = 0;
+        double sp_work = 0.0;
         for (int b = 0; b < B; b++) {
             if (!on_all[b]) continue;
             if (hi[b].sparse) {
                 ids.push_back(b);
+                sp_work += hi[b].sp_flops;
                 rmax_used = std::max(rmax_used, hi[b].spR);
             } else {
                 on[b] = 1;
@@ -1010,7 +1014,7 @@ struct SplrSolver {
             int* hid = (int*)pinned_ids.get(ids.size() / 2 + 1);
             std::memcpy(hid, ids.data(), sizeof(int) * ids.size());
             CG_HIP(hipMemcpyAsync(bt.d_ids, hid, sizeof(int) * ids.size(), hipMemcpyHostToDevice, bt.stream));
-            ProfScope ps(bt.stream, PROF_SPNEWTON, 0.0);
+            ProfScope ps(bt.stream, PROF_SPNEWTON, sp_work);
             hipLaunchKernelGGL(k_sp_newton, dim3((unsigned)ids.size()), dim3(SP_WG), sp_lds_bytes(rmax_used),
                                bt.stream, bt.d_insts, bt.d_ids);
         }

@@ -109,6 +109,19 @@ inline double sp_cost_us(const SpHdr& h)
 }
 inline double sp_dense_cost_us(int nblk) { return 150.0 + 32.0 * nblk; }
 
+// Arithmetic of one factorization and solve with a plan (two right-hand
+// sides), counting an FMA as two flops: the periphery gathers (one FMA per
+// pair) and their square roots or divisions, the core's Schur complement
+// gathers, the dense core's right-looking elimination (about R^3 / 3) and
+// scaling, and the forward and backward substitutions of both right-hand
+// sides through the periphery and the core.
+inline double sp_solve_flops(const SpHdr& h)
+{
+    const double R = h.R, P = h.nnzP, t = h.t;
+    return 2.0 * (double)h.npairs + P + 2.0 * (double)h.ncpairs + R * (R - 1.0) * (R + 1.0) / 3.0 +
+           R * (R + 1.0) / 2.0 + 4.0 * (double)h.nrowpairs + 4.0 * R * (R - 1.0) + 4.0 * (P - t) + 4.0 * h.N;
+}
+
 #ifdef __cplusplus
 // Host analysis.  Returns true and fills `plan` (the integer part of the
 // plan image, hdr.o_meta_end bytes, offsets relative to its start; the value
