@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
+    ap.add_argument("--newton", choices=("auto", "sparse", "dense"), default=None,
+                    help="Newton solve: sparse factor by latency estimate (auto), wherever a plan fits, or dense")
     ap.add_argument("--serve", action="store_true", help="measure the serving loop on the mixed-size stream")
     ap.add_argument("--single", action="store_true", help="single n=m=2048 instance latency beside the CPU oracle")
     ap.add_argument("--lse", action="store_true",
@@ -114,6 +116,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=64)
     args = ap.parse_args()
+    if args.newton is not None:
+        from curegot_backward_amd import _lib as _l
+        _l.set_sparse_solve(args.newton)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (instances are independent; a multi-GPU "
               "run would shard the batch by HBM footprint). Use --gpus 1.", file=sys.stderr)

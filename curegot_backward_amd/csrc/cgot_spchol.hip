@@ -98,14 +98,31 @@ CG_DEV SpView sp_view(const CG_AS1 SpHdr* hp, CG_AS1 double* plan)
 }
 
 // Streams the plan's integer arrays through the cache (coalesced, 16 bytes
-// per thread and step); the returned value only keeps the loads alive.
+// per thread and load, SP_PF loads per thread in flight before any is used);
+// the returned value only keeps the loads alive.
+#define SP_PF 8
 CG_DEV unsigned sp_prefetch(const SpView& v)
 {
     unsigned acc = 0;
     const int64_t n4 = v.meta_ints & ~(int64_t)3;
-    for (int64_t k = 4 * (int64_t)threadIdx.x; k < n4; k += 4 * SP_WG) {
-        const CG_AS1 int* m = v.meta + k;
-        acc ^= (unsigned)m[0] ^ (unsigned)m[1] ^ (unsigned)m[2] ^ (unsigned)m[3];
+    const CG_AS1 unsigned long long* m8 = reinterpret_cast<const CG_AS1 unsigned long long*>(v.meta);
+    const int64_t nv = n4 / 4;   // 16-byte words
+    if (nv == 0) return 0u;
+    for (int64_t base = threadIdx.x; base < nv; base += (int64_t)SP_PF * SP_WG) {
+        // indices past the end load the last word again (no branch, so that
+        // all SP_PF loads are issued before the first wait)
+        unsigned long long lo[SP_PF], hi[SP_PF];
+#pragma unroll
+        for (int u = 0; u < SP_PF; u++) {
+            const int64_t k = min(base + (int64_t)u * SP_WG, nv - 1);
+            lo[u] = m8[2 * k];
+            hi[u] = m8[2 * k + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < SP_PF; u++) {
+            const unsigned long long x = lo[u] ^ hi[u];
+            acc ^= (unsigned)x ^ (unsigned)(x >> 32);
+        }
     }
     return acc;
 }
