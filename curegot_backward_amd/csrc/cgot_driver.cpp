@@ -694,6 +694,9 @@ struct HostInst {
     long long Te, Kmax, K;
     int P, nblk;
     bool done = false, converged = false, invalid = false;
+    bool live = false;      // started (begin) and not yet finished
+    bool x0 = false;        // warm start given
+    int it = 0;             // iterations taken
     int niter = 0;
     double f = 0, gnorm = 0, gnorm_init = 0, step_prev = 1.0;
     bool analyze = false, update = false;
@@ -724,6 +727,81 @@ long long kmax_of(double dmax, int n, int m)
     return std::max(K, 1LL);
 }
 
+// Geometry, cost source, Kmax, Schur side and outputs of one SPLR instance.
+void inst_geometry(CgInst& I, const cgot_splr_problem& p)
+{
+    set_geometry(I, p.n, p.m, p.reg);
+    double dmax, shmax;
+    int cyc, cand;
+    resolve_defaults(p, dmax, shmax, cyc, cand);
+    I.Kmax = kmax_of(dmax, p.n, p.m);
+    I.beta_side = (p.m - 1 <= p.n) ? 1 : 0;
+    I.P = I.beta_side ? (p.m - 1) : p.n;
+    set_cost(I, p.M, p.xy);
+    I.plan = p.plan;
+    I.dual = p.dual;
+}
+
+// Zero-initialised scratch of one instance: 4 counters, the top-K histogram,
+// the backward-substitution block flags.
+size_t zero_words(const CgInst& I) { return 4 + CG_HBINS + (size_t)std::max(cdiv(I.P, CG_NB), 1); }
+void set_zero_region(CgInst& I, unsigned int* z)
+{
+    I.counter = z;
+    I.hist = z ? I.counter + 4 : nullptr;
+    I.flags = z ? (int*)(I.hist + CG_HBINS) : nullptr;
+}
+
+// Per-instance arrays (all but the descriptor, the zero region and the scalars).
+void inst_arrays(CgInst& I, Bump& bp)
+{
+    const int n = I.n, m = I.m, N = I.N;
+    I.ab = bp.take<double>(n + m);
+    I.logab = bp.take<double>(n + m);
+    I.gamma = bp.take<double>(n + m);
+    I.gamma_prev = bp.take<double>(n + m);
+    I.gamma_bcd = bp.take<double>(n + m);
+    I.grad = bp.take<double>(N);
+    I.grad_prev = bp.take<double>(N);
+    I.grad_bcd = bp.take<double>(N);
+    I.dir = bp.take<double>(N);
+    I.yv = bp.take<double>(N);
+    I.sv = bp.take<double>(N);
+    I.uv = bp.take<double>(N);
+    I.rowsum = bp.take<double>(n);
+    I.colsum = bp.take<double>(m);
+    I.rp0 = bp.take<double>((size_t)n * I.nstrips);
+    I.rp1 = bp.take<double>((size_t)n * I.nstrips);
+    I.cp0 = bp.take<double>((size_t)I.nrb * m);
+    I.cp1 = bp.take<double>((size_t)I.nrb * m);
+    I.wgpart = bp.take<double>((size_t)8 * I.nvec);
+    I.w_rowptr = bp.take<int>(n + 1);
+    I.w_colj = bp.take<int>(I.Kmax);
+    I.w_rval = bp.take<double>(I.Kmax);
+    I.w_colptr = bp.take<int>(m);
+    I.w_rowi = bp.take<int>(I.Kmax);
+    I.w_cval = bp.take<double>(I.Kmax);
+    I.Da = bp.take<double>(n);
+    I.Db = bp.take<double>(m);
+    I.rcnt = bp.take<int>(n);
+    I.rtie = bp.take<int>(n);
+    I.rcut = bp.take<int>(n);
+    I.ccnt = bp.take<int>((size_t)I.nrb * m);
+    I.S = bp.take<double>((size_t)I.P * I.P);
+    I.sph = bp.take<SpHdr>(1);
+    I.Winv = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
+    I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
+    I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
+    I.inv_diag = bp.take<double>(N);
+}
+
+// The value refresh runs on the wave map when some instance's pattern may
+// hold more than 32 items per vector-map thread (N + 2 Kmax items).
+bool wants_refresh_rows(const CgInst& I)
+{
+    return (double)I.N + 2.0 * (double)I.Kmax > 32.0 * CG_WG * (double)I.nvec;
+}
+
 // Lay out the workspace of an SPLR batch; with base == nullptr only sizes.
 size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
 {
@@ -731,74 +809,23 @@ size_t splr_layout(Batch& bt, const cgot_splr_problem* probs, void* base)
     bt.h.assign(B, CgInst{});
     int nblk_max = 0;
     for (int b = 0; b < B; b++) {
-        const auto& p = probs[b];
-        CgInst& I = bt.h[b];
-        set_geometry(I, p.n, p.m, p.reg);
-        double dmax, shmax;
-        int cyc, cand;
-        resolve_defaults(p, dmax, shmax, cyc, cand);
-        I.Kmax = kmax_of(dmax, p.n, p.m);
-        I.beta_side = (p.m - 1 <= p.n) ? 1 : 0;
-        I.P = I.beta_side ? (p.m - 1) : p.n;
-        nblk_max = std::max(nblk_max, cdiv(I.P, CG_NB));
-        set_cost(I, p.M, p.xy);
-        I.plan = p.plan;
-        I.dual = p.dual;
+        inst_geometry(bt.h[b], probs[b]);
+        nblk_max = std::max(nblk_max, cdiv(bt.h[b].P, CG_NB));
     }
     bt.assign_maps();
     bt.refresh_rows = false;
     for (const CgInst& I : bt.h)
-        if ((double)I.N + 2.0 * (double)I.Kmax > 32.0 * CG_WG * (double)I.nvec) bt.refresh_rows = true;
+        if (wants_refresh_rows(I)) bt.refresh_rows = true;
     Bump bp(base);
     bt.layout_common(bp, nblk_max);
     // scratch that must start at zero, contiguous for one clear
-    // per instance: 4 counters, the top-K histogram, the block flags
     const size_t zstride = 4 + CG_HBINS + (size_t)std::max(nblk_max, 1);
     bt.zero_base = bp.take<unsigned int>((size_t)B * zstride);
     bt.zero_bytes = (size_t)B * zstride * sizeof(unsigned int);
     for (int b = 0; b < B; b++) {
         CgInst& I = bt.h[b];
-        const int n = I.n, m = I.m, N = I.N;
-        I.counter = bt.zero_base ? bt.zero_base + (size_t)b * zstride : nullptr;
-        I.hist = bt.zero_base ? I.counter + 4 : nullptr;
-        I.flags = bt.zero_base ? (int*)(I.hist + CG_HBINS) : nullptr;
-        I.ab = bp.take<double>(n + m);
-        I.logab = bp.take<double>(n + m);
-        I.gamma = bp.take<double>(n + m);
-        I.gamma_prev = bp.take<double>(n + m);
-        I.gamma_bcd = bp.take<double>(n + m);
-        I.grad = bp.take<double>(N);
-        I.grad_prev = bp.take<double>(N);
-        I.grad_bcd = bp.take<double>(N);
-        I.dir = bp.take<double>(N);
-        I.yv = bp.take<double>(N);
-        I.sv = bp.take<double>(N);
-        I.uv = bp.take<double>(N);
-        I.rowsum = bp.take<double>(n);
-        I.colsum = bp.take<double>(m);
-        I.rp0 = bp.take<double>((size_t)n * I.nstrips);
-        I.rp1 = bp.take<double>((size_t)n * I.nstrips);
-        I.cp0 = bp.take<double>((size_t)I.nrb * m);
-        I.cp1 = bp.take<double>((size_t)I.nrb * m);
-        I.wgpart = bp.take<double>((size_t)8 * I.nvec);
-        I.w_rowptr = bp.take<int>(n + 1);
-        I.w_colj = bp.take<int>(I.Kmax);
-        I.w_rval = bp.take<double>(I.Kmax);
-        I.w_colptr = bp.take<int>(m);
-        I.w_rowi = bp.take<int>(I.Kmax);
-        I.w_cval = bp.take<double>(I.Kmax);
-        I.Da = bp.take<double>(n);
-        I.Db = bp.take<double>(m);
-        I.rcnt = bp.take<int>(n);
-        I.rtie = bp.take<int>(n);
-        I.rcut = bp.take<int>(n);
-        I.ccnt = bp.take<int>((size_t)I.nrb * m);
-        I.S = bp.take<double>((size_t)I.P * I.P);
-        I.sph = bp.take<SpHdr>(1);
-        I.Winv = bp.take<double>((size_t)cdiv(I.P, CG_NB) * CG_NB * CG_NB);
-        I.rhs = bp.take<double>(2 * (size_t)std::max(I.P, 1));
-        I.ltmp = bp.take<double>(2 * (size_t)std::max(n, m));
-        I.inv_diag = bp.take<double>(N);
+        set_zero_region(I, bt.zero_base ? bt.zero_base + (size_t)b * zstride : nullptr);
+        inst_arrays(I, bp);
     }
     return bp.off + 256;
 }
@@ -1125,73 +1152,94 @@ struct SplrSolver {
         bt.vec0(k_woodbury_apply, dc);
     }
 
-    void init_host(const cgot_splr_problem* probs)
+    // Host state of instance b from its problem (its iteration counter at 0).
+    void init_inst(int b, const cgot_splr_problem& p)
+    {
+        HostInst& H = hi[b];
+        H = HostInst{};
+        H.n = p.n; H.m = p.m; H.N = p.n + p.m - 1;
+        H.reg = p.reg; H.tol = p.tol; H.max_iter = p.max_iter;
+        resolve_defaults(p, H.dmax, H.shift_max, H.cycle, H.cand);
+        H.dmin = 0.01 * H.dmax;
+        H.density = 0.1 * H.dmax;
+        H.Te = (long long)p.n * (p.m - 1);
+        H.Kmax = kmax_of(H.dmax, p.n, p.m);
+        H.K = 0;
+        H.P = bt.h[b].P;
+        H.nblk = cdiv(H.P, CG_NB);
+        H.trace = p.trace;
+        H.x0 = p.x0 != nullptr;
+    }
+
+    // Start the instances in `jn` (zeroed scratch, descriptors and input
+    // pointers on the device): ab, log ab, gamma = x0 (gauge shifted) or
+    // beta = 0 and alpha = row LSE, the first T pass, the first pattern.
+    void begin(const std::vector<char>& jn)
     {
         const int B = bt.B;
-        hi.assign(B, HostInst{});
-        for (int b = 0; b < B; b++) {
-            const auto& p = probs[b];
-            HostInst& H = hi[b];
-            H.n = p.n; H.m = p.m; H.N = p.n + p.m - 1;
-            H.reg = p.reg; H.tol = p.tol; H.max_iter = p.max_iter;
-            resolve_defaults(p, H.dmax, H.shift_max, H.cycle, H.cand);
-            H.dmin = 0.01 * H.dmax;
-            H.density = 0.1 * H.dmax;
-            H.Te = (long long)p.n * (p.m - 1);
-            H.Kmax = kmax_of(H.dmax, p.n, p.m);
-            H.K = 0;
-            H.P = bt.h[b].P;
-            H.nblk = cdiv(H.P, CG_NB);
-            H.trace = p.trace;
+        const double* dj = bt.ctl(mask(jn));
+        hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dj,
+                           (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
+        {
+            std::vector<char> wx(B, 0), nox(B, 0);
+            for (int b = 0; b < B; b++)
+                if (jn[b]) (hi[b].x0 ? wx : nox)[b] = 1;
+            if (std::count(wx.begin(), wx.end(), 1)) bt.per_inst(k_gauge_shift, bt.ctl(mask(wx)), CG_PT_GAMMA);
+            if (std::count(nox.begin(), nox.end(), 1)) row_lse(bt, bt.ctl(mask(nox)), CG_PT_GAMMA);
         }
+        tpass(bt, dj, CG_PT_GAMMA);
+        bt.check_launch();
+        bt.read_scal();
+        for (int b = 0; b < B; b++) {
+            if (!jn[b]) continue;
+            HostInst& H = hi[b];
+            H.f = bt.sc(b, CS_F);
+            H.gnorm = std::sqrt(bt.sc(b, CS_G2));
+            H.gnorm_init = H.gnorm;
+            H.live = true;
+        }
+        sphess(jn, jn);
     }
 
     void run(const cgot_splr_problem* probs, cgot_splr_info* info)
     {
         const int B = bt.B;
-        init_host(probs);
+        hi.assign(B, HostInst{});
+        for (int b = 0; b < B; b++) init_inst(b, probs[b]);
         zero_scratch();
         std::vector<char> all(B, 1);
-        const double* dall = bt.ctl(mask(all));
-        // init: ab, log ab, gamma = x0 (gauge shifted) or beta = 0 and alpha = row LSE
-        hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dall,
-                           (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
-        {
-            std::vector<char> wx(B, 0), nox(B, 0);
-            for (int b = 0; b < B; b++) (probs[b].x0 ? wx : nox)[b] = 1;
-            if (std::count(wx.begin(), wx.end(), 1)) bt.per_inst(k_gauge_shift, bt.ctl(mask(wx)), CG_PT_GAMMA);
-            if (std::count(nox.begin(), nox.end(), 1)) row_lse(bt, bt.ctl(mask(nox)), CG_PT_GAMMA);
+        begin(all);
+        while (step()) {
         }
-        tpass(bt, dall, CG_PT_GAMMA);
-        bt.check_launch();
-        bt.read_scal();
-        for (int b = 0; b < B; b++) {
-            HostInst& H = hi[b];
-            H.f = bt.sc(b, CS_F);
-            H.gnorm = std::sqrt(bt.sc(b, CS_G2));
-            H.gnorm_init = H.gnorm;
-        }
-        sphess(all, all);
+        bt.use_full();
+        finish(all, info);
+    }
 
-        for (int iter = 0;; iter++) {
+    // One SPLR iteration of every live instance that has not stopped; each
+    // instance counts its own iterations.  Returns false (and does nothing)
+    // when none is left iterating.
+    bool step()
+    {
+        const int B = bt.B;
+        {
             // loop-entry tests
             std::vector<char> act(B, 0);
             int nact = 0;
             for (int b = 0; b < B; b++) {
                 HostInst& H = hi[b];
-                if (H.done) continue;
-                if (iter >= H.max_iter) { H.done = true; continue; }
+                if (!H.live || H.done) continue;
+                if (H.it >= H.max_iter) { H.done = true; continue; }
                 if (H.gnorm < H.tol) { H.converged = true; H.done = true; continue; }
                 if (!std::isfinite(H.f)) { H.done = true; continue; }
-                H.analyze = (iter % H.cycle == 0);
-                H.update = (iter % H.cycle == H.cycle - 1);
+                H.analyze = (H.it % H.cycle == 0);
+                H.update = (H.it % H.cycle == H.cycle - 1);
                 act[b] = 1;
                 nact++;
             }
-            if (nact == 0) break;
+            if (nact == 0) return false;
             bt.pack_live(act);
             auto c = mask(act);
-            for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ALLOW_LR] = iter > 0 ? 1.0 : 0.0;
+            for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ALLOW_LR] = hi[b].it > 0 ? 1.0 : 0.0;
             const double* dc = bt.ctl(c);
             bt.vec0(k_low_rank, dc);
             // Sinkhorn candidate (3 BCD sweeps from the current beta)
@@ -1299,15 +1347,15 @@ struct SplrSolver {
                     H.density = std::min(H.dmax, std::max(H.dmin, H.density));
                 }
                 if (H.trace) {
-                    double* tr = H.trace + 8 * (size_t)iter;
+                    double* tr = H.trace + 8 * (size_t)H.it;
                     tr[0] = H.f; tr[1] = H.gnorm; tr[2] = H.ls.result; tr[3] = bt.sc(b, CS_LOWRANK);
                     tr[4] = use_bcd[b]; tr[5] = (double)H.K; tr[6] = H.ls.trials;
                     tr[7] = (H.ls.recompute && H.ls.result != H.ls.last_eval) ? 1.0 : 0.0;
                 }
                 const bool conv_after = std::isfinite(H.gnorm) && (H.gnorm < H.tol);
-                const bool max_reached = (iter + 1 >= H.max_iter);
+                const bool max_reached = (H.it + 1 >= H.max_iter);
                 const bool inv_after = !std::isfinite(H.f) || !std::isfinite(H.gnorm);
-                H.niter = iter + 1;
+                H.niter = H.it + 1;
                 if (conv_after || max_reached || inv_after) {
                     H.done = true;
                     if (inv_after) H.invalid = true;
@@ -1324,17 +1372,25 @@ struct SplrSolver {
             }
             sphess(newpat, refresh);
             bt.check_launch();
+            for (int b = 0; b < B; b++)
+                if (act[b]) hi[b].it++;
         }
+        return true;
+    }
 
-        // outputs and context
-        bt.use_full();
-        const double* dall2 = bt.ctl(mask(all));
-        bt.vec0(k_output_dual, dall2);
-        bt.tiles0(k_plan, dall2);
-        bt.vec(k_vcopy, dall2, 2);
+    // Outputs of the instances in `which` (the maps must cover them): the
+    // gauge-fixed dual, the plan, the context's dual copy, and their infos.
+    void finish(const std::vector<char>& which, cgot_splr_info* info)
+    {
+        const int B = bt.B;
+        const double* dw = bt.ctl(mask(which));
+        bt.vec0(k_output_dual, dw);
+        bt.tiles0(k_plan, dw);
+        bt.vec(k_vcopy, dw, 2);
         bt.check_launch();
         bt.read_scal();
         for (int b = 0; b < B; b++) {
+            if (!which[b]) continue;
             const HostInst& H = hi[b];
             cgot_splr_info& o = info[b];
             const CgInst& I = bt.h[b];
@@ -2565,7 +2621,8 @@ int cgot_T_sparsify(const double* gamma, const double* M, const double* a, const
     std::vector<const double*> ptrs = {a, b, gamma};
     bt.upload_maps(ptrs);
     SplrSolver s(bt);
-    s.init_host(&p);
+    s.hi.assign(1, HostInst{});
+    s.init_inst(0, p);
     s.zero_scratch();
     std::vector<char> all(1, 1);
     const double* dc = bt.ctl(s.mask(all));
