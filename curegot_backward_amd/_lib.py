@@ -110,6 +110,8 @@ EXPORTS = (
     "cgot_T_sparsify_workspace_size", "cgot_T_sparsify",
     "cgot_profile_enable", "cgot_profile_reset", "cgot_profile_read",
     "cgot_sparse_plan",
+    "cgot_session_workspace_size", "cgot_session_inst_workspace_size", "cgot_session_inst_tiles",
+    "cgot_session_create", "cgot_session_add", "cgot_session_step", "cgot_session_live", "cgot_session_destroy",
     "cgot_shutdown", "cgot_last_error", "cgot_version",
 )
 
@@ -172,6 +174,22 @@ def load():
     L.cgot_profile_read.restype = ctypes.c_int
     L.cgot_sparse_plan.argtypes = [i32, i32, vp, vp, i64, i32, vp, vp, i64]
     L.cgot_sparse_plan.restype = i64
+    L.cgot_session_workspace_size.argtypes = [i32, i64, i64, i32]
+    L.cgot_session_workspace_size.restype = sz
+    L.cgot_session_inst_workspace_size.argtypes = [ctypes.POINTER(SplrProblem)]
+    L.cgot_session_inst_workspace_size.restype = sz
+    L.cgot_session_inst_tiles.argtypes = [i32, i32]
+    L.cgot_session_inst_tiles.restype = i64
+    L.cgot_session_create.argtypes = [i32, i64, i64, i32, vp, sz, vp, ctypes.POINTER(vp)]
+    L.cgot_session_create.restype = ctypes.c_int
+    L.cgot_session_add.argtypes = [vp, ctypes.POINTER(SplrProblem), vp, sz, i64]
+    L.cgot_session_add.restype = ctypes.c_int
+    L.cgot_session_step.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(SplrInfo), i32, ctypes.POINTER(i32)]
+    L.cgot_session_step.restype = ctypes.c_int
+    L.cgot_session_live.argtypes = [vp]
+    L.cgot_session_live.restype = ctypes.c_int
+    L.cgot_session_destroy.argtypes = [vp]
+    L.cgot_session_destroy.restype = ctypes.c_int
     L.cgot_shutdown.argtypes = []
     L.cgot_shutdown.restype = ctypes.c_int
     L.cgot_last_error.restype = ctypes.c_char_p

@@ -24,6 +24,7 @@ __all__ = [
     "sinkhorn_splr",
     "sinkhorn_bcd",
     "SPLRBackwardContext",
+    "SplrSession",
 ]
 
 
@@ -443,6 +444,109 @@ def sinkhorn_splr_bwd_batch(Ms: Sequence[torch.Tensor], as_: Sequence[torch.Tens
                   f"converged={bool(info.converged)} gnorm={info.gnorm:.3e}")
         results.append(res)
     return results
+
+
+class SplrSession:
+    """Continuous batching: one persistent lockstep group of SPLR solves on a
+    stream (C ABI cgot_session_*; no counterpart in the reference, whose
+    splr_ot_batch is a sequential loop, curegot/autograd.py:340-367).
+
+    add() queues a problem (returns False while the group is full); step()
+    runs up to `iters` iterations of every instance in the group, newly added
+    ones starting first, and returns [(tag, result)] for the instances that
+    stopped, each result as sinkhorn_splr_bwd's (plan, dual, niter, converged,
+    context, valid, gnorm, objective).  An instance's result is bitwise the
+    same as solving it alone.  Use a session from one thread at a time; its
+    work runs on `stream` (a torch.cuda.Stream; default: the current one when
+    created) and is complete when step() returns.
+    """
+
+    def __init__(self, device=None, stream=None, capacity=64, max_sum_nm=1 << 20, max_tiles=1 << 20,
+                 max_nblk=128):
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.capacity = int(capacity)
+        L = _lib.load()
+        with torch.cuda.device(self.device):
+            nbytes = L.cgot_session_workspace_size(self.capacity, int(max_sum_nm), int(max_tiles), int(max_nblk))
+            self._ws = _workspace(nbytes, self.device)
+            h = ctypes.c_void_p()
+            rc = L.cgot_session_create(self.capacity, int(max_sum_nm), int(max_tiles), int(max_nblk),
+                                       self._ws.data_ptr(), nbytes, int(self.stream.cuda_stream), ctypes.byref(h))
+            _lib.check(rc, "session_create")
+        self._h = h
+        self._jobs = {}
+        self._next = 0
+
+    @staticmethod
+    def tiles(n, m) -> int:
+        """Launch-map entries an (n, m) instance takes (for the max_tiles limit)."""
+        return int(_lib.load().cgot_session_inst_tiles(int(n), int(m)))
+
+    def live(self) -> int:
+        return int(_lib.load().cgot_session_live(self._h))
+
+    def add(self, M, a, b, reg, tol=1e-6, max_iter=1000, tag=None, **kwargs) -> bool:
+        if self._h is None:
+            raise RuntimeError("session is closed")
+        _check_shapes(M, a, b)
+        M, a, b = _to_dev(M, self.device), _to_dev(a, self.device), _to_dev(b, self.device)
+        n, m = M.shape
+        x0 = kwargs.get("x0", None)
+        if x0 is not None:
+            if x0.dim() != 1 or x0.shape[0] != n + m:
+                raise RuntimeError("x0 must be a 1D tensor of length n + m")
+            x0 = _to_dev(x0, self.device)
+        plan = torch.empty((n, m), dtype=torch.float64, device=self.device)
+        dual = torch.empty(n + m, dtype=torch.float64, device=self.device)
+        p = _make_problem(M, a, b, reg, tol, max_iter, kwargs, plan, dual, x0)
+        L = _lib.load()
+        nbytes = L.cgot_session_inst_workspace_size(ctypes.byref(p))
+        ws = _workspace(nbytes, self.device)
+        key = self._next
+        with torch.cuda.device(self.device):
+            rc = L.cgot_session_add(self._h, ctypes.byref(p), ws.data_ptr(), nbytes, key)
+        if rc == -3:
+            return False
+        if rc < 0:
+            _lib.check(rc, "session_add")
+        self._next += 1
+        self._jobs[key] = (tag, M, a, b, x0, plan, dual, float(reg), ws, kwargs.get("context_mode", "adopt"))
+        return True
+
+    def step(self, iters=1):
+        if self._h is None:
+            raise RuntimeError("session is closed")
+        L = _lib.load()
+        cap = max(len(self._jobs), 1)
+        tags = (ctypes.c_int64 * cap)()
+        infos = (_lib.SplrInfo * cap)()
+        nd = ctypes.c_int32(0)
+        with torch.cuda.device(self.device):
+            rc = L.cgot_session_step(self._h, int(iters), tags, infos, cap, ctypes.byref(nd))
+        _lib.check(rc, "session_step")
+        out = []
+        for k in range(nd.value):
+            tag, M, a, b, x0, plan, dual, reg, ws, mode = self._jobs.pop(int(tags[k]))
+            info = infos[k]
+            ctx = (SPLRBackwardContext(M, dual, reg, info, ws, snapshot=(mode == "snapshot"))
+                   if info.valid else None)
+            out.append((tag, {"niter": int(info.niter), "converged": bool(info.converged), "plan": plan,
+                              "dual": dual, "context": ctx, "valid": bool(info.valid), "gnorm": float(info.gnorm),
+                              "objective": float(info.f)}))
+        return out
+
+    def close(self):
+        if self._h is not None:
+            _lib.check(_lib.load().cgot_session_destroy(self._h), "session_destroy")
+            self._h = None
+            self._jobs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def sinkhorn_splr_bwd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):

@@ -29,6 +29,7 @@
 #include <array>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -357,11 +358,29 @@ struct Batch {
     // the next call or use_full().
     void pack_live(const std::vector<char>& live)
     {
+        if (exact_maps) {
+            set_exact(live);
+            return;
+        }
         long long w = 0;
         for (int b = 0; b < B; b++) if (live[b]) w += (long long)h[b].ntiles + h[b].nvec + h[b].nrow;
         if (8 * w > 7 * packed_wgs) return;
         int nt, nv, nr;
         write_maps(d_live, live, nt, nv, nr);
+        set_maps(d_live, nt, nv, nr);
+    }
+
+    // Exact maps (a session, whose instances come and go): the maps cover
+    // exactly the instances with keep[b], rewritten into d_live whenever that
+    // set changes (pinned staging: at most once between two synchronisations).
+    bool exact_maps = false;
+    std::vector<char> map_set;
+    void set_exact(const std::vector<char>& keep)
+    {
+        if (keep == map_set) return;
+        map_set = keep;
+        int nt, nv, nr;
+        write_maps(d_live, keep, nt, nv, nr);
         set_maps(d_live, nt, nv, nr);
     }
 
@@ -891,6 +910,18 @@ struct SplrSolver {
         std::vector<SpHdr> hdr(todo.size());
         std::vector<std::vector<char>> img(todo.size());
         std::vector<char> ok(todo.size(), 0);
+        // the exported patterns must be those of the selection (row pointers
+        // from 0 to K, non-decreasing, columns below m - 1) before the
+        // analysis indexes with them
+        for (int b : todo) {
+            const HostInst& H = hi[b];
+            const int* rp = hp + H.pat_off;
+            const int* cj = rp + H.n + 1;
+            bool good = rp[0] == 0 && rp[H.n] == H.K;
+            for (int i = 0; good && i < H.n; i++) good = rp[i] <= rp[i + 1];
+            for (long long q = 0; good && q < H.K; q++) good = cj[q] >= 0 && cj[q] < H.m - 1;
+            if (!good) throw std::runtime_error("internal: inconsistent sparsity pattern read back for the analysis");
+        }
         parallel_for((int)todo.size(), [&](int k) {
             const int b = todo[k];
             const HostInst& H = hi[b];
@@ -1177,6 +1208,9 @@ struct SplrSolver {
     void begin(const std::vector<char>& jn)
     {
         const int B = bt.B;
+        // patterns still awaiting their analysis are consumed first: the
+        // starting instances' patterns reuse the pinned staging
+        prepare_plans();
         const double* dj = bt.ctl(mask(jn));
         hipLaunchKernelGGL(k_init_logab, dim3(bt.nvec), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.d_vec, dj,
                            (const double* const*)bt.d_ptrs, (const double* const*)nullptr);
@@ -2005,6 +2039,169 @@ hipStream_t group_stream(int device, int k)
     return it->second[k];
 }
 
+// ---------------------------------------------------------------------------
+// Sessions: one persistent lockstep group into which instances are added at
+// iteration boundaries and from which they leave when they stop (continuous
+// batching).  Each instance's arrays, its scalars and a copy of its
+// descriptor live in a workspace of its own, so that its backward context
+// outlives its slot; the session workspace holds the group's shared arrays
+// (descriptors, maps, control blocks, scalars, prefix tables) for at most
+// `cap` instances within the stated totals.
+// ---------------------------------------------------------------------------
+struct SessionLimits {
+    int cap;
+    int64_t max_sum_nm, max_tiles;
+    int max_nblk;
+};
+
+int64_t inst_tiles(int n, int m)
+{
+    CgInst I{};
+    set_geometry(I, n, m, 1.0);
+    return I.ntiles;
+}
+
+// Shared arrays of a session; with base == nullptr only sizes.
+size_t session_layout(Batch& bt, const SessionLimits& L, void* base)
+{
+    bt.B = L.cap;
+    bt.h.assign(L.cap, CgInst{});
+    // map capacity: tiles, vector groups (ceil((n+m)/CG_WG) each) and waves
+    bt.ntiles = (int)L.max_tiles;
+    bt.nvec = (int)(L.max_sum_nm / CG_WG + L.cap);
+    bt.nrow = (int)(L.max_sum_nm / CG_WAVES + L.cap);
+    Bump bp(base);
+    bt.layout_common(bp, L.max_nblk);
+    // the staging of map uploads is sized by the *_full counts: the capacity
+    bt.ntiles_full = bt.ntiles;
+    bt.nvec_full = bt.nvec;
+    bt.nrow_full = bt.nrow;
+    bt.ntiles = bt.nvec = bt.nrow = 0;
+    bt.packed_wgs = 0;
+    return bp.off + 256;
+}
+
+// One instance's own workspace: descriptor copy, scalars, zero region, arrays.
+struct InstWs {
+    CgInst* desc = nullptr;
+    double* scal = nullptr;
+    unsigned int* zero = nullptr;
+    size_t zbytes = 0;
+};
+size_t inst_ws_layout(CgInst& I, const cgot_splr_problem& p, void* base, InstWs& w)
+{
+    inst_geometry(I, p);
+    Bump bp(base);
+    w.desc = bp.take<CgInst>(1);
+    w.scal = bp.take<double>(CG_NSCAL);
+    const size_t zw = zero_words(I);
+    w.zero = bp.take<unsigned int>(zw);
+    w.zbytes = zw * sizeof(unsigned int);
+    set_zero_region(I, w.zero);
+    inst_arrays(I, bp);
+    return bp.off + 256;
+}
+
+struct Session {
+    SessionLimits lim;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Batch bt;
+    SplrSolver solver;
+    std::vector<char> used, pending;
+    std::vector<int64_t> tag;
+    std::vector<InstWs> iw;
+    std::vector<const double*> ptrs;
+    int64_t sum_nm = 0, tiles = 0;
+    int live = 0;
+    Pinned up_insts, up_ptrs, up_desc;   // staging of descriptor and pointer uploads
+    Pinned ctl_host, scal_host;          // control ring and scalar read-backs
+    bool xy_set = false;
+
+    Session() : solver(bt) {}
+
+    void start_pending()
+    {
+        const int B = bt.B;
+        bool any = false;
+        for (int b = 0; b < B; b++) any = any || pending[b];
+        if (!any) return;
+        for (int b = 0; b < B; b++)
+            if (used[b]) bt.h[b].scal = bt.d_scal + (size_t)b * CG_NSCAL;
+        // descriptors and input pointers of every slot (free slots: zero
+        // descriptors, which the per-instance kernels skip); the staging was
+        // last read before the previous step's final synchronisation
+        CgInst* hs = (CgInst*)up_insts.get((sizeof(CgInst) * B) / sizeof(double) + 1);
+        std::memcpy((void*)hs, bt.h.data(), sizeof(CgInst) * B);
+        CG_HIP(hipMemcpyAsync(bt.d_insts, hs, sizeof(CgInst) * B, hipMemcpyHostToDevice, stream));
+        const double** hp = (const double**)up_ptrs.get(3 * (size_t)B + 1);
+        std::memcpy((void*)hp, ptrs.data(), sizeof(void*) * 3 * B);
+        CG_HIP(hipMemcpyAsync(bt.d_ptrs, hp, sizeof(void*) * 3 * B, hipMemcpyHostToDevice, stream));
+        for (int b = 0; b < B; b++) {
+            if (!pending[b]) continue;
+            CG_HIP(hipMemsetAsync(iw[b].zero, 0, iw[b].zbytes, stream));
+            CG_HIP(hipMemsetAsync(bt.d_scal + (size_t)b * CG_NSCAL, 0, sizeof(double) * CG_NSCAL, stream));
+        }
+        hipLaunchKernelGGL(k_wgpart_init, dim3(B), dim3(CG_WG), 0, stream, bt.d_insts);
+        // slots changed owners: the cached maps are keyed by slot sets only
+        bt.map_set.clear();
+        bt.sub_key.clear();
+        std::vector<char> cover(B, 0);
+        for (int b = 0; b < B; b++) cover[b] = (used[b] && solver.hi[b].live && !solver.hi[b].done) || pending[b];
+        bt.set_exact(cover);
+        solver.begin(pending);
+        std::fill(pending.begin(), pending.end(), 0);
+    }
+
+    // Outputs of the instances that stopped; frees their slots.
+    int collect(int64_t* tags_out, cgot_splr_info* info_out, int cap_out, int have)
+    {
+        const int B = bt.B;
+        std::vector<char> fin(B, 0);
+        int nf = 0;
+        for (int b = 0; b < B; b++)
+            if (used[b] && solver.hi[b].live && solver.hi[b].done) { fin[b] = 1; nf++; }
+        if (nf == 0) return 0;
+        if (have + nf > cap_out) throw std::invalid_argument("session step: output arrays too small");
+        bt.set_exact(fin);
+        std::vector<cgot_splr_info> info(B);
+        solver.finish(fin, info.data());
+        // each finished instance keeps its scalars and its descriptor in its
+        // own workspace; the backward context refers to those
+        CgInst* hd = (CgInst*)up_desc.get((sizeof(CgInst) * nf) / sizeof(double) + 1);
+        int k = 0;
+        for (int b = 0; b < B; b++) {
+            if (!fin[b]) continue;
+            CG_HIP(hipMemcpyAsync(iw[b].scal, bt.d_scal + (size_t)b * CG_NSCAL, sizeof(double) * CG_NSCAL,
+                                  hipMemcpyDeviceToDevice, stream));
+            hd[k] = bt.h[b];
+            hd[k].scal = iw[b].scal;
+            CG_HIP(hipMemcpyAsync(iw[b].desc, hd + k, sizeof(CgInst), hipMemcpyHostToDevice, stream));
+            cgot_splr_info o = info[b];
+            if (o.factor) o.factor = iw[b].desc;
+            tags_out[have + k] = tag[b];
+            info_out[have + k] = o;
+            // free the slot (its host state too: no pattern left pending for it)
+            solver.hi[b] = HostInst{};
+            used[b] = 0;
+            sum_nm -= bt.h[b].n + bt.h[b].m;
+            tiles -= bt.h[b].ntiles;
+            bt.h[b] = CgInst{};
+            ptrs[3 * b] = ptrs[3 * b + 1] = ptrs[3 * b + 2] = nullptr;
+            live--;
+            k++;
+        }
+        bt.map_set.clear();
+        bt.sub_key.clear();
+        // the descriptor copies are read by other streams (the backward): complete them here
+        CG_HIP(hipStreamSynchronize(stream));
+        bt.refresh_rows = false;
+        for (int b = 0; b < B; b++)
+            if (used[b] && wants_refresh_rows(bt.h[b])) bt.refresh_rows = true;
+        return nf;
+    }
+};
+
 void splr_solve_group(const cgot_splr_problem* probs, int B, void* ws, hipStream_t stream, cgot_splr_info* info)
 {
     Batch bt;
@@ -2205,6 +2402,168 @@ int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, 
         if (!err[k].empty()) throw std::runtime_error(err[k]);
     for (int k = 0; k < G; k++)
         for (size_t q = 0; q < g.idx[k].size(); q++) info[g.idx[k][q]] = sub[k][q];
+    return 0;
+    CG_API_END
+}
+
+namespace {
+SessionLimits session_limits(int32_t cap, int64_t max_sum_nm, int64_t max_tiles, int32_t max_nblk)
+{
+    if (cap < 1 || max_sum_nm < 2 || max_tiles < 1 || max_nblk < 1)
+        throw std::invalid_argument("session limits must be positive");
+    if (max_tiles > (1LL << 28) || max_sum_nm > (1LL << 34))
+        throw std::invalid_argument("session limits too large");
+    return SessionLimits{cap, max_sum_nm, max_tiles, max_nblk};
+}
+}  // namespace
+
+size_t cgot_session_workspace_size(int32_t capacity, int64_t max_sum_nm, int64_t max_tiles, int32_t max_nblk)
+{
+    try {
+        Batch bt;
+        return session_layout(bt, session_limits(capacity, max_sum_nm, max_tiles, max_nblk), nullptr);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int64_t cgot_session_inst_tiles(int32_t n, int32_t m)
+{
+    if (n < 1 || m < 1) return -2;
+    return inst_tiles(n, m);
+}
+
+size_t cgot_session_inst_workspace_size(const cgot_splr_problem* p)
+{
+    try {
+        if (!p) throw std::invalid_argument("null problem");
+        CgInst I{};
+        InstWs w;
+        return inst_ws_layout(I, *p, nullptr, w);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 0;
+    }
+}
+
+int cgot_session_create(int32_t capacity, int64_t max_sum_nm, int64_t max_tiles, int32_t max_nblk, void* workspace,
+                        size_t workspace_bytes, void* stream, void** session)
+{
+    CG_API_BEGIN
+    if (!session) throw std::invalid_argument("null session handle");
+    *session = nullptr;
+    const SessionLimits L = session_limits(capacity, max_sum_nm, max_tiles, max_nblk);
+    std::unique_ptr<Session> S(new Session());
+    S->lim = L;
+    S->stream = (hipStream_t)stream;
+    CG_HIP(hipGetDevice(&S->device));
+    Batch& bt = S->bt;
+    bt.stream = S->stream;
+    const size_t need = session_layout(bt, L, nullptr);
+    if (!workspace || workspace_bytes < need)
+        throw std::invalid_argument("session workspace too small: need " + std::to_string(need) + " bytes");
+    session_layout(bt, L, workspace);
+    bt.exact_maps = true;
+    bt.map_set.assign(L.cap, 0);
+    S->used.assign(L.cap, 0);
+    S->pending.assign(L.cap, 0);
+    S->tag.assign(L.cap, 0);
+    S->iw.assign(L.cap, InstWs{});
+    S->ptrs.assign(3 * (size_t)L.cap, nullptr);
+    S->solver.hi.assign(L.cap, HostInst{});
+    // control ring and scalar read-back staging of this session
+    bt.hctl = S->ctl_host.get((size_t)kCtlSlots * L.cap * CG_NCTL);
+    bt.hscal = S->scal_host.get((size_t)L.cap * CG_NSCAL);
+    CG_HIP(hipMemsetAsync(bt.d_insts, 0, sizeof(CgInst) * L.cap, bt.stream));
+    CG_HIP(hipMemsetAsync(bt.d_scal, 0, sizeof(double) * CG_NSCAL * L.cap, bt.stream));
+    CG_HIP(hipStreamSynchronize(bt.stream));
+    register_atexit();
+    *session = S.release();
+    return 0;
+    CG_API_END
+}
+
+int cgot_session_add(void* session, const cgot_splr_problem* p, void* inst_workspace, size_t inst_workspace_bytes,
+                     int64_t tag)
+{
+    CG_API_BEGIN
+    Session* S = (Session*)session;
+    if (!S || !p) throw std::invalid_argument("null session or problem");
+    check_problem(*p);
+    const bool xy = p->xy.X != nullptr;
+    if (S->live > 0 && S->xy_set && xy != S->bt.xy)
+        throw std::invalid_argument("a session must not mix geometric (X, Y) and stored (M) costs");
+    int b = -1;
+    for (int k = 0; k < S->lim.cap; k++)
+        if (!S->used[k]) { b = k; break; }
+    const int64_t t = inst_tiles(p->n, p->m);
+    CgInst probe{};
+    inst_geometry(probe, *p);
+    if (b < 0 || S->sum_nm + p->n + p->m > S->lim.max_sum_nm || S->tiles + t > S->lim.max_tiles ||
+        cdiv(probe.P, CG_NB) > S->lim.max_nblk)
+        return -3;   // no room now
+    CgInst& I = S->bt.h[b];
+    I = CgInst{};
+    InstWs w;
+    const size_t need = inst_ws_layout(I, *p, nullptr, w);
+    if (!inst_workspace || inst_workspace_bytes < need)
+        throw std::invalid_argument("instance workspace too small: need " + std::to_string(need) + " bytes");
+    inst_ws_layout(I, *p, inst_workspace, w);
+    S->iw[b] = w;
+    S->bt.xy = xy;
+    S->xy_set = true;
+    if (wants_refresh_rows(I)) S->bt.refresh_rows = true;
+    S->ptrs[3 * b] = p->a;
+    S->ptrs[3 * b + 1] = p->b;
+    S->ptrs[3 * b + 2] = p->x0;
+    S->solver.init_inst(b, *p);
+    S->used[b] = 1;
+    S->pending[b] = 1;
+    S->tag[b] = tag;
+    S->sum_nm += p->n + p->m;
+    S->tiles += t;
+    S->live++;
+    return b;
+    CG_API_END
+}
+
+int cgot_session_step(void* session, int32_t iters, int64_t* tags_out, cgot_splr_info* info_out, int32_t out_cap,
+                      int32_t* ndone)
+{
+    CG_API_BEGIN
+    Session* S = (Session*)session;
+    if (!S || !ndone || (out_cap > 0 && (!tags_out || !info_out))) throw std::invalid_argument("bad arguments");
+    *ndone = 0;
+    int dev = 0;
+    CG_HIP(hipGetDevice(&dev));
+    if (dev != S->device) throw std::invalid_argument("session used on another device");
+    S->start_pending();
+    int have = 0;
+    for (int k = 0; k < std::max(iters, 1); k++) {
+        const bool more = S->solver.step();
+        have += S->collect(tags_out, info_out, out_cap, have);
+        if (!more) break;
+    }
+    *ndone = have;
+    return 0;
+    CG_API_END
+}
+
+int cgot_session_live(void* session)
+{
+    Session* S = (Session*)session;
+    return S ? S->live : -2;
+}
+
+int cgot_session_destroy(void* session)
+{
+    CG_API_BEGIN
+    Session* S = (Session*)session;
+    if (S) {
+        (void)hipStreamSynchronize(S->stream);
+        delete S;
+    }
     return 0;
     CG_API_END
 }

@@ -114,6 +114,41 @@ int cgot_set_sparse_solve(int32_t mode);
 int cgot_splr_solve(int32_t B, const cgot_splr_problem* probs, void* workspace, size_t workspace_bytes,
                     void* stream, cgot_splr_info* info);
 
+/* Sessions: continuous batching of SPLR solves.  No counterpart in the
+ * reference (its splr_ot_batch is a sequential loop, curegot/autograd.py:340-367);
+ * for request loops that admit new problems while others are iterating.
+ * A session is one persistent lockstep group on `stream`: cgot_session_add
+ * queues a problem, which starts at the next cgot_session_step; each step runs
+ * up to `iters` SPLR iterations of every instance in the group and returns the
+ * instances that stopped in it (their plan and dual written, tags and infos in
+ * the output arrays, slots freed).  Each instance runs exactly the iterations
+ * of a solo cgot_splr_solve and its results are bitwise the same.
+ *   - Limits, fixed at creation: at most `capacity` instances at once, with
+ *     sum of (n + m) <= max_sum_nm, sum of cgot_session_inst_tiles(n, m) <=
+ *     max_tiles and ceil(min(n, m-1) / 64) <= max_nblk each; cgot_session_add
+ *     returns -3 (and keeps nothing) while the group is full.
+ *   - Memory: the session workspace (cgot_session_workspace_size) holds the
+ *     group's shared arrays; every instance brings a workspace of its own
+ *     (cgot_session_inst_workspace_size) that holds all of its state.  After
+ *     the instance stops, its info.factor and the other info pointers refer to
+ *     that workspace (the backward context): keep it alive while they are used.
+ *   - One thread at a time per session; the session's work is ordered on
+ *     `stream`, and a step returns with it complete.
+ *   - A geometric-cost instance and a stored-M instance cannot be in one group. */
+size_t cgot_session_workspace_size(int32_t capacity, int64_t max_sum_nm, int64_t max_tiles, int32_t max_nblk);
+size_t cgot_session_inst_workspace_size(const cgot_splr_problem* p);
+int64_t cgot_session_inst_tiles(int32_t n, int32_t m);
+int cgot_session_create(int32_t capacity, int64_t max_sum_nm, int64_t max_tiles, int32_t max_nblk, void* workspace,
+                        size_t workspace_bytes, void* stream, void** session);
+/* Returns the slot (>= 0), -3 when the group has no room now, or an error. */
+int cgot_session_add(void* session, const cgot_splr_problem* p, void* inst_workspace, size_t inst_workspace_bytes,
+                     int64_t tag);
+int cgot_session_step(void* session, int32_t iters, int64_t* tags_out, cgot_splr_info* info_out, int32_t out_cap,
+                      int32_t* ndone);
+/* Instances in the group (added and not yet returned by a step). */
+int cgot_session_live(void* session);
+int cgot_session_destroy(void* session);
+
 /* Implicit backward of the SPLR plan w.r.t. M for one instance of a batch:
  * solve Q(T*) lambda = mu by PCG (T* recomputed from M and the duals),
  * then dL/dM = T*.(lambda_a 1^T + 1 lambda_b^T - G)/reg.  The same lambda
