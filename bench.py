@@ -114,7 +114,9 @@ def main():
                     help="requests of the stream also solved one at a time (--serve)")
     ap.add_argument("--rate", type=float, default=2000.0, help="arrival rate of the serving trace (req/s)")
     ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--max-batch", type=int, default=256)
+    ap.add_argument("--step-iters", type=int, default=3, help="serving: iterations between admissions")
+    ap.add_argument("--serve-batch-mode", action="store_true", help="serving: admission per batch")
     args = ap.parse_args()
     if args.newton is not None:
         from curegot_backward_amd import _lib as _l
@@ -324,10 +326,12 @@ def serve_leg(args, seq_n=None):
     cells = sum(q[0].numel() for q in reqs)
     log(f"cells {cells / 1e6:.1f}M, {sum(1 for q in reqs if q[4] is not None)} with backward")
 
-    with OTServer(slots=args.slots, max_batch=args.max_batch, tol=1e-8) as srv:
+    kw = dict(slots=args.slots, max_batch=args.max_batch, tol=1e-8, continuous=not args.serve_batch_mode,
+              step_iters=args.step_iters)
+    with OTServer(**kw) as srv:
         for q in reqs[: min(4, len(reqs))]:           # warm-up (allocator, kernels)
             srv.submit(*q).result()
-    with OTServer(slots=args.slots, max_batch=args.max_batch, tol=1e-8) as srv:
+    with OTServer(**kw) as srv:
         t0 = time.perf_counter()
         futs = []
         for r, q in zip(trace, reqs):
@@ -367,6 +371,7 @@ def serve_leg(args, seq_n=None):
                     "{0.01,0.05,0.1}, 30% forward+backward (G ~ N(0,1)), tol 1e-8",
         "requests": len(outs), "arrival_span_s": trace[-1]["t"], "makespan_s": el, "cells": cells,
         "slots": args.slots, "max_batch": args.max_batch, "converged": conv, "batches": st["batches"],
+        "admission": "per batch" if args.serve_batch_mode else f"continuous (every {args.step_iters} iterations)",
         "latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "max": st["max_ms"],
                        "queue_p50": st["queue_p50_ms"], "queue_p99": st["queue_p99_ms"]},
         "sequential": {"solves_per_s": seq, "requests": seq_n, "p50_ms": float(np.percentile(lat, 50)),

@@ -1,19 +1,28 @@
 """Continuous-batching request loop for OT solves on one GPU.
 
-Requests (M, a, b, eta[, G]) arrive at any time.  A fixed number of slots
-(worker threads, each with its own HIP stream) take work from one FIFO queue:
-a slot that becomes free opens a batch with the oldest waiting request and
-adds the later waiting requests of the same size class (the log2 of the Schur
-block count, which sets how long one lockstep iteration takes), in arrival
-order, while their HBM footprint (solver workspace, plan, dual and, for a
-backward request, the gradient and the adjoint workspace) fits in the memory
-the other slots leave free.  A batch is one
-ragged native solve (every instance advances in lockstep inside it), followed
-by one batched adjoint solve for the requests that carry an upstream gradient
-G.  Requests that arrive while other batches are on the GPU are admitted by
-the next slot that frees up, so their work overlaps on the device.  Every
-result is bitwise the same as solving the request alone: the kernels are
-deterministic per instance.
+Requests (M, a, b, eta[, G]) arrive at any time.  Two admission modes:
+
+continuous (default)
+    Each forward slot (worker thread + HIP stream) owns a session
+    (curegot_backward_amd._native.SplrSession): one persistent lockstep group
+    into which waiting requests of the slot's size classes (the log2 of the
+    Schur block count; larger classes go to lower slots, so that one slow
+    instance does not set the pace of many fast ones) are admitted at every
+    iteration boundary while the group's limits and the HBM budget allow, and
+    from which each request leaves as soon as it has converged.  With more
+    than one slot, the last one is an adjoint worker that runs the backward
+    requests' implicit adjoints, batched, as their forwards finish.
+batch (continuous=False)
+    A free slot opens a batch with the oldest request and adds later requests
+    of the same size class; the batch is one ragged native solve, followed by
+    one batched adjoint for the requests with G, and holds the slot until its
+    slowest member finishes.
+
+In both, admission is by HBM footprint (solver workspace, plan, dual and, for
+a backward request, the gradient and the adjoint workspace) against a budget,
+requests that arrive while others run are admitted as slots and memory free
+up, so their work overlaps on the device, and every result is bitwise the same
+as solving the request alone: the kernels are deterministic per instance.
 
 The reference has no serving layer (its splr_ot_batch is a Python loop over
 instances, curegot/autograd.py:340-367); this module is built on the same
@@ -33,6 +42,7 @@ import numpy as np
 import torch
 
 from . import _lib, _native
+from ._native import SplrSession
 from .backward import solve_backward_batch
 
 
@@ -74,12 +84,16 @@ class _Request:
 class OTServer:
     """Continuous batching of SPLR-OT requests.
 
-    slots:        concurrent batches (each on its own stream; the device has a
-                  few hardware queues, so 2-4 is the useful range)
-    hbm_budget:   bytes the in-flight batches may use together (default: 80%
+    slots:        worker streams (a process has GPU_MAX_HW_QUEUES hardware
+                  queues, 4 by default, so 2-4 is the useful range; more need
+                  more queues).  Continuous mode: slots - 1 forward slots (at
+                  most 7) plus one adjoint worker (one slot does both)
+    hbm_budget:   bytes the in-flight requests may use together (default: 80%
                   of the device memory free at start-up)
-    max_batch:    largest number of requests in one batch
-    groups:       concurrent size classes inside one batch (see
+    max_batch:    largest number of requests in one lockstep group
+    continuous:   admission at iteration granularity (default) or per batch
+    step_iters:   continuous mode: iterations a group runs between admissions
+    groups:       batch mode: concurrent size classes inside one batch (see
                   cgot_set_max_groups; 1 keeps each batch on its slot's stream)
 
     The inputs of a request may still be in production on the submitting
@@ -87,17 +101,20 @@ class OTServer:
     submit() call.  submit() returns a concurrent.futures.Future whose result is
     a dict with
     plan, dual, niter, converged (and grad, lambda, bwd_iterations for a request
-    with G), plus queue_s (submit to start of its batch) and latency_s (submit
+    with G), plus queue_s (submit to start of its solve) and latency_s (submit
     to result).
     """
 
-    def __init__(self, device=0, slots: int = 4, hbm_budget: Optional[int] = None, max_batch: int = 64,
-                 tol: float = 1e-8, max_iter: int = 1000, groups: int = 1, preconditioner: str = "diag"):
+    NCLASS = 16
+
+    def __init__(self, device=0, slots: int = 4, hbm_budget: Optional[int] = None, max_batch: int = 256,
+                 tol: float = 1e-8, max_iter: int = 1000, groups: int = 1, preconditioner: str = "diag",
+                 continuous: bool = True, step_iters: int = 3):
         self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("OTServer needs a CUDA (HIP) device")
-        if slots < 1 or max_batch < 1:
-            raise ValueError("slots and max_batch must be >= 1")
+        if slots < 1 or max_batch < 1 or step_iters < 1:
+            raise ValueError("slots, max_batch and step_iters must be >= 1")
         if hbm_budget is None:
             free, _total = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(0.8 * free)
@@ -106,16 +123,30 @@ class OTServer:
         self.tol, self.max_iter = float(tol), int(max_iter)
         self.groups = int(groups)
         self.preconditioner = preconditioner
+        self.continuous = bool(continuous)
+        self.step_iters = int(step_iters)
         self._queue: collections.deque = collections.deque()
+        self._cq = [collections.deque() for _ in range(self.NCLASS)]   # continuous mode, per size class
+        self._bq: collections.deque = collections.deque()              # continuous mode, adjoint jobs
         self._cv = threading.Condition()
         self._in_use = 0
         self._closed = False
+        self._fwd_open = 0
         self._done: list = []          # (latency_s, queue_s, n, m, backward)
-        self._log: list = []           # (slot, size class, requests, start, end) per batch
+        self._log: list = []           # (slot, size class, requests, start, end) per batch or admission
         self._batches = 0
         self._t_first: Optional[float] = None
         self._t_last: Optional[float] = None
-        self._workers = [threading.Thread(target=self._run, args=(k,), daemon=True) for k in range(slots)]
+        if self.continuous:
+            self._nfwd = min(max(1, slots - 1) if slots > 1 else 1, 7)
+            self._adj_worker = slots > 1
+            self._fwd_open = self._nfwd
+            self._workers = [threading.Thread(target=self._run_session, args=(k,), daemon=True)
+                             for k in range(self._nfwd)]
+            if self._adj_worker:
+                self._workers.append(threading.Thread(target=self._run_adjoints, args=(self._nfwd,), daemon=True))
+        else:
+            self._workers = [threading.Thread(target=self._run, args=(k,), daemon=True) for k in range(slots)]
         for w in self._workers:
             w.start()
 
@@ -143,7 +174,10 @@ class OTServer:
                 req.future.set_exception(MemoryError(
                     f"request needs {fp} bytes of HBM, more than the server budget {self.hbm_budget}"))
                 return req.future
-            self._queue.append(req)
+            if self.continuous:
+                self._cq[min(self._size_class(n, m), self.NCLASS - 1)].append(req)
+            else:
+                self._queue.append(req)
             self._cv.notify_all()
         return req.future
 
@@ -239,6 +273,187 @@ class OTServer:
             self._in_use -= used
             self._cv.notify_all()
 
+    # ------------------------------------------------------- continuous mode
+    # Lower class bounds of the forward slots (slot 0 takes the largest
+    # classes); a size class is log2 of the Schur block count, 0..7 for n, m
+    # up to 8192.  The mixed-size stream puts about half of its requests in
+    # classes 0-2 and a third in classes 3-4, while the work per request grows
+    # with the class.
+    _CLASS_BOUNDS = {1: [0], 2: [4, 0], 3: [5, 3, 0], 4: [6, 4, 2, 0], 5: [6, 5, 3, 2, 0], 6: [7, 6, 5, 3, 2, 0],
+                     7: [7, 6, 5, 4, 3, 2, 0]}
+
+    def _slot_classes(self, k: int) -> list:
+        """Size classes forward slot k admits (contiguous ranges, the largest
+        classes on slot 0)."""
+        lo = self._CLASS_BOUNDS[self._nfwd]
+        hi = [self.NCLASS] + lo[:-1]
+        return list(range(lo[k], hi[k]))
+
+    def _take_for_session(self, k: int, classes: list, room: int, empty: bool) -> list:
+        """Oldest waiting requests of `classes`, in arrival order, while the
+        group has room and their footprints fit the HBM budget (their memory
+        is reserved here).  Blocks while the group is empty and nothing waits;
+        returns None once the server is closed and nothing is left."""
+        with self._cv:
+            while True:
+                picked = []
+                while len(picked) < room:
+                    best = None
+                    for c in classes:
+                        q = self._cq[c]
+                        if q and (best is None or q[0].t_submit < best[0].t_submit):
+                            best = q
+                    if best is None or self._in_use + best[0].footprint > self.hbm_budget:
+                        break
+                    r = best.popleft()
+                    self._in_use += r.footprint
+                    picked.append(r)
+                if picked or not empty:
+                    if picked and self._t_first is None:
+                        self._t_first = min(r.t_submit for r in picked)
+                    return picked
+                if self._closed and not any(self._cq[c] for c in classes):
+                    return None
+                self._cv.wait()
+
+    def _put_back(self, r: "_Request"):
+        with self._cv:
+            self._cq[min(self._size_class(r.M.shape[0], r.M.shape[1]), self.NCLASS - 1)].appendleft(r)
+            self._in_use -= r.footprint
+            self._cv.notify_all()
+
+    def _complete(self, r: "_Request", out: dict, t_start: float, fail: Optional[BaseException] = None):
+        t_end = time.perf_counter()
+        if fail is not None:
+            r.future.set_exception(fail)
+        else:
+            out["queue_s"] = t_start - r.t_submit
+            out["latency_s"] = t_end - r.t_submit
+            # the outputs were allocated on a server stream: the caching
+            # allocator must not hand their blocks to later work there before
+            # the caller's stream has finished reading them
+            for key in ("plan", "dual", "grad", "lambda"):
+                t = out.get(key)
+                if torch.is_tensor(t):
+                    t.record_stream(r.stream)
+            r.future.set_result(out)
+        with self._cv:
+            self._in_use -= r.footprint
+            self._t_last = t_end
+            if fail is None:
+                self._done.append((t_end - r.t_submit, t_start - r.t_submit, r.M.shape[0], r.M.shape[1],
+                                   r.G is not None))
+            self._cv.notify_all()
+
+    @staticmethod
+    def _fwd_out(o: dict) -> dict:
+        return {"plan": o["plan"], "dual": o["dual"], "niter": o["niter"], "converged": o["converged"],
+                "valid": o["valid"]}
+
+    def _adjoint_batch(self, jobs: list, stream):
+        """Implicit adjoints of finished forwards [(request, forward result, t_start)]."""
+        ok = [(r, o, t) for (r, o, t) in jobs if o["context"] is not None]
+        for (r, o, t) in jobs:
+            if o["context"] is None:
+                self._complete(r, dict(self._fwd_out(o), grad=None, bwd_converged=False), t)
+        if not ok:
+            return
+        with torch.cuda.stream(stream):
+            for (r, _o, _t) in ok:
+                stream.wait_event(r.ready)
+            res = solve_backward_batch([(r.G, o["context"], r.eta) for (r, o, _t) in ok],
+                                       preconditioner=self.preconditioner)
+            stream.synchronize()
+        for (r, o, t), x in zip(ok, res):
+            self._complete(r, dict(self._fwd_out(o), grad=x["grad"], **{"lambda": x["lambda"]},
+                                   bwd_iterations=x["iterations"], bwd_converged=x["converged"]), t)
+
+    def _run_session(self, k: int):
+        """Forward slot k: one session fed at every step boundary."""
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(self.device)          # default priority
+        classes = self._slot_classes(k)
+        sess = None
+        inflight = {}                                     # tag -> (request, t_start)
+        try:
+            while True:
+                if sess is None:
+                    sess = SplrSession(device=self.device, stream=stream, capacity=self.max_batch,
+                                       max_sum_nm=1 << 21, max_tiles=1 << 21, max_nblk=256)
+                picked = self._take_for_session(k, classes, self.max_batch - len(inflight), not inflight)
+                if picked is None:
+                    return
+                if picked:
+                    t0 = time.perf_counter()
+                    added = 0
+                    with torch.cuda.stream(stream):
+                        for i, r in enumerate(picked):
+                            stream.wait_event(r.ready)
+                            tag = id(r)
+                            if not sess.add(r.M, r.a, r.b, r.eta, self.tol, self.max_iter, tag=tag):
+                                for rr in picked[i:]:
+                                    self._put_back(rr)
+                                break
+                            inflight[tag] = (r, t0)
+                            added += 1
+                    if added:
+                        with self._cv:
+                            self._batches += 1
+                            self._log.append((k, self._size_class(picked[0].M.shape[0], picked[0].M.shape[1]),
+                                              added, t0, t0))
+                if not inflight:
+                    continue
+                try:
+                    with torch.cuda.stream(stream):
+                        done = sess.step(self.step_iters)
+                except Exception as e:                    # pragma: no cover - surfaced to the callers
+                    for (r, t) in inflight.values():
+                        self._complete(r, {}, t, fail=e)
+                    inflight.clear()
+                    sess.close()
+                    sess = None
+                    continue
+                bwd = []
+                for tag, o in done:
+                    r, t = inflight.pop(tag)
+                    if r.G is None:
+                        self._complete(r, self._fwd_out(o), t)
+                    else:
+                        bwd.append((r, o, t))
+                if bwd:
+                    if self._adj_worker:
+                        with self._cv:
+                            self._bq.extend(bwd)
+                            self._cv.notify_all()
+                    else:
+                        self._adjoint_batch(bwd, stream)
+        finally:
+            if sess is not None:
+                sess.close()
+            with self._cv:
+                self._fwd_open -= 1
+                self._cv.notify_all()
+
+    def _run_adjoints(self, k: int):
+        """Adjoint worker: the backward requests' adjoints, batched as their
+        forwards finish."""
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(self.device)          # default priority
+        while True:
+            with self._cv:
+                while not self._bq and self._fwd_open > 0:
+                    self._cv.wait()
+                if not self._bq:
+                    return
+                jobs = [self._bq.popleft() for _ in range(min(len(self._bq), self.max_batch))]
+            try:
+                self._adjoint_batch(jobs, stream)
+            except Exception as e:                        # pragma: no cover - surfaced to the callers
+                for (r, _o, t) in jobs:
+                    if not r.future.done():
+                        self._complete(r, {}, t, fail=e)
+
+    # ------------------------------------------------------------ batch mode
     def _run(self, k: int):
         torch.cuda.set_device(self.device)
         stream = torch.cuda.Stream(self.device)          # default priority

@@ -25,11 +25,14 @@ def _problems(k, seed):
     return reqs
 
 
-def test_served_results_equal_solo():
+@pytest.mark.parametrize("continuous", [True, False])
+def test_served_results_equal_solo(continuous):
+    """Both admission modes: at iteration boundaries into running groups
+    (continuous) and per batch."""
     import curegot_backward_amd as cg
     from curegot_backward_amd.serving import OTServer
     reqs = _problems(18, 7)
-    with OTServer(slots=3, max_batch=5, tol=1e-8) as srv:
+    with OTServer(slots=3, max_batch=5, tol=1e-8, continuous=continuous) as srv:
         futs = []
         for i, (M, a, b, eta, G) in enumerate(reqs):
             futs.append(srv.submit(M, a, b, eta, G))
