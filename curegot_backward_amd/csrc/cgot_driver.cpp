@@ -253,6 +253,7 @@ struct Batch {
     int ntiles_full = 0, nvec_full = 0, nrow_full = 0;
     long long packed_wgs = 0;  // workgroups of the current maps
     double* d_ctl = nullptr;   // kCtlSlots * B * CG_NCTL
+    double* d_fixctl = nullptr;   // B * CG_NCTL, a control block at a fixed address (captured graphs)
     double* d_scal = nullptr;  // B * CG_NSCAL
     int* d_pref = nullptr;     // dense prefix tables
     int pref_cap = 0;
@@ -291,6 +292,7 @@ struct Batch {
         d_full = bp.take<int>(std::max<size_t>(nmap, 2));
         d_live = bp.take<int>(std::max<size_t>(nmap, 2));
         d_ctl = bp.take<double>((size_t)kCtlSlots * B * CG_NCTL);
+        d_fixctl = bp.take<double>((size_t)B * CG_NCTL);
         d_scal = bp.take<double>((size_t)B * CG_NSCAL);
         pref_cap = (4 * nblk_max + 3) * (B + 1) + nblk_max + 1;
         d_pref = bp.take<int>(pref_cap);
@@ -444,6 +446,19 @@ struct Batch {
         CG_HIP(hipMemcpyAsync(ds, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
         slot++;
         return ds;
+    }
+
+    // Upload a control block to the fixed-address block (staged through the
+    // ring like ctl()).
+    const double* ctl_fixed(const std::vector<double>& c)
+    {
+        if (slot == kCtlSlots) sync();
+        double* hs = hctl + (size_t)slot * B * CG_NCTL;
+        std::memcpy(hs, c.data(), sizeof(double) * B * CG_NCTL);
+        hmap[d_fixctl] = hs;
+        CG_HIP(hipMemcpyAsync(d_fixctl, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
+        slot++;
+        return d_fixctl;
     }
 
     void sync()
@@ -1616,6 +1631,23 @@ struct PcSolver {
     void lowrank_apply(const double* ctlp) { bt.vec0(k_pc_wapply, ctlp); }
 };
 
+// A stream of the calling thread that only ever records captures.
+hipStream_t capture_stream()
+{
+    thread_local hipStream_t s = nullptr;
+    thread_local int gen = -1;
+    if (gen != g_res_gen.load()) { s = nullptr; gen = g_res_gen.load(); }
+    if (!s) {
+        CG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        {
+            std::lock_guard<std::mutex> lk(g_res_mu);
+            g_res_streams.push_back(s);
+        }
+        register_atexit();
+    }
+    return s;
+}
+
 void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info* info)
 {
     const int B = bt.B;
@@ -1645,19 +1677,73 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         running[b] = (bt.sc(b, PCG_STATUS) == PCG_RUNNING) && probs[b].max_iter > 0;
         nrun += running[b];
     }
+    // One PCG step of the running instances (status words on the device stop
+    // the others): Q p with T recomputed, its finish, the update, the
+    // preconditioner.
+    auto pcg_step = [&](const double* d2) {
+        ProfScope ps(bt.stream, PROF_QV, bt.mpass_bytes(bt.hmap[d2], 24.0));
+        hipLaunchKernelGGL(bt.xy ? k_qv_pcg<true> : k_qv_pcg<false>, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream,
+                           bt.d_insts, bt.d_tile, d2);
+        bt.vec0(k_pcg_q_fin, d2);
+        bt.vec0(k_pcg_update, d2);
+        precondition(d2, 0);
+    };
+    // Chunks of kGraphChunk steps (the loop's steady state) run as a captured
+    // graph: the same launches with the same arguments every time (the
+    // control at a fixed address), replayed until the maps change.  Not with
+    // dense stored factors, whose preconditioner passes a per-application
+    // epoch, nor under the profiler's per-launch events.
+    constexpr int kGraphChunk = 32;
+    const bool graphs = pc.ord.empty() && !g_prof;
+    hipGraphExec_t gexec = nullptr;
+    const int* g_tile = nullptr;
+    int g_ntiles = -1, g_nvec = -1, g_nrow = -1;
+    struct GraphGuard {
+        hipGraphExec_t& e;
+        ~GraphGuard() { if (e) (void)hipGraphExecDestroy(e); }
+    } gguard{gexec};
     int chunk = 4;
     while (nrun > 0) {
         bt.pack_live(running);
         auto c2 = c;
         for (int b = 0; b < B; b++) c2[(size_t)b * CG_NCTL + CTL_ACTIVE] = running[b] ? 1.0 : 0.0;
-        const double* d2 = bt.ctl(c2);
-        for (int k = 0; k < chunk; k++) {
-            ProfScope ps(bt.stream, PROF_QV, bt.mpass_bytes(bt.hmap[d2], 24.0));
-            hipLaunchKernelGGL(bt.xy ? k_qv_pcg<true> : k_qv_pcg<false>, dim3(bt.ntiles), dim3(CG_WG), 0, bt.stream,
-                               bt.d_insts, bt.d_tile, d2);
-            bt.vec0(k_pcg_q_fin, d2);
-            bt.vec0(k_pcg_update, d2);
-            precondition(d2, 0);
+        if (graphs && chunk == kGraphChunk) {
+            const double* d2 = bt.ctl_fixed(c2);
+            if (gexec && (g_tile != bt.d_tile || g_ntiles != bt.ntiles || g_nvec != bt.nvec || g_nrow != bt.nrow)) {
+                CG_HIP(hipGraphExecDestroy(gexec));
+                gexec = nullptr;
+            }
+            if (!gexec) {
+                bt.check_launch();
+                // the null stream cannot be captured: capture on a side
+                // stream (it runs nothing), launch on the solve stream
+                const hipStream_t keep = bt.stream;
+                const hipStream_t cs = keep ? keep : capture_stream();
+                hipGraph_t g = nullptr;
+                CG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                bt.stream = cs;
+                try {
+                    for (int k = 0; k < chunk; k++) pcg_step(d2);
+                } catch (...) {
+                    bt.stream = keep;
+                    if (hipStreamEndCapture(cs, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                bt.stream = keep;
+                const hipError_t ce = hipStreamEndCapture(cs, &g);
+                CG_HIP(ce);
+                const hipError_t ie = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                CG_HIP(ie);
+                g_tile = bt.d_tile;
+                g_ntiles = bt.ntiles;
+                g_nvec = bt.nvec;
+                g_nrow = bt.nrow;
+            }
+            CG_HIP(hipGraphLaunch(gexec, bt.stream));
+        } else {
+            const double* d2 = bt.ctl(c2);
+            for (int k = 0; k < chunk; k++) pcg_step(d2);
         }
         bt.check_launch();
         bt.read_scal();
