@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile rule instead of running the tile autotuner first")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--groups", type=int, default=4, help="concurrent size classes of the batched solve (1-4)")
     ap.add_argument("--newton", choices=("auto", "sparse", "dense"), default=None,
@@ -163,6 +165,16 @@ def main():
 
     def batched():
         return cg.sinkhorn_splr_bwd_batch(Ms, As, Bs, etas, 1e-8, 1000)
+
+    tuned = None
+    if not args.no_tune:
+        # tile shapes of the M passes picked on this GPU before anything is timed
+        from curegot_backward_amd import autotune
+        t = time.perf_counter()
+        tuned = autotune.tune(log=log)
+        autotune.apply(tuned)
+        tuned["tune_s"] = time.perf_counter() - t
+        log(f"tile table {tuned['table']} ({tuned['tune_s']:.1f} s)")
 
     _lib.set_max_groups(args.groups)
     for w in range(args.warmup):
@@ -268,6 +280,10 @@ def main():
         "batched_latency_ms": {"p50": batch_ms, "p99": batch_ms,
                                "note": "one synchronous call returns every instance of the batch together"},
         "kernel_times_profiled_solve_ms": {k: v[0] for k, v in prof.items() if v[2]},
+        "tile_table": ({"rows_m_max_cols": tuned["table"], "picked_by": "curegot_backward_amd.autotune.tune "
+                        "(short batched solves per m band, each admissible columns-per-lane, best of 3)",
+                        "bands": tuned["bands"], "tune_s": tuned["tune_s"]} if tuned else
+                       {"rows_m_max_cols": [], "picked_by": "default rule (--no-tune)"}),
     }
 
     if not args.no_sequential:

@@ -101,6 +101,7 @@ class PrecondProfile(ctypes.Structure):
 
 EXPORTS = (
     "cgot_splr_workspace_size", "cgot_splr_solve", "cgot_set_max_groups", "cgot_set_sparse_solve",
+    "cgot_set_tile_table", "cgot_get_tile_table", "cgot_tile_cols",
     "cgot_adjoint_workspace_size", "cgot_adjoint_solve",
     "cgot_splr_precond_workspace_size", "cgot_splr_precond_apply", "cgot_splr_precond_apply_profiled",
     "cgot_splr_precond_apply_noalloc",
@@ -137,6 +138,12 @@ def load():
     L.cgot_set_sparse_solve.restype = ctypes.c_int
     L.cgot_set_max_groups.argtypes = [i32]
     L.cgot_set_max_groups.restype = ctypes.c_int
+    L.cgot_set_tile_table.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
+    L.cgot_set_tile_table.restype = ctypes.c_int
+    L.cgot_get_tile_table.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
+    L.cgot_get_tile_table.restype = ctypes.c_int
+    L.cgot_tile_cols.argtypes = [i32]
+    L.cgot_tile_cols.restype = ctypes.c_int
     L.cgot_adjoint_workspace_size.argtypes = [i32, ctypes.POINTER(AdjointProblem)]
     L.cgot_adjoint_workspace_size.restype = sz
     L.cgot_adjoint_solve.argtypes = [i32, ctypes.POINTER(AdjointProblem), vp, sz, vp,
@@ -195,6 +202,12 @@ def load():
     L.cgot_last_error.restype = ctypes.c_char_p
     L.cgot_version.restype = ctypes.c_char_p
     _lib = L
+    path = os.environ.get("CUREGOT_AMD_TILE_TABLE")
+    if path:
+        # a saved autotuner result (JSON with a "table" of [m_max, c] rows)
+        import json
+        with open(path) as f:
+            set_tile_table(json.load(f)["table"])
     return L
 
 
@@ -215,6 +228,35 @@ def set_max_groups(groups: int):
     from the calling thread is split into (1..4; 1 = one lockstep group on the
     caller's stream)."""
     check(load().cgot_set_max_groups(int(groups)), "set_max_groups")
+
+
+def set_tile_table(rows):
+    """Columns per lane of the M-pass tiles by m, process-wide: rows
+    [(m_max, c), ...] with ascending m_max and c in {1, 2, 4}; row k covers the
+    m <= m_max not covered before it, larger m take the default rule.  An
+    empty list restores the default everywhere."""
+    rows = [(int(mx), int(c)) for mx, c in rows]
+    arr_m = (ctypes.c_int32 * max(1, len(rows)))(*[r[0] for r in rows])
+    arr_c = (ctypes.c_int32 * max(1, len(rows)))(*[r[1] for r in rows])
+    check(load().cgot_set_tile_table(arr_m, arr_c, len(rows)), "set_tile_table")
+
+
+def get_tile_table():
+    """The current tile table as [(m_max, c), ...] (empty: the default rule)."""
+    L = load()
+    cnt = L.cgot_get_tile_table(None, None, 0)
+    arr_m = (ctypes.c_int32 * max(1, cnt))()
+    arr_c = (ctypes.c_int32 * max(1, cnt))()
+    L.cgot_get_tile_table(arr_m, arr_c, cnt)
+    return [(int(arr_m[k]), int(arr_c[k])) for k in range(cnt)]
+
+
+def tile_cols(m: int) -> int:
+    """Columns per lane an instance with m columns is tiled with now."""
+    c = load().cgot_tile_cols(int(m))
+    if c < 1:
+        raise ValueError("m must be >= 1")
+    return c
 
 
 SPARSE_MODES = {"dense": 0, "auto": 1, "sparse": 2}

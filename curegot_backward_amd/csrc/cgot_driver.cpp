@@ -192,19 +192,37 @@ void prof_collect()
 
 int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// Columns per lane of the M-pass tiles (a strip is 64c columns wide).  The
+// default: the next power of two >= ceil(m/64), at most CG_CMAX.
+int default_cols(int m)
+{
+    const int need = std::max(1, cdiv(m, 64));
+    int c = 1;
+    while (c < need && c < CG_CMAX) c *= 2;
+    return c;
+}
+
+// The tile table, process-wide: rows {m_max, c} by ascending m_max, row k
+// covering the m <= m_max not covered by an earlier row; m past the last row
+// takes the default.  Written by cgot_set_tile_table (the autotuner's choice).
+std::mutex g_tile_mu;
+std::vector<std::pair<int, int>> g_tile_table;
+
+int tile_cols(int m)
+{
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    for (const auto& r : g_tile_table)
+        if (m <= r.first) return r.second;
+    return default_cols(m);
+}
+
 // Geometry common to every instance kind.
 void set_geometry(CgInst& I, int n, int m, double reg)
 {
     I.n = n;
     I.m = m;
     I.N = n + m - 1;
-    {
-        // columns per lane: the next power of two >= ceil(m/64), at most CG_CMAX
-        const int need = std::max(1, cdiv(m, 64));
-        int c = 1;
-        while (c < need && c < CG_CMAX) c *= 2;
-        I.c = c;
-    }
+    I.c = tile_cols(m);
     I.sw = 64 * I.c;
     I.nstrips = cdiv(m, I.sw);
     I.nrb = cdiv(n, CG_RB);
@@ -2659,6 +2677,42 @@ int cgot_set_sparse_solve(int32_t mode)
     if (mode < 0 || mode > 2) return -1;
     return g_sparse_on.exchange(mode);
 }
+
+int cgot_set_tile_table(const int32_t* m_max, const int32_t* cols, int32_t count)
+{
+    if (count < 0 || count > 64 || (count > 0 && (!m_max || !cols))) {
+        g_last_error = "tile table: count must be in [0, 64] with non-null rows";
+        return -2;
+    }
+    std::vector<std::pair<int, int>> t;
+    for (int k = 0; k < count; k++) {
+        if (m_max[k] < 1 || (k > 0 && m_max[k] <= m_max[k - 1])) {
+            g_last_error = "tile table: m_max must be positive and strictly ascending";
+            return -2;
+        }
+        if (cols[k] != 1 && cols[k] != 2 && cols[k] != 4) {
+            g_last_error = "tile table: columns per lane must be 1, 2 or 4";
+            return -2;
+        }
+        t.emplace_back(m_max[k], cols[k]);
+    }
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_table = std::move(t);
+    return 0;
+}
+
+int cgot_get_tile_table(int32_t* m_max, int32_t* cols, int32_t cap)
+{
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const int cnt = (int)g_tile_table.size();
+    for (int k = 0; k < cnt && k < cap; k++) {
+        if (m_max) m_max[k] = g_tile_table[k].first;
+        if (cols) cols[k] = g_tile_table[k].second;
+    }
+    return cnt;
+}
+
+int cgot_tile_cols(int32_t m) { return m < 1 ? -1 : tile_cols(m); }
 
 int cgot_set_max_groups(int32_t groups)
 {

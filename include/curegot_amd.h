@@ -101,6 +101,23 @@ size_t cgot_splr_workspace_size(int32_t B, const cgot_splr_problem* probs);
  * calling thread; 1 runs the whole batch in one lockstep group on `stream`.  No counterpart in the reference, whose
  * splr_ot_batch loops over instances (curegot/autograd.py:340-367). */
 int cgot_set_max_groups(int32_t groups);
+/* Tile shapes of the M passes (T pass, log-sum-exp passes, top-K selection,
+ * adjoint Q v): an instance with m columns is cut into strips of 64c columns,
+ * c = 1, 2 or 4 columns per lane.  Row k of the table covers the m <= m_max[k]
+ * not covered by an earlier row (m_max strictly ascending, at most 64 rows);
+ * m past the last row take the default c, the smallest power of two with
+ * 64c >= m, at most 4.  count 0 restores the default everywhere.  Process-wide,
+ * read when a solve lays out its instances.  c fixes the summation order of the
+ * row sums: results are bitwise reproducible under one table, and a batch and
+ * the single solves of its instances agree bitwise under the same table.  The
+ * table is what the autotuner (curegot_backward_amd.autotune) picks.  No
+ * counterpart in the reference, whose kernels use fixed launch shapes.
+ * Returns 0, or -2 (cgot_last_error) for bad rows. */
+int cgot_set_tile_table(const int32_t* m_max, const int32_t* cols, int32_t count);
+/* Copies up to cap rows of the current table; returns its row count. */
+int cgot_get_tile_table(int32_t* m_max, int32_t* cols, int32_t cap);
+/* Columns per lane an instance with m columns takes now (-1 for m < 1). */
+int cgot_tile_cols(int32_t m);
 /* Newton step of the SPLR iteration: how A is factored.  mode 0: the dense
  * Cholesky of the Schur complement for every instance; mode 2: the sparse
  * factorization (host minimum-degree analysis once per pattern,
