@@ -1098,6 +1098,12 @@ struct SplrSolver {
                 on[b] = 1;
             }
         }
+        bool anydense = false;
+        for (int b = 0; b < B; b++) anydense = anydense || on[b];
+        // both kinds in the group: the sparse blocks share one launch with the
+        // Schur rows (k_newton_mixed) instead of running before the dense
+        // chain; the library profiler keeps them apart to time each
+        const bool mixed = !ids.empty() && anydense && !g_prof;
         if (!ids.empty()) {
             // ids staged through a pinned buffer: the previous copy out of it
             // completed at the last iteration's read-back
@@ -1105,12 +1111,12 @@ struct SplrSolver {
             int* hid = (int*)pinned_ids.get(ids.size() / 2 + 1);
             std::memcpy(hid, ids.data(), sizeof(int) * ids.size());
             CG_HIP(hipMemcpyAsync(bt.d_ids, hid, sizeof(int) * ids.size(), hipMemcpyHostToDevice, bt.stream));
-            ProfScope ps(bt.stream, PROF_SPNEWTON, sp_work);
-            hipLaunchKernelGGL(k_sp_newton, dim3((unsigned)ids.size()), dim3(SP_WG), sp_lds_bytes(rmax_used),
-                               bt.stream, bt.d_insts, bt.d_ids);
+            if (!mixed) {
+                ProfScope ps(bt.stream, PROF_SPNEWTON, sp_work);
+                hipLaunchKernelGGL(k_sp_newton, dim3((unsigned)ids.size()), dim3(SP_WG), sp_lds_bytes(rmax_used),
+                                   bt.stream, bt.d_insts, bt.d_ids);
+            }
         }
-        bool anydense = false;
-        for (int b = 0; b < B; b++) anydense = anydense || on[b];
         if (!anydense) return;
         const double* dc = bt.ctl(mask(on));
         int nblk_max = 0;
@@ -1171,8 +1177,16 @@ struct SplrSolver {
         for (int b = 0; b < B; b++)
             if (on[b] && bt.h[b].P <= CG_SCHUR_LDS) pcap = std::max(pcap, bt.h[b].P);
         bt.sub_rows(on);
-        bt.rows_sub(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
-        bt.rows_sub(k_schur_rhs, dc);
+        if (mixed) {
+            const size_t lds = std::max(sp_lds_bytes(rmax_used), sizeof(double) * (SP_WG / 64) * (size_t)pcap);
+            const int per = SP_WG / CG_WG;   // wave-map entries per block
+            const int nb = (int)ids.size() + cdiv(2LL * bt.nsub, per);
+            hipLaunchKernelGGL(k_newton_mixed, dim3(nb), dim3(SP_WG), lds, bt.stream, bt.d_insts, bt.d_ids,
+                               (int)ids.size(), bt.d_sub, bt.nsub, dc);
+        } else {
+            bt.rows_sub(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
+            bt.rows_sub(k_schur_rhs, dc);
+        }
         if (nblk_max > 0 && total(1 + 4 * nblk_max) > 0)
             hipLaunchKernelGGL(k_chol_diag0, dim3(total(1 + 4 * nblk_max)), dim3(CG_WG), 0, bt.stream, bt.d_insts, B,
                                dtab(1 + 4 * nblk_max));
@@ -2083,6 +2097,7 @@ int sp_rmax()
         // a launch above what the device allows fails at the launch check
         (void)hipFuncSetAttribute((const void*)k_sp_newton, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         (void)hipFuncSetAttribute((const void*)k_sp_precond, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_newton_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         (void)hipGetLastError();
         return R;
     }();

@@ -86,6 +86,8 @@ __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, i
 // sparse Newton solve (one workgroup of SP_WG threads per listed instance,
 // dynamic LDS sp_lds_bytes(R) of the largest core)
 __global__ void k_sp_newton(const CgInst*, const int* ids);
+__global__ void k_newton_mixed(const CgInst*, const int* ids, int nsp, const int* sub_map, int nsub,
+                               const double* ctl);   // k_sp_newton blocks, then Schur build / rhs rows
 __global__ void k_sp_precond(const CgInst*, const int* ids);
 // pattern export / plan import through pinned host memory (grid = B, and
 // B * SP_IMPORT_WGS)

@@ -29,6 +29,7 @@
 // Every sum runs in the fixed order of its list, so the result depends only
 // on the instance's own data.
 #include "cgot_kernels.h"
+#include "cgot_schur_rows.h"
 
 namespace {
 
@@ -605,14 +606,10 @@ CG_DEV void periphery_bwd(const SpView& v, CG_AS1 double* x0, CG_AS1 double* x1,
     }
 }
 
-}  // namespace
-
-// Newton step of the instances ids[0 .. gridDim.x): factor A at the current
-// values (Da, Db, w_rval) and solve A [dir, uv] = [grad, yv].
-__global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ insts, const int* __restrict__ ids)
+// Newton step of instance b (one workgroup of SP_WG threads): factor A at the
+// current values (Da, Db, w_rval) and solve A [dir, uv] = [grad, yv].
+CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_lds)
 {
-    extern __shared__ double sp_lds[];
-    const int b = ids[blockIdx.x];
     const CgInstD& I = cg_inst(insts, b);
     const SpView v = sp_view(I.sph, I.S);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N, t = v.t, R = v.R, T = v.T;
@@ -788,6 +785,58 @@ __global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ 
     if (!ok) I.scal[CS_PIVOT] = 1.0;
     periphery_bwd<2>(v, I.dir, I.uv, yx);
     if (acc == 0x9e3779b9u && N < 0) yx[0] = 0.0;   // never true: keeps the prefetch loads
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ insts, const int* __restrict__ ids)
+{
+    extern __shared__ double sp_lds[];
+    sp_newton_block(insts, ids[blockIdx.x], sp_lds);
+}
+
+// The Newton step of a group whose instances split between the sparse and
+// the dense factorization, in one launch so that the two overlap: blocks
+// [0, nsp) factor and solve the sparse instances ids[block] (as k_sp_newton);
+// every later block runs four entries of the dense instances' wave map
+// (CG_WAVES rows each, one 256-thread group per entry): entries [0, nsub)
+// build Schur rows (as k_schur_build), entries [nsub, 2 nsub) form the Schur
+// right-hand sides (as k_schur_rhs).  The parts read the iteration's A values
+// and write disjoint outputs; each row is computed exactly as in its own
+// kernel.  Dynamic LDS: the larger of the sparse blocks' need and SP_WG / 64
+// row buffers of the largest P <= CG_SCHUR_LDS among the dense instances.
+// The sparse blocks of k_newton_mixed: out of line, so that the kernel's
+// Schur branch does not change the factorization's register allocation.
+__device__ __attribute__((noinline)) void sp_newton_call(const CgInst* __restrict__ insts, int b, double* lds)
+{
+    sp_newton_block(insts, b, lds);
+}
+
+// The Schur rows of one wave-map entry of k_newton_mixed.
+CG_DEV void schur_rows_entry(const CgInst* __restrict__ insts, int nsp, const int* __restrict__ sub_map,
+                             int nsub, const double* __restrict__ ctl, double* lds)
+{
+    const int e = ((int)blockIdx.x - nsp) * (SP_WG / CG_WG) + (int)(threadIdx.x / CG_WG);
+    if (e >= 2 * nsub) return;
+    const int2 me = reinterpret_cast<const int2*>(sub_map)[e < nsub ? e : e - nsub];
+    const int b = me.x;
+    if (ctl[b * CG_NCTL + CTL_ACTIVE] == 0.0) return;
+    const CgInstD& I = cg_inst(insts, b);
+    const int s = me.y * CG_WAVES + (int)((threadIdx.x >> 6) % CG_WAVES);
+    if (s >= I.P) return;
+    if (e < nsub) cg_schur_build_row(I, s, lds + (size_t)(threadIdx.x >> 6) * I.P);
+    else cg_schur_rhs_row(I, s);
+}
+
+__global__ __launch_bounds__(SP_WG) void k_newton_mixed(const CgInst* __restrict__ insts, const int* __restrict__ ids,
+                                                        int nsp, const int* __restrict__ sub_map, int nsub,
+                                                        const double* __restrict__ ctl)
+{
+    extern __shared__ double sp_lds[];
+    if ((int)blockIdx.x < nsp)
+        sp_newton_call(insts, ids[blockIdx.x], sp_lds);
+    else
+        schur_rows_entry(insts, nsp, sub_map, nsub, ctl, sp_lds);
 }
 
 // Backward preconditioner of the adjoint instances ids[..] whose forward used
