@@ -412,15 +412,16 @@ CG_DEV void cg_lse_merge(double& mx, double& s, double mx2, double s2)
 // sum_{k < count} p[k*stride] in index order (the same rounding as a plain
 // loop), with CG_SUM_U loads in flight per round trip.
 #define CG_SUM_U 16
+template <int U = CG_SUM_U>
 CG_DEV double cg_ordered_sum(const CG_AS1 double* p, int count, size_t stride)
 {
     double s = 0.0;
-    for (int k = 0; k < count; k += CG_SUM_U) {
-        double t[CG_SUM_U];
+    for (int k = 0; k < count; k += U) {
+        double t[U];
 #pragma unroll
-        for (int u = 0; u < CG_SUM_U; u++) t[u] = (k + u < count) ? p[(size_t)(k + u) * stride] : 0.0;
+        for (int u = 0; u < U; u++) t[u] = (k + u < count) ? p[(size_t)(k + u) * stride] : 0.0;
 #pragma unroll
-        for (int u = 0; u < CG_SUM_U; u++)
+        for (int u = 0; u < U; u++)
             if (k + u < count) s += t[u];
     }
     return s;
@@ -522,22 +523,27 @@ CG_DEV void cg_sent_gather(unsigned long long (&u)[NQ], Addr addr, CG_AS1 double
     }
 }
 
-// Ordered sum of p[0..count) once the values have arrived, resetting every
-// slot to CG_SENT for the next use (CG_SUM_U loads in flight).
-CG_DEV double cg_ordered_take(CG_AS1 double* p, int count, CG_AS1 double* scal)
+// Ordered sum of the hand-off slots p[0..count) once their values have
+// arrived, every slot reset to CG_SENT for the next use; read by a whole wave
+// (every lane returns the sum), one slot per lane in flight, and folded in
+// index order from 0.0 through lane reads (the rounding of a plain loop).
+// Call with the whole wave active.
+CG_DEV double cg_ordered_take_wave(CG_AS1 double* p, int count, CG_AS1 double* scal)
 {
+    const int lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int k = 0; k < count; k += CG_SUM_U) {
-        unsigned long long t[CG_SUM_U];
-#pragma unroll
-        for (int u = 0; u < CG_SUM_U; u++) t[u] = (k + u < count) ? cg_sent_peek(p + k + u) : 0ull;
-        cg_sent_gather(t, [&](int u) { return p + k + u; }, scal);
-#pragma unroll
-        for (int u = 0; u < CG_SUM_U; u++)
-            if (k + u < count) {
-                s += __longlong_as_double((long long)t[u]);
-                cg_sent_reset(p + k + u);
-            }
+    for (int k = 0; k < count; k += 64) {
+        const bool ok = k + lane < count;
+        unsigned long long t[1] = {ok ? cg_sent_peek(p + k + lane) : 0ull};
+        cg_sent_gather(t, [&](int) { return p + k + lane; }, scal);
+        if (ok) cg_sent_reset(p + k + lane);
+        const int lo = (int)(unsigned)t[0], hi = (int)(unsigned)(t[0] >> 32);
+        const int cnt = min(64, count - k);
+        for (int j = 0; j < cnt; j++) {
+            const unsigned long long w = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, j) << 32) |
+                                         (unsigned)__builtin_amdgcn_readlane(lo, j);
+            s += __longlong_as_double((long long)w);
+        }
     }
     return s;
 }
@@ -565,7 +571,10 @@ CG_DEV bool cg_inst_reduce(const CgInstD& I, int wgi, double (&v)[NV], double (&
     }
     __syncthreads();
     if (!is_last) return false;
-    if (threadIdx.x < NV) tot[threadIdx.x] = cg_ordered_take(I.wgpart + (size_t)threadIdx.x * I.nvec, I.nvec, I.scal);
+    for (int k = threadIdx.x >> 6; k < NV; k += CG_WAVES) {   // a wave per value
+        const double t = cg_ordered_take_wave(I.wgpart + (size_t)k * I.nvec, I.nvec, I.scal);
+        if ((threadIdx.x & 63) == 0) tot[k] = t;
+    }
     if (threadIdx.x == 0) __hip_atomic_store(&I.counter[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
 #pragma unroll

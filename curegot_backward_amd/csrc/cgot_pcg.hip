@@ -93,7 +93,7 @@ __global__ __launch_bounds__(CG_WG) void k_pcg_q_fin(const CgInst* __restrict__ 
         v[0] += p * w;
     }
     if (q < m - 1) {
-        const double a = cg_ordered_sum(I.cp0 + q, I.nrb, m);
+        const double a = cg_ordered_sum<32>(I.cp0 + q, I.nrb, m);
         const int k = n + q;
         const double p = I.sv[k] + beta * I.uv[k];
         const double w = a + I.colsum[q] * p;

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(CG_WG) void k_tpass_fin(const CgInst* __restrict__ 
         v[1] = p * I.ab[q];
     }
     if (q < m) {
-        const double cs = cg_ordered_sum(I.cp0 + q, I.nrb, m);
+        const double cs = cg_ordered_sum<32>(I.cp0 + q, I.nrb, m);
         I.colsum[q] = cs;
         const int k = n + q;
         if (k < N) {
