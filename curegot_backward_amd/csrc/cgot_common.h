@@ -162,7 +162,7 @@ CG_DEV void cg_cost_row(const CgInstD& I, int i, int s0, int lane, const bool (&
 {
     const auto Mi = I.M + (size_t)i * I.m + s0 + lane;
 #pragma unroll
-    for (int k = 0; k < C; k++) mv[k] = ok[k] ? Mi[64 * k] : 0.0;
+    for (int k = 0; k < C; k++) mv[k] = ok[k] ? __builtin_nontemporal_load(&Mi[64 * k]) : 0.0;
 }
 
 // Rows min(i0 + r, n - 1), r < R, of a geometric cost at the lane's strip
