@@ -131,6 +131,7 @@ struct Bump {
 };
 
 constexpr int kCtlSlots = 16;
+constexpr int kSticky = 2;   // control blocks kept at fixed addresses, uploaded only when they change
 
 // ---------------------------------------------------------------------------
 // Optional per-kernel-class timing with HIP events on the solve stream
@@ -272,6 +273,9 @@ struct Batch {
     long long packed_wgs = 0;  // workgroups of the current maps
     double* d_ctl = nullptr;   // kCtlSlots * B * CG_NCTL
     double* d_fixctl = nullptr;   // B * CG_NCTL, a control block at a fixed address (captured graphs)
+    double* d_sticky = nullptr;   // kSticky * B * CG_NCTL (ctl_sticky)
+    std::vector<double> sticky_host[kSticky];   // what each sticky block holds
+    std::vector<int> ids_held, pref_held;       // what d_ids and d_pref hold (search_direction)
     double* d_scal = nullptr;  // B * CG_NSCAL
     int* d_pref = nullptr;     // dense prefix tables
     int pref_cap = 0;
@@ -311,6 +315,10 @@ struct Batch {
         d_live = bp.take<int>(std::max<size_t>(nmap, 2));
         d_ctl = bp.take<double>((size_t)kCtlSlots * B * CG_NCTL);
         d_fixctl = bp.take<double>((size_t)B * CG_NCTL);
+        d_sticky = bp.take<double>((size_t)kSticky * B * CG_NCTL);
+        for (auto& v : sticky_host) v.clear();
+        ids_held.clear();
+        pref_held.clear();
         d_scal = bp.take<double>((size_t)B * CG_NSCAL);
         pref_cap = (4 * nblk_max + 3) * (B + 1) + nblk_max + 1;
         d_pref = bp.take<int>(pref_cap);
@@ -464,6 +472,25 @@ struct Batch {
         CG_HIP(hipMemcpyAsync(ds, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
         slot++;
         return ds;
+    }
+
+    // A control block that tends to repeat from one iteration to the next
+    // (the active set, the dense set): sticky block `key` is uploaded (staged
+    // through the ring like ctl()) only when c differs from what it holds.
+    const double* ctl_sticky(int key, const std::vector<double>& c)
+    {
+        double* dst = d_sticky + (size_t)key * B * CG_NCTL;
+        auto& held = sticky_host[key];
+        if (held != c) {
+            if (slot == kCtlSlots) sync();
+            double* hs = hctl + (size_t)slot * B * CG_NCTL;
+            std::memcpy(hs, c.data(), sizeof(double) * B * CG_NCTL);
+            CG_HIP(hipMemcpyAsync(dst, hs, sizeof(double) * B * CG_NCTL, hipMemcpyHostToDevice, stream));
+            slot++;
+            held = c;
+        }
+        hmap[dst] = held.data();
+        return dst;
     }
 
     // Upload a control block to the fixed-address block (staged through the
@@ -1107,10 +1134,14 @@ struct SplrSolver {
         if (!ids.empty()) {
             // ids staged through a pinned buffer: the previous copy out of it
             // completed at the last iteration's read-back
-            static thread_local Pinned pinned_ids;
-            int* hid = (int*)pinned_ids.get(ids.size() / 2 + 1);
-            std::memcpy(hid, ids.data(), sizeof(int) * ids.size());
-            CG_HIP(hipMemcpyAsync(bt.d_ids, hid, sizeof(int) * ids.size(), hipMemcpyHostToDevice, bt.stream));
+            // (skipped when the list is the one the device holds)
+            if (ids != bt.ids_held) {
+                static thread_local Pinned pinned_ids;
+                int* hid = (int*)pinned_ids.get(ids.size() / 2 + 1);
+                std::memcpy(hid, ids.data(), sizeof(int) * ids.size());
+                CG_HIP(hipMemcpyAsync(bt.d_ids, hid, sizeof(int) * ids.size(), hipMemcpyHostToDevice, bt.stream));
+                bt.ids_held = ids;
+            }
             if (!mixed) {
                 ProfScope ps(bt.stream, PROF_SPNEWTON, sp_work);
                 hipLaunchKernelGGL(k_sp_newton, dim3((unsigned)ids.size()), dim3(SP_WG), sp_lds_bytes(rmax_used),
@@ -1118,7 +1149,7 @@ struct SplrSolver {
             }
         }
         if (!anydense) return;
-        const double* dc = bt.ctl(mask(on));
+        const double* dc = bt.ctl_sticky(1, mask(on));
         int nblk_max = 0;
         for (int b = 0; b < B; b++)
             if (on[b]) nblk_max = std::max(nblk_max, hi[b].nblk);
@@ -1161,11 +1192,12 @@ struct SplrSolver {
         // scalars back (a stream sync) before the next search direction, so the
         // previous copy out of this buffer has completed when it is rewritten.
         static thread_local Pinned pinned_pref;
-        if (nblk_max > 0) {
+        if (nblk_max > 0 && pref != bt.pref_held) {
             if ((int)pref.size() > bt.pref_cap) throw std::runtime_error("internal: prefix table overflow");
             int* hp = (int*)pinned_pref.get((pref.size() + 1) / 2 + 1);
             std::memcpy(hp, pref.data(), pref.size() * sizeof(int));
             CG_HIP(hipMemcpyAsync(bt.d_pref, hp, pref.size() * sizeof(int), hipMemcpyHostToDevice, bt.stream));
+            bt.pref_held = pref;
         }
         const int* dtab0 = bt.d_pref;
         auto dtab = [&](int t) { return dtab0 + (size_t)t * (B + 1); };
@@ -1321,7 +1353,7 @@ struct SplrSolver {
             bt.pack_live(act);
             auto c = mask(act);
             for (int b = 0; b < B; b++) c[(size_t)b * CG_NCTL + CTL_ALLOW_LR] = hi[b].it > 0 ? 1.0 : 0.0;
-            const double* dc = bt.ctl(c);
+            const double* dc = bt.ctl_sticky(0, c);
             bt.vec0(k_low_rank, dc);
             // Sinkhorn candidate (3 BCD sweeps from the current beta)
             std::vector<char> cand(B, 0);
@@ -1624,6 +1656,8 @@ struct PcSolver {
             while (cnt < (int)ord.size() && cdiv(bt.h[ord[cnt]].P, CG_NB) > t) cnt++;
             lvl[t + 1] = lvl[t] + cnt;
         }
+        bt.ids_held.clear();    // d_ids and d_pref now hold this solver's lists
+        bt.pref_held.clear();
         d_ord = bt.d_pref;
         d_lvl = bt.d_pref + bt.B;
         if (!ord.empty()) {
