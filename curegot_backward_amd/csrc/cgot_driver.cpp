@@ -1377,8 +1377,7 @@ struct SplrSolver {
                 tpass(bt, cc, CG_PT_BCD);
             }
             search_direction(act, dc);
-            woodbury(dc);
-            bt.vec0(k_save_history, dc);
+            woodbury(dc);   // also saves gamma and grad as the next secant pair's base
             // line search rounds
             for (int b = 0; b < B; b++)
                 if (act[b]) {

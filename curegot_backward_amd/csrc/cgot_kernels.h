@@ -40,9 +40,7 @@ __global__ void k_gauge_shift(const CgInst*, const double*, int mode);      // g
 __global__ void k_init_logab(const CgInst*, const int*, const double*, const double* const*,
                              const double* const*);
 __global__ void k_low_rank(const CgInst*, const int*, const double*);
-__global__ void k_save_history(const CgInst*, const int*, const double*);
 __global__ void k_copy_bcd(const CgInst*, const int*, const double*);
-__global__ void k_gamma_from_step(const CgInst*, const int*, const double*);
 __global__ void k_woodbury_dots(const CgInst*, const int*, const double*);
 __global__ void k_woodbury_apply(const CgInst*, const int*, const double*);
 __global__ void k_output_dual(const CgInst*, const int*, const double*);

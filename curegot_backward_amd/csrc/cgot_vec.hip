@@ -197,22 +197,6 @@ __global__ __launch_bounds__(CG_WG) void k_low_rank(const CgInst* __restrict__ i
     }
 }
 
-// gamma_prev = gamma, grad_prev = grad on the first N entries.
-// Reference: SPLRSolver::save_history, src/sinkhorn_splr.cu:652-664.
-__global__ __launch_bounds__(CG_WG) void k_save_history(const CgInst* __restrict__ insts,
-                                                        const int* __restrict__ wg_inst,
-                                                        const double* __restrict__ ctl)
-{
-    int b, wgi;
-    if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInstD& I = cg_inst(insts, b);
-    const int k = wgi * CG_WG + threadIdx.x;
-    if (k < I.N) {
-        I.gamma_prev[k] = I.gamma[k];
-        I.grad_prev[k] = I.grad[k];
-    }
-}
-
 // gamma = gamma_bcd.  Reference: update_gamma_sinkhorn, src/sinkhorn_splr.cu:929-934.
 __global__ __launch_bounds__(CG_WG) void k_copy_bcd(const CgInst* __restrict__ insts,
                                                     const int* __restrict__ wg_inst,
@@ -223,19 +207,6 @@ __global__ __launch_bounds__(CG_WG) void k_copy_bcd(const CgInst* __restrict__ i
     const CgInstD& I = cg_inst(insts, b);
     const int k = wgi * CG_WG + threadIdx.x;
     if (k < I.n + I.m) I.gamma[k] = I.gamma_bcd[k];
-}
-
-// gamma = gamma_prev + step*dir on the first N entries (gamma_N untouched).
-// Reference: update_gamma, src/sinkhorn_splr.cu:922-927.
-__global__ __launch_bounds__(CG_WG) void k_gamma_from_step(const CgInst* __restrict__ insts,
-                                                           const int* __restrict__ wg_inst,
-                                                           const double* __restrict__ ctl)
-{
-    int b, wgi;
-    if (!cg_decode_vec(insts, wg_inst, ctl, b, wgi)) return;
-    const CgInstD& I = cg_inst(insts, b);
-    const int k = wgi * CG_WG + threadIdx.x;
-    if (k < I.N) I.gamma[k] = I.gamma_prev[k] + ctl[b * CG_NCTL + CTL_STEP] * I.dir[k];
 }
 
 // Dots for the BFGS Woodbury correction: s.g, y.u, y.dir.
@@ -263,9 +234,11 @@ __global__ __launch_bounds__(CG_WG) void k_woodbury_dots(const CgInst* __restric
     }
 }
 
-// dir += t1*s - t2*u (low rank), then dir <- -reg*dir.
+// dir += t1*s - t2*u (low rank), then dir <- -reg*dir; and the history for
+// the next iteration's secant pair: gamma_prev = gamma, grad_prev = grad
+// (nothing reads them between here and the next k_low_rank).
 // Reference: update_direc_kernel + compute_scaling_inplace, src/sinkhorn_splr_kernel_low_rank.cu:257-276,
-// src/sinkhorn_splr.cu:636-638.
+// src/sinkhorn_splr.cu:636-638; SPLRSolver::save_history, src/sinkhorn_splr.cu:652-664.
 __global__ __launch_bounds__(CG_WG) void k_woodbury_apply(const CgInst* __restrict__ insts,
                                                           const int* __restrict__ wg_inst,
                                                           const double* __restrict__ ctl)
@@ -283,6 +256,8 @@ __global__ __launch_bounds__(CG_WG) void k_woodbury_apply(const CgInst* __restri
         d += (t1 * I.sv[k] - sgys * I.uv[k]);
     }
     I.dir[k] = -I.reg * d;
+    I.gamma_prev[k] = I.gamma[k];
+    I.grad_prev[k] = I.grad[k];
 }
 
 // dual = gamma (n+m).
