@@ -106,6 +106,9 @@ def main():
                     help="row/column LSE and T pass in isolation on one n=m=16384 instance (2 GiB of M)")
     ap.add_argument("--adjoint", action="store_true",
                     help="n=m=4096 forward + backward times and dM vs central finite differences")
+    ap.add_argument("--adjoint-sizes", action="store_true",
+                    help="solve_backward wall time on the reference's published backward sizes "
+                         "(100x120 .. 1000x1200, eta 0.05, density 0.10, seed 42)")
     ap.add_argument("--fd", type=int, default=64, help="entries checked by finite differences (--adjoint)")
     ap.add_argument("--fd-sweeps", type=int, default=2000, help="BCD polish sweeps per FD solve (--adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -132,6 +135,9 @@ def main():
         return
     if args.adjoint:
         adjoint_main(args)
+        return
+    if args.adjoint_sizes:
+        adjoint_sizes_main(args)
         return
     if args.lse:
         lse_main(args)
@@ -469,6 +475,50 @@ def single_main(args):
 
 
 LSE_METRIC = "row/column log-sum-exp HBM throughput in isolation, n=m=16384 FP64 (2 GiB of M)"
+
+
+def adjoint_sizes_main(args):
+    """solve_backward on the problems of the reference's backward benchmark
+    (reference benchmark_stage3c3.py:175-230 with stage3b_validate.py:61-69):
+    M ~ U[0,1), a, b ~ U[0,1) + 0.5 normalised (CPU generator, seed 42), eta
+    0.05, forward tol 1e-8 at density 0.10, G ~ N(0,1) seeded per size
+    ((1000003 n + 10007 m + 7) mod (2^31 - 1), benchmark_stage3c2c.py:109-112),
+    PCG tol 1e-12.  Wall time of one solve_backward call (synchronised), the
+    median of 20 after 3 warm-ups, per preconditioner.  The reference's table
+    (STAGE3C_FINAL.md:70-75) times its Python PCG loop on an unnamed GPU."""
+    import curegot_backward_amd as cg
+
+    dev = torch.device("cuda", 0)
+    eta = 0.05
+    published = {(100, 120): 12.03, (300, 360): 9.57, (500, 600): 8.69, (1000, 1200): 7.30}
+    rows = []
+    for (n, m), ref_ms in published.items():
+        gen = torch.Generator(device="cpu").manual_seed(42)
+        M = torch.rand(n, m, generator=gen, dtype=torch.float64)
+        a = torch.rand(n, generator=gen, dtype=torch.float64) + 0.5
+        b = torch.rand(m, generator=gen, dtype=torch.float64) + 0.5
+        M, a, b = M.to(dev), (a / a.sum()).to(dev), (b / b.sum()).to(dev)
+        g = torch.Generator(device="cpu").manual_seed((1000003 * n + 10007 * m + 7) % (2 ** 31 - 1))
+        G = torch.randn((n, m), generator=g, dtype=torch.float64).to(dev)
+        out = cg.sinkhorn_splr_bwd(M, a, b, eta, 1e-8, 1000, density=0.10)
+        row = {"n": n, "m": m, "forward_niter": int(out["niter"]), "reference_diag_ms": ref_ms}
+        for pc in ("diag", "none", "splr"):
+            ts = []
+            for k in range(23):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = cg.solve_backward(out["plan"], G, out["context"], eta, preconditioner=pc)
+                torch.cuda.synchronize()
+                if k >= 3:
+                    ts.append(1000 * (time.perf_counter() - t0))
+            row[pc] = {"ms": float(np.median(ts)), "iterations": int(r["iterations"]),
+                       "converged": bool(r["converged"])}
+        log(f"{n}x{m}: forward niter {row['forward_niter']}; backward diag {row['diag']['ms']:.3f} ms "
+            f"({row['diag']['iterations']} it), none {row['none']['ms']:.3f} ({row['none']['iterations']}), "
+            f"splr {row['splr']['ms']:.3f} ({row['splr']['iterations']}); reference diag {ref_ms} ms")
+        rows.append(row)
+    print(json.dumps({"metric": "solve_backward wall ms, reference backward benchmark sizes", "unit": "ms",
+                      "higher_is_better": False, "dtype": "f64", "data": "synthetic", "rows": rows}), flush=True)
 
 
 def lse_main(args):

@@ -1767,12 +1767,14 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
         hipGraphExec_t& e;
         ~GraphGuard() { if (e) (void)hipGraphExecDestroy(e); }
     } gguard{gexec};
-    int chunk = 4;
+    // the first full chunk runs eagerly: a loop that ends within it (most
+    // well-conditioned systems) would not repay the capture
+    int chunk = 4, eager32 = 0;
     while (nrun > 0) {
         bt.pack_live(running);
         auto c2 = c;
         for (int b = 0; b < B; b++) c2[(size_t)b * CG_NCTL + CTL_ACTIVE] = running[b] ? 1.0 : 0.0;
-        if (graphs && chunk == kGraphChunk) {
+        if (graphs && chunk == kGraphChunk && eager32 > 0) {
             const double* d2 = bt.ctl_fixed(c2);
             if (gexec && (g_tile != bt.d_tile || g_ntiles != bt.ntiles || g_nvec != bt.nvec || g_nrow != bt.nrow)) {
                 CG_HIP(hipGraphExecDestroy(gexec));
@@ -1817,6 +1819,7 @@ void adjoint_run(Batch& bt, const cgot_adjoint_problem* probs, cgot_adjoint_info
             if (running[b] && bt.sc(b, PCG_STATUS) != PCG_RUNNING) running[b] = 0;
             nrun += running[b];
         }
+        if (chunk == kGraphChunk) eager32++;
         chunk = std::min(chunk * 2, 32);
     }
     // a fresh copy of the all-active control: the slot of dc may have been
