@@ -606,18 +606,27 @@ def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
 
 def adjoint_batch(items, tol, max_iter):
     """Batched implicit backward.  items: list of (M, dual, G, reg, inv_diag or
-    None[, factor pointer or None]).  Returns (lambdas, grads, infos)."""
+    None[, factor pointer or None]).  Returns (lambdas, grads, infos,
+    lambda_ext): lambda_ext[i] is [n+m], lambdas[i] followed by a 0 (its first
+    n+m-1 entries are lambdas[i]; all are views of one zero-filled buffer, so
+    the batch costs one fill and no per-instance copies)."""
     B = len(items)
     device = items[0][0].device
     _check_one_cost_kind([it[0] for it in items], "solve_backward_batch")
     probs = (_lib.AdjointProblem * B)()
-    lams, grads, keep = [], [], []
+    lams, grads, keep, exts = [], [], [], []
+    sizes = [it[0].shape[0] + it[0].shape[1] for it in items]
+    lam_all = torch.zeros(sum(sizes), dtype=torch.float64, device=device)
+    off = 0
     for i, item in enumerate(items):
         M, dual, G, reg, inv = item[:5]
         factor = item[5] if len(item) > 5 else None
         n, m = M.shape
         Gd = G.to(device=device, dtype=torch.float64).contiguous()
-        lam = torch.empty(n + m - 1, dtype=torch.float64, device=device)
+        ext = lam_all[off:off + n + m]
+        off += n + m
+        lam = ext[:n + m - 1]
+        exts.append(ext)
         grad = torch.empty((n, m), dtype=torch.float64, device=device)
         p = probs[i]
         _set_cost(p, M)
@@ -637,7 +646,7 @@ def adjoint_batch(items, tol, max_iter):
         ws = _workspace(nbytes, device)
         rc = L.cgot_adjoint_solve(B, probs, ws.data_ptr(), nbytes, _stream(device), infos)
         _lib.check(rc, "solve_backward")
-    return lams, grads, [infos[i] for i in range(B)]
+    return lams, grads, [infos[i] for i in range(B)], exts
 
 
 def lse_update(M, a, b, dual, reg, which, repeats=1):

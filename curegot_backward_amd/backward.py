@@ -67,9 +67,9 @@ def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=20
         ctx.check_unmodified()
         inv, factor = select_preconditioner(ctx, preconditioner)
         jobs.append((ctx.M, ctx.dual, G, float(eta), inv, factor))
-    lams, grads, infos = _native.adjoint_batch(jobs, tol, max_iter)
+    lams, grads, infos, exts = _native.adjoint_batch(jobs, tol, max_iter)
     out = []
-    for lam, grad, info, (G, ctx, eta) in zip(lams, grads, infos, items):
+    for lam, grad, info, ext, (G, ctx, eta) in zip(lams, grads, infos, exts, items):
         if info.status == 2:
             raise RuntimeError("PCG breakdown: p^T A p <= 0 (matrix not SPD?)")
         n = ctx.M.shape[0]
@@ -81,7 +81,7 @@ def solve_backward_batch(items, *, preconditioner="diag", tol=1e-12, max_iter=20
             # the last column takes up any imbalance), so dL/da = lambda_alpha,
             # dL/db = (lambda_beta, 0)
             "grad_a": lam[:n],
-            "grad_b": torch.cat([lam[n:], lam.new_zeros(1)]),
+            "grad_b": ext[n:],
             "iterations": int(info.iterations),
             "converged": bool(info.converged),
             "preconditioner": preconditioner,
