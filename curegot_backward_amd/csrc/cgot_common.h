@@ -156,13 +156,18 @@ CG_DEV double cg_cost(const CgInstD& I, int i, int j)
 }
 
 // Row i of a stored cost at the lane's strip columns j = s0 + lane + 64 k (0
-// where !ok[k]).
-template <int C>
+// where !ok[k]).  NT: non-temporal loads, for passes that stream M once (the
+// forward's); the adjoint's Q v passes re-read M every PCG step and keep the
+// ordinary loads, so that an M within the memory-side cache stays there.
+template <int C, bool NT = true>
 CG_DEV void cg_cost_row(const CgInstD& I, int i, int s0, int lane, const bool (&ok)[C], double (&mv)[C])
 {
     const auto Mi = I.M + (size_t)i * I.m + s0 + lane;
 #pragma unroll
-    for (int k = 0; k < C; k++) mv[k] = ok[k] ? __builtin_nontemporal_load(&Mi[64 * k]) : 0.0;
+    for (int k = 0; k < C; k++) {
+        if constexpr (NT) mv[k] = ok[k] ? __builtin_nontemporal_load(&Mi[64 * k]) : 0.0;
+        else mv[k] = ok[k] ? Mi[64 * k] : 0.0;
+    }
 }
 
 // Rows min(i0 + r, n - 1), r < R, of a geometric cost at the lane's strip

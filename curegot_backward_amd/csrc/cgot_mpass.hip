@@ -386,7 +386,7 @@ CG_DEV void cg_qv_tile(const CgInstD& I, int b, int rb, int st, const double* co
             const int i = min(i0 + r0 + r, n - 1);   // clamped rows are loaded but not used
             ai[r] = I.gamma[i];
             va[r] = MV ? vat(i) : 0.0;
-            if constexpr (!XY) cg_cost_row<C>(I, i, s0, lane, ok, mv[r]);
+            if constexpr (!XY) cg_cost_row<C, false>(I, i, s0, lane, ok, mv[r]);
             if (RHS) {
                 const auto Gi = G + (size_t)i * m + s0 + lane;
 #pragma unroll
