@@ -83,7 +83,11 @@ __global__ void k_pc_wapply(const CgInst*, const int* wg_inst, const double* ctl
 __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, int init);
 // sparse Newton solve (one workgroup of SP_WG threads per listed instance,
 // dynamic LDS sp_lds_bytes(R) of the largest core)
+// LV: the plans' values in LDS (dynamic LDS: the largest sp_lds_vals_bytes of
+// the listed instances); else sp_lds_bytes(R) of the largest core
+template <bool LV>
 __global__ void k_sp_newton(const CgInst*, const int* ids);
+template <bool LV>
 __global__ void k_newton_mixed(const CgInst*, const int* ids, int nsp, const int* sub_map, int nsub,
                                const double* ctl);   // k_sp_newton blocks, then Schur build / rhs rows
 __global__ void k_sp_precond(const CgInst*, const int* ids);

@@ -33,7 +33,10 @@
 
 namespace {
 
-struct SpView {
+// PV: the type of the value pointers (lval, yx): global memory, or LDS for
+// plans whose values fit beside the core's buffers (LdsVals).
+template <class PV>
+struct SpViewT {
     int N, n, t, R, H, ncore, T;
     int64_t meta_ints;   // integer words in [0, o_meta_end)
     const CG_AS1 int* meta;
@@ -57,10 +60,11 @@ struct SpView {
     const CG_AS1 int* blong;
     const CG_AS1 int* cflags;
     int nclong, nylong;
-    CG_AS1 double* lval;
+    PV lval;
     CG_AS1 double* lcore;
-    CG_AS1 double* yx;
+    PV yx;
 };
+using SpView = SpViewT<CG_AS1 double*>;
 
 CG_DEV SpView sp_view(const CG_AS1 SpHdr* hp, CG_AS1 double* plan)
 {
@@ -102,7 +106,8 @@ CG_DEV SpView sp_view(const CG_AS1 SpHdr* hp, CG_AS1 double* plan)
 // per thread and load, SP_PF loads per thread in flight before any is used);
 // the returned value only keeps the loads alive.
 #define SP_PF 8
-CG_DEV unsigned sp_prefetch(const SpView& v)
+template <class V>
+CG_DEV unsigned sp_prefetch(const V& v)
 {
     unsigned acc = 0;
     const int64_t n4 = v.meta_ints & ~(int64_t)3;
@@ -151,10 +156,9 @@ CG_DEV double group8_sum(double v)
 // Up to four independent gathers s[k] -= sum over [p0[k], p1[k]) of
 // X[x] * Y0[y] (and t[k] -= X[x] * Y1[y] with two right-hand sides), in list
 // order, interleaved so that the four items' loads are in flight together.
-template <int NR>
-CG_DEV void gather4(const CG_AS1 int* pairs, const CG_AS1 double* X, const CG_AS1 double* Y0,
-                    const CG_AS1 double* Y1, const bool (&on)[4], const int (&p0)[4], const int (&p1)[4],
-                    double (&s)[4], double (&t)[4])
+template <int NR, class PX, class PY>
+CG_DEV void gather4(const CG_AS1 int* pairs, PX X, PY Y0, PY Y1, const bool (&on)[4], const int (&p0)[4],
+                    const int (&p1)[4], double (&s)[4], double (&t)[4])
 {
     int len = 0;
 #pragma unroll
@@ -337,9 +341,9 @@ CG_DEV void core_fwd_tiles(const CoreTiles& ct, int R, int T, double* w0, double
 // entries (FACTOR; diagonal entries take the square root) and of its
 // columns' forward substitutions, b_j - sum L(j,k) y_k into yx[j] (divided
 // by L_jj right away when the factor is already known, !FACTOR).
-template <bool FACTOR, int NR>
-CG_DEV void level_gathers(const CgInstD& I, const SpView& v, int l, const CG_AS1 double* b0,
-                          const CG_AS1 double* b1, CG_AS1 double* yx, bool& ok)
+template <bool FACTOR, int NR, class V>
+CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 double* b0,
+                          const CG_AS1 double* b1, decltype(V::yx) yx, bool& ok)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N;
     const int e0 = v.lvptr[l], ne = v.lvptr[l + 1] - e0;
@@ -360,7 +364,7 @@ CG_DEV void level_gathers(const CgInstD& I, const SpView& v, int l, const CG_AS1
             act = FACTOR || isy;
         }
         const CG_AS1 int* pr = isy ? v.rowpairs : v.pairs;
-        const CG_AS1 double* y0 = isy ? (const CG_AS1 double*)yx : (const CG_AS1 double*)v.lval;
+        const auto y0 = isy ? yx : v.lval;
         double s0 = 0.0, s1 = 0.0;
         if (act)
             for (int p = p0 + gl; p < p1; p += 8) {
@@ -463,8 +467,8 @@ CG_DEV void level_gathers(const CgInstD& I, const SpView& v, int l, const CG_AS1
 
 // Second phase of a factor level: off-diagonal entries and the forward
 // substitutions divided by their column's diagonal.
-template <int NR>
-CG_DEV void level_divisions(const SpView& v, int l, CG_AS1 double* yx)
+template <int NR, class V>
+CG_DEV void level_divisions(const V& v, int l, decltype(V::yx) yx)
 {
     const int tid = threadIdx.x, N = v.N;
     const int e0 = v.lvptr[l], ne = v.lvptr[l + 1] - e0;
@@ -485,8 +489,8 @@ CG_DEV void level_divisions(const SpView& v, int l, CG_AS1 double* yx)
 
 // Core right-hand sides b_i - sum L(i,k) y_k of the core rows into out0/out1
 // (row i at out[i]); short rows by threads, long rows by waves.
-template <int NR>
-CG_DEV void core_rhs(const SpView& v, const CG_AS1 double* b0, const CG_AS1 double* b1, const CG_AS1 double* yx,
+template <int NR, class V>
+CG_DEV void core_rhs(const V& v, const CG_AS1 double* b0, const CG_AS1 double* b1, decltype(V::yx) yx,
                      double* out0, double* out1)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N, R = v.R;
@@ -543,8 +547,8 @@ CG_DEV void core_rhs(const SpView& v, const CG_AS1 double* b0, const CG_AS1 doub
 }
 
 // Periphery backward substitution (x overwrites y in yx), then the result by vertex.
-template <int NR>
-CG_DEV void periphery_bwd(const SpView& v, CG_AS1 double* x0, CG_AS1 double* x1, CG_AS1 double* yx)
+template <int NR, class V>
+CG_DEV void periphery_bwd(const V& v, CG_AS1 double* x0, CG_AS1 double* x1, decltype(V::yx) yx)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N;
     for (int l = v.H - 1; l >= 0; l--) {
@@ -606,14 +610,14 @@ CG_DEV void periphery_bwd(const SpView& v, CG_AS1 double* x0, CG_AS1 double* x1,
     }
 }
 
-// Newton step of instance b (one workgroup of SP_WG threads): factor A at the
-// current values (Da, Db, w_rval) and solve A [dir, uv] = [grad, yv].
-CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_lds)
+// Newton step of one instance (one workgroup of SP_WG threads): factor A at
+// the current values (Da, Db, w_rval) and solve A [dir, uv] = [grad, yv], with
+// the plan's values (periphery factor entries, work vectors) where v points.
+template <class V>
+CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
 {
-    const CgInstD& I = cg_inst(insts, b);
-    const SpView v = sp_view(I.sph, I.S);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N, t = v.t, R = v.R, T = v.T;
-    CG_AS1 double* yx = v.yx;
+    const auto yx = v.yx;
     const unsigned acc = sp_prefetch(v);
     bool ok = true;
     // ---- periphery: factor entries and forward substitution, by level ----
@@ -787,13 +791,54 @@ CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_
     if (acc == 0x9e3779b9u && N < 0) yx[0] = 0.0;   // never true: keeps the prefetch loads
 }
 
+// The view with its values in LDS: lval at base (the plan's nnzP periphery
+// entries), then the two work vectors (2N).
+CG_DEV SpViewT<double*> sp_lds_view(const SpView& g, double* base)
+{
+    SpViewT<double*> v;
+    v.N = g.N; v.n = g.n; v.t = g.t; v.R = g.R; v.H = g.H; v.ncore = g.ncore; v.T = g.T;
+    v.meta_ints = g.meta_ints;
+    v.meta = g.meta; v.perm = g.perm; v.lvptr = g.lvptr; v.lcptr = g.lcptr; v.lcols = g.lcols;
+    v.emeta = g.emeta; v.erow = g.erow; v.pairs = g.pairs; v.ymeta = g.ymeta; v.ycore = g.ycore;
+    v.rowpairs = g.rowpairs; v.cmeta = g.cmeta; v.cpairs = g.cpairs; v.lgptr = g.lgptr; v.longs = g.longs;
+    v.clong = g.clong; v.ylong = g.ylong; v.blptr = g.blptr; v.blong = g.blong; v.cflags = g.cflags;
+    v.nclong = g.nclong; v.nylong = g.nylong;
+    v.lcore = g.lcore;
+    v.lval = base;
+    v.yx = base + g.lcols[g.t];
+    return v;
+}
+
+// Newton step of instance b.  LV: the values live in LDS after the core's
+// buffers (the launch's dynamic LDS holds sp_lds_vals_bytes of every listed
+// instance); the periphery factor is then written to the plan at the end, for
+// the backward's preconditioner.  The arithmetic is the same either way.
+template <bool LV>
+CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_lds)
+{
+    const CgInstD& I = cg_inst(insts, b);
+    const SpView g = sp_view(I.sph, I.S);
+    if constexpr (LV) {
+        const SpViewT<double*> v = sp_lds_view(g, sp_lds + SP_LDS_CORE(g.R));
+        sp_newton_body(I, v, sp_lds);
+        __syncthreads();
+        const int nnzP = g.lcols[g.t];
+        for (int q = threadIdx.x; q < nnzP; q += SP_WG) g.lval[q] = v.lval[q];
+    } else {
+        sp_newton_body(I, g, sp_lds);
+    }
+}
+
 }  // namespace
 
+template <bool LV>
 __global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ insts, const int* __restrict__ ids)
 {
     extern __shared__ double sp_lds[];
-    sp_newton_block(insts, ids[blockIdx.x], sp_lds);
+    sp_newton_block<LV>(insts, ids[blockIdx.x], sp_lds);
 }
+template __global__ void k_sp_newton<false>(const CgInst*, const int*);
+template __global__ void k_sp_newton<true>(const CgInst*, const int*);
 
 // The Newton step of a group whose instances split between the sparse and
 // the dense factorization, in one launch so that the two overlap: blocks
@@ -807,9 +852,10 @@ __global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ 
 // row buffers of the largest P <= CG_SCHUR_LDS among the dense instances.
 // The sparse blocks of k_newton_mixed: out of line, so that the kernel's
 // Schur branch does not change the factorization's register allocation.
+template <bool LV>
 __device__ __attribute__((noinline)) void sp_newton_call(const CgInst* __restrict__ insts, int b, double* lds)
 {
-    sp_newton_block(insts, b, lds);
+    sp_newton_block<LV>(insts, b, lds);
 }
 
 // The Schur rows of one wave-map entry of k_newton_mixed.
@@ -828,16 +874,19 @@ CG_DEV void schur_rows_entry(const CgInst* __restrict__ insts, int nsp, const in
     else cg_schur_rhs_row(I, s);
 }
 
+template <bool LV>
 __global__ __launch_bounds__(SP_WG) void k_newton_mixed(const CgInst* __restrict__ insts, const int* __restrict__ ids,
                                                         int nsp, const int* __restrict__ sub_map, int nsub,
                                                         const double* __restrict__ ctl)
 {
     extern __shared__ double sp_lds[];
     if ((int)blockIdx.x < nsp)
-        sp_newton_call(insts, ids[blockIdx.x], sp_lds);
+        sp_newton_call<LV>(insts, ids[blockIdx.x], sp_lds);
     else
         schur_rows_entry(insts, nsp, sub_map, nsub, ctl, sp_lds);
 }
+template __global__ void k_newton_mixed<false>(const CgInst*, const int*, int, const int*, int, const double*);
+template __global__ void k_newton_mixed<true>(const CgInst*, const int*, int, const int*, int, const double*);
 
 // Backward preconditioner of the adjoint instances ids[..] whose forward used
 // the sparse factor: sv = A^{-1} yv with the forward's stored factor.

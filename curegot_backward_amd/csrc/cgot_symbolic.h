@@ -136,4 +136,12 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
 // broadcast buffers (4 ceil(R/4) + 2 <= R + 5 each), the pivots and two
 // right-hand sides.
 inline size_t sp_lds_bytes(int R) { return sizeof(double) * (5 * (size_t)R + 16); }
+// doubles of the core's LDS buffers (the values of an LDS-resident launch follow them)
+#define SP_LDS_CORE(R) (5 * (R) + 16)
+// LDS of a launch that keeps the plan's values in LDS: the core's buffers, the
+// nnzP periphery factor entries and the two work vectors of N
+inline size_t sp_lds_vals_bytes(const SpHdr& h)
+{
+    return sp_lds_bytes(h.R) + sizeof(double) * ((size_t)h.nnzP + 2 * (size_t)h.N);
+}
 #endif
