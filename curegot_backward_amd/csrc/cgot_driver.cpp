@@ -790,6 +790,7 @@ struct HostInst {
     int spR = 0, spH = 0;   // sparse plan: core size, periphery levels
     double sp_flops = 0.0;  // sparse plan: flops of one factorization and solve (sp_solve_flops)
     size_t sp_vals = 0;     // sparse plan: LDS of a launch holding its values (sp_lds_vals_bytes)
+    size_t sp_all = 0;      // ... its values and integer arrays (sp_lds_all_bytes)
     size_t pat_off = 0;
 };
 
@@ -1021,6 +1022,7 @@ struct SplrSolver {
             hi[b].spH = ok[k] ? hdr[k].H : 0;
             hi[b].sp_flops = ok[k] ? sp_solve_flops(hdr[k]) : 0.0;
             hi[b].sp_vals = ok[k] ? sp_lds_vals_bytes(hdr[k]) : 0;
+            hi[b].sp_all = ok[k] ? sp_lds_all_bytes(hdr[k]) : 0;
             hi[b].pend = false;
         }
         hipLaunchKernelGGL(k_plan_import, dim3(B * SP_IMPORT_WGS), dim3(CG_WG), 0, bt.stream, bt.d_insts, bt.ctl(c),
@@ -1117,7 +1119,7 @@ struct SplrSolver {
         std::vector<char> on(B, 0);
         std::vector<int> ids;
         int rmax_used = 0;
-        size_t vals_max = 0;
+        size_t vals_max = 0, all_max = 0;
         double sp_work = 0.0;
         for (int b = 0; b < B; b++) {
             if (!on_all[b]) continue;
@@ -1126,15 +1128,18 @@ struct SplrSolver {
                 sp_work += hi[b].sp_flops;
                 rmax_used = std::max(rmax_used, hi[b].spR);
                 vals_max = std::max(vals_max, hi[b].sp_vals);
+                all_max = std::max(all_max, hi[b].sp_all);
             } else {
                 on[b] = 1;
             }
         }
         bool anydense = false;
         for (int b = 0; b < B; b++) anydense = anydense || on[b];
-        // the plans' values in LDS when every listed plan's fit (the same
-        // arithmetic: only where the factor entries and work vectors live)
-        const bool lv = !ids.empty() && vals_max <= (size_t)g_lds_max;
+        // the plans' values in LDS when every listed plan's fit, and the
+        // integer arrays of each plan that fits with them (the same
+        // arithmetic: only where they live)
+        const int lv = (!ids.empty() && vals_max <= (size_t)g_lds_max) ? 1 : 0;
+        const size_t sp_lds = lv ? std::max(vals_max, all_max <= (size_t)g_lds_max ? all_max : 0) : sp_lds_bytes(rmax_used);
         // both kinds in the group: the sparse blocks share one launch with the
         // Schur rows (k_newton_mixed) instead of running before the dense
         // chain; the library profiler keeps them apart to time each
@@ -1152,9 +1157,8 @@ struct SplrSolver {
             }
             if (!mixed) {
                 ProfScope ps(bt.stream, PROF_SPNEWTON, sp_work);
-                hipLaunchKernelGGL(lv ? k_sp_newton<true> : k_sp_newton<false>, dim3((unsigned)ids.size()), dim3(SP_WG),
-                                   lv ? vals_max : sp_lds_bytes(rmax_used),
-                                   bt.stream, bt.d_insts, bt.d_ids);
+                hipLaunchKernelGGL(lv ? k_sp_newton<1> : k_sp_newton<0>, dim3((unsigned)ids.size()), dim3(SP_WG),
+                                   sp_lds, bt.stream, bt.d_insts, bt.d_ids, (int)sp_lds);
             }
         }
         if (!anydense) return;
@@ -1219,13 +1223,12 @@ struct SplrSolver {
             if (on[b] && bt.h[b].P <= CG_SCHUR_LDS) pcap = std::max(pcap, bt.h[b].P);
         bt.sub_rows(on);
         if (mixed) {
-            const size_t lds = std::max(lv ? vals_max : sp_lds_bytes(rmax_used),
-                                        sizeof(double) * (SP_WG / 64) * (size_t)pcap);
+            const size_t lds = std::max(sp_lds, sizeof(double) * (SP_WG / 64) * (size_t)pcap);
             const int per = SP_WG / CG_WG;   // wave-map entries per block
             const int nb = (int)ids.size() + cdiv(2LL * bt.nsub, per);
-            hipLaunchKernelGGL(lv ? k_newton_mixed<true> : k_newton_mixed<false>, dim3(nb), dim3(SP_WG), lds,
+            hipLaunchKernelGGL(lv ? k_newton_mixed<1> : k_newton_mixed<0>, dim3(nb), dim3(SP_WG), lds,
                                bt.stream, bt.d_insts, bt.d_ids,
-                               (int)ids.size(), bt.d_sub, bt.nsub, dc);
+                               (int)ids.size(), bt.d_sub, bt.nsub, dc, (int)lds);
         } else {
             bt.rows_sub(k_schur_build, dc, sizeof(double) * CG_WAVES * (size_t)pcap);
             bt.rows_sub(k_schur_rhs, dc);
@@ -2142,11 +2145,11 @@ int sp_rmax()
         const int bytes = (int)sp_lds_bytes(R);
         // raise the kernels' dynamic-LDS limit where the runtime asks for it;
         // a launch above what the device allows fails at the launch check
-        (void)hipFuncSetAttribute((const void*)k_sp_newton<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_sp_newton<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        (void)hipFuncSetAttribute((const void*)k_sp_newton<0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_sp_newton<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         (void)hipFuncSetAttribute((const void*)k_sp_precond, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void*)k_newton_mixed<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        (void)hipFuncSetAttribute((const void*)k_newton_mixed<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        (void)hipFuncSetAttribute((const void*)k_newton_mixed<0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        (void)hipFuncSetAttribute((const void*)k_newton_mixed<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         g_lds_max = lds;
         (void)hipGetLastError();
         return R;

@@ -83,13 +83,15 @@ __global__ void k_pc_wapply(const CgInst*, const int* wg_inst, const double* ctl
 __global__ void k_pcg_rz(const CgInst*, const int* wg_inst, const double* ctl, int init);
 // sparse Newton solve (one workgroup of SP_WG threads per listed instance,
 // dynamic LDS sp_lds_bytes(R) of the largest core)
-// LV: the plans' values in LDS (dynamic LDS: the largest sp_lds_vals_bytes of
-// the listed instances); else sp_lds_bytes(R) of the largest core
-template <bool LV>
-__global__ void k_sp_newton(const CgInst*, const int* ids);
-template <bool LV>
+// LV 0: plans in global memory (dynamic LDS sp_lds_bytes(R) of the largest
+// core); 1: their values in LDS (at least the largest sp_lds_vals_bytes of
+// the listed instances), and the integer arrays of each plan whose
+// sp_lds_all_bytes fits the launch's lds_bytes too
+template <int LV>
+__global__ void k_sp_newton(const CgInst*, const int* ids, int lds_bytes);
+template <int LV>
 __global__ void k_newton_mixed(const CgInst*, const int* ids, int nsp, const int* sub_map, int nsub,
-                               const double* ctl);   // k_sp_newton blocks, then Schur build / rhs rows
+                               const double* ctl, int lds_bytes);   // k_sp_newton blocks, then Schur rows
 __global__ void k_sp_precond(const CgInst*, const int* ids);
 // pattern export / plan import through pinned host memory (grid = B, and
 // B * SP_IMPORT_WGS)

@@ -33,32 +33,32 @@
 
 namespace {
 
-// PV: the type of the value pointers (lval, yx): global memory, or LDS for
-// plans whose values fit beside the core's buffers (LdsVals).
-template <class PV>
+// PV: the type of the value pointers (lval, yx), PI of the plan's integer
+// arrays: global memory, or LDS for plans that fit beside the core's buffers.
+template <class PV, class PI = const CG_AS1 int*>
 struct SpViewT {
     int N, n, t, R, H, ncore, T;
     int64_t meta_ints;   // integer words in [0, o_meta_end)
-    const CG_AS1 int* meta;
-    const CG_AS1 int* perm;
-    const CG_AS1 int* lvptr;
-    const CG_AS1 int* lcptr;
-    const CG_AS1 int* lcols;
-    const CG_AS1 int* emeta;      // 4 ints per entry
-    const CG_AS1 int* erow;
-    const CG_AS1 int* pairs;      // 2 ints per pair
-    const CG_AS1 int* ymeta;      // 4 ints per column
-    const CG_AS1 int* ycore;      // 4 ints per core row
-    const CG_AS1 int* rowpairs;   // 2 ints per pair
-    const CG_AS1 int* cmeta;      // 4 ints per core entry
-    const CG_AS1 int* cpairs;
-    const CG_AS1 int* lgptr;
-    const CG_AS1 int* longs;
-    const CG_AS1 int* clong;
-    const CG_AS1 int* ylong;
-    const CG_AS1 int* blptr;
-    const CG_AS1 int* blong;
-    const CG_AS1 int* cflags;
+    PI meta;
+    PI perm;
+    PI lvptr;
+    PI lcptr;
+    PI lcols;
+    PI emeta;      // 4 ints per entry
+    PI erow;
+    PI pairs;      // 2 ints per pair
+    PI ymeta;      // 4 ints per column
+    PI ycore;      // 4 ints per core row
+    PI rowpairs;   // 2 ints per pair
+    PI cmeta;      // 4 ints per core entry
+    PI cpairs;
+    PI lgptr;
+    PI longs;
+    PI clong;
+    PI ylong;
+    PI blptr;
+    PI blong;
+    PI cflags;
     int nclong, nylong;
     PV lval;
     CG_AS1 double* lcore;
@@ -156,8 +156,8 @@ CG_DEV double group8_sum(double v)
 // Up to four independent gathers s[k] -= sum over [p0[k], p1[k]) of
 // X[x] * Y0[y] (and t[k] -= X[x] * Y1[y] with two right-hand sides), in list
 // order, interleaved so that the four items' loads are in flight together.
-template <int NR, class PX, class PY>
-CG_DEV void gather4(const CG_AS1 int* pairs, PX X, PY Y0, PY Y1, const bool (&on)[4], const int (&p0)[4],
+template <int NR, class PP, class PX, class PY>
+CG_DEV void gather4(PP pairs, PX X, PY Y0, PY Y1, const bool (&on)[4], const int (&p0)[4],
                     const int (&p1)[4], double (&s)[4], double (&t)[4])
 {
     int len = 0;
@@ -358,12 +358,12 @@ CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 doub
         if (act) {
             code = v.longs[li];
             isy = code < 0;
-            const CG_AS1 int* r = isy ? v.ymeta + 4 * (-code - 1) : v.emeta + 4 * code;
+            const auto r = isy ? v.ymeta + 4 * (-code - 1) : v.emeta + 4 * code;
             p0 = r[1];
             p1 = r[2];
             act = FACTOR || isy;
         }
-        const CG_AS1 int* pr = isy ? v.rowpairs : v.pairs;
+        const auto pr = isy ? v.rowpairs : v.pairs;
         const auto y0 = isy ? yx : v.lval;
         double s0 = 0.0, s1 = 0.0;
         if (act)
@@ -377,7 +377,7 @@ CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 doub
         if (NR > 1) s1 = group8_sum(s1);
         if (act && gl == 0) {
             if (!isy) {
-                const CG_AS1 int* e = v.emeta + 4 * code;
+                const auto e = v.emeta + 4 * code;
                 double x = sp_aval(I, e[0], N) - s0;
                 if ((e[3] & ~SP_LONG_BIT) == code) {
                     ok = ok && (x > 0.0);
@@ -385,7 +385,7 @@ CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 doub
                 }
                 v.lval[code] = x;
             } else {
-                const CG_AS1 int* y = v.ymeta + 4 * (-code - 1);
+                const auto y = v.ymeta + 4 * (-code - 1);
                 const int d = y[3] & ~SP_LONG_BIT;
                 const int j = v.erow[d];
                 const double div = FACTOR ? 1.0 : v.lval[d];
@@ -408,7 +408,7 @@ CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 doub
                 p0[k] = p1[k] = dp[k] = 0;
                 s[k] = t[k] = 0.0;
                 if (on[k]) {
-                    const CG_AS1 int* e = v.emeta + 4 * q[k];
+                    const auto e = v.emeta + 4 * q[k];
                     const int asrc = e[0];
                     p0[k] = e[1];
                     p1[k] = e[2];
@@ -441,7 +441,7 @@ CG_DEV void level_gathers(const CgInstD& I, const V& v, int l, const CG_AS1 doub
             p0[k] = p1[k] = d[k] = 0;
             s[k] = t[k] = 0.0;
             if (on[k]) {
-                const CG_AS1 int* y = v.ymeta + 4 * (c0 + it);
+                const auto y = v.ymeta + 4 * (c0 + it);
                 const int vtx = y[0];
                 p0[k] = y[1];
                 p1[k] = y[2];
@@ -505,7 +505,7 @@ CG_DEV void core_rhs(const V& v, const CG_AS1 double* b0, const CG_AS1 double* b
             p0[k] = p1[k] = 0;
             s[k] = t[k] = 0.0;
             if (on[k]) {
-                const CG_AS1 int* y = v.ycore + 4 * row[k];
+                const auto y = v.ycore + 4 * row[k];
                 const int vtx = y[0];
                 p0[k] = y[1];
                 p1[k] = y[2];
@@ -527,7 +527,7 @@ CG_DEV void core_rhs(const V& v, const CG_AS1 double* b0, const CG_AS1 double* b
     for (int base = wave * 8; base < v.nylong; base += SP_WG / 8) {
         const int li = base + (lane >> 3), gl = lane & 7;
         const bool act = li < v.nylong;
-        const CG_AS1 int* y = v.ycore + 4 * (act ? v.ylong[li] : 0);
+        const auto y = v.ycore + 4 * (act ? v.ylong[li] : 0);
         double s0 = 0.0, s1 = 0.0;
         if (act)
             for (int p = y[1] + gl; p < y[2]; p += 8) {
@@ -613,12 +613,13 @@ CG_DEV void periphery_bwd(const V& v, CG_AS1 double* x0, CG_AS1 double* x1, decl
 // Newton step of one instance (one workgroup of SP_WG threads): factor A at
 // the current values (Da, Db, w_rval) and solve A [dir, uv] = [grad, yv], with
 // the plan's values (periphery factor entries, work vectors) where v points.
-template <class V>
+template <bool PF, class V>
 CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, N = v.N, t = v.t, R = v.R, T = v.T;
     const auto yx = v.yx;
-    const unsigned acc = sp_prefetch(v);
+    unsigned acc = 0u;
+    if constexpr (PF) acc = sp_prefetch(v);
     bool ok = true;
     // ---- periphery: factor entries and forward substitution, by level ----
     for (int l = 0; l < v.H; l++) {
@@ -650,7 +651,7 @@ CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
                 p0[k] = p1[k] = 0;
                 s[k] = tt[k] = 0.0;
                 if (on[k]) {
-                    const CG_AS1 int* e = v.cmeta + 4 * c[k];
+                    const auto e = v.cmeta + 4 * c[k];
                     const int asrc = e[0];
                     p0[k] = e[1];
                     p1[k] = e[2];
@@ -667,7 +668,7 @@ CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
             const int li = base + (lane >> 3), gl = lane & 7;
             const bool act = li < v.nclong;
             const int c = act ? v.clong[li] : 0;
-            const CG_AS1 int* e = v.cmeta + 4 * c;
+            const auto e = v.cmeta + 4 * c;
             double sum = 0.0;
             if (act)
                 for (int p = e[1] + gl; p < e[2]; p += 8) sum += v.lval[v.cpairs[2 * p]] * v.lval[v.cpairs[2 * p + 1]];
@@ -792,53 +793,95 @@ CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
 }
 
 // The view with its values in LDS: lval at base (the plan's nnzP periphery
-// entries), then the two work vectors (2N).
-CG_DEV SpViewT<double*> sp_lds_view(const SpView& g, double* base)
+// entries), then the two work vectors (2N); with META also the plan's integer
+// arrays, copied to LDS from sp_lds_meta_off on (same offsets as in the plan).
+CG_DEV int sp_lds_meta_off(int nnzP, int N) { return (nnzP + 2 * N + 1) & ~1; }   // doubles, 16-byte aligned
+
+template <bool META>
+CG_DEV auto sp_lds_view(const SpView& g, double* base)
 {
-    SpViewT<double*> v;
+    using PI = typename std::conditional<META, const int*, const CG_AS1 int*>::type;
+    SpViewT<double*, PI> v;
     v.N = g.N; v.n = g.n; v.t = g.t; v.R = g.R; v.H = g.H; v.ncore = g.ncore; v.T = g.T;
     v.meta_ints = g.meta_ints;
-    v.meta = g.meta; v.perm = g.perm; v.lvptr = g.lvptr; v.lcptr = g.lcptr; v.lcols = g.lcols;
-    v.emeta = g.emeta; v.erow = g.erow; v.pairs = g.pairs; v.ymeta = g.ymeta; v.ycore = g.ycore;
-    v.rowpairs = g.rowpairs; v.cmeta = g.cmeta; v.cpairs = g.cpairs; v.lgptr = g.lgptr; v.longs = g.longs;
-    v.clong = g.clong; v.ylong = g.ylong; v.blptr = g.blptr; v.blong = g.blong; v.cflags = g.cflags;
     v.nclong = g.nclong; v.nylong = g.nylong;
     v.lcore = g.lcore;
+    const int nnzP = g.lcols[g.t];
     v.lval = base;
-    v.yx = base + g.lcols[g.t];
+    v.yx = base + nnzP;
+    if constexpr (META) {
+        const int* m = reinterpret_cast<const int*>(base + sp_lds_meta_off(nnzP, g.N));
+        auto at = [&](const CG_AS1 int* p) { return m + (p - g.meta); };
+        v.meta = m; v.perm = at(g.perm); v.lvptr = at(g.lvptr); v.lcptr = at(g.lcptr); v.lcols = at(g.lcols);
+        v.emeta = at(g.emeta); v.erow = at(g.erow); v.pairs = at(g.pairs); v.ymeta = at(g.ymeta);
+        v.ycore = at(g.ycore); v.rowpairs = at(g.rowpairs); v.cmeta = at(g.cmeta); v.cpairs = at(g.cpairs);
+        v.lgptr = at(g.lgptr); v.longs = at(g.longs); v.clong = at(g.clong); v.ylong = at(g.ylong);
+        v.blptr = at(g.blptr); v.blong = at(g.blong); v.cflags = at(g.cflags);
+    } else {
+        v.meta = g.meta; v.perm = g.perm; v.lvptr = g.lvptr; v.lcptr = g.lcptr; v.lcols = g.lcols;
+        v.emeta = g.emeta; v.erow = g.erow; v.pairs = g.pairs; v.ymeta = g.ymeta; v.ycore = g.ycore;
+        v.rowpairs = g.rowpairs; v.cmeta = g.cmeta; v.cpairs = g.cpairs; v.lgptr = g.lgptr; v.longs = g.longs;
+        v.clong = g.clong; v.ylong = g.ylong; v.blptr = g.blptr; v.blong = g.blong; v.cflags = g.cflags;
+    }
     return v;
 }
 
-// Newton step of instance b.  LV: the values live in LDS after the core's
-// buffers (the launch's dynamic LDS holds sp_lds_vals_bytes of every listed
-// instance); the periphery factor is then written to the plan at the end, for
-// the backward's preconditioner.  The arithmetic is the same either way.
-template <bool LV>
-CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_lds)
+// Newton step of one instance with its plan in LDS.  LV 1: the values
+// (periphery factor entries, work vectors) after the core's buffers; LV 2:
+// the integer arrays copied there as well (in global memory they are
+// streamed through the cache first).  The periphery factor is written to the
+// plan at the end, for the backward's preconditioner.  The arithmetic is the
+// same as with the plan in global memory.
+template <int LV>
+CG_DEV void sp_newton_in_lds(const CgInstD& I, const SpView& g, double* sp_lds)
+{
+    {
+        double* base = sp_lds + SP_LDS_CORE(g.R);
+        const int nnzP = g.lcols[g.t];
+        if constexpr (LV == 2) {
+            // the plan's integer arrays, 8 bytes per load (o_meta_end is a multiple of 8)
+            long long* dst = reinterpret_cast<long long*>(base + sp_lds_meta_off(nnzP, g.N));
+            const CG_AS1 long long* src = reinterpret_cast<const CG_AS1 long long*>(g.meta);
+            const int64_t n8 = g.meta_ints / 2;
+            for (int64_t x = threadIdx.x; x < n8; x += SP_WG) dst[x] = src[x];
+            __syncthreads();
+        }
+        const auto v = sp_lds_view<LV == 2>(g, base);
+        sp_newton_body<LV == 1>(I, v, sp_lds);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nnzP; q += SP_WG) g.lval[q] = v.lval[q];
+    }
+}
+
+// LV 0: the plan in global memory; LV 1: its values in LDS, and its integer
+// arrays too when they fit in the launch's lds_bytes (decided per workgroup,
+// uniformly).
+template <int LV>
+CG_DEV void sp_newton_block(const CgInst* __restrict__ insts, int b, double* sp_lds, int lds_bytes)
 {
     const CgInstD& I = cg_inst(insts, b);
     const SpView g = sp_view(I.sph, I.S);
-    if constexpr (LV) {
-        const SpViewT<double*> v = sp_lds_view(g, sp_lds + SP_LDS_CORE(g.R));
-        sp_newton_body(I, v, sp_lds);
-        __syncthreads();
-        const int nnzP = g.lcols[g.t];
-        for (int q = threadIdx.x; q < nnzP; q += SP_WG) g.lval[q] = v.lval[q];
+    if constexpr (LV == 0) {
+        sp_newton_body<true>(I, g, sp_lds);
     } else {
-        sp_newton_body(I, g, sp_lds);
+        const int64_t all = 8 * (int64_t)SP_LDS_CORE(g.R) + 8 * (int64_t)sp_lds_meta_off(g.lcols[g.t], g.N) +
+                            4 * g.meta_ints;
+        if (all <= lds_bytes) sp_newton_in_lds<2>(I, g, sp_lds);
+        else sp_newton_in_lds<1>(I, g, sp_lds);
     }
 }
 
 }  // namespace
 
-template <bool LV>
-__global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ insts, const int* __restrict__ ids)
+template <int LV>
+__global__ __launch_bounds__(SP_WG) void k_sp_newton(const CgInst* __restrict__ insts, const int* __restrict__ ids,
+                                                     int lds_bytes)
 {
     extern __shared__ double sp_lds[];
-    sp_newton_block<LV>(insts, ids[blockIdx.x], sp_lds);
+    sp_newton_block<LV>(insts, ids[blockIdx.x], sp_lds, lds_bytes);
 }
-template __global__ void k_sp_newton<false>(const CgInst*, const int*);
-template __global__ void k_sp_newton<true>(const CgInst*, const int*);
+template __global__ void k_sp_newton<0>(const CgInst*, const int*, int);
+template __global__ void k_sp_newton<1>(const CgInst*, const int*, int);
 
 // The Newton step of a group whose instances split between the sparse and
 // the dense factorization, in one launch so that the two overlap: blocks
@@ -852,10 +895,11 @@ template __global__ void k_sp_newton<true>(const CgInst*, const int*);
 // row buffers of the largest P <= CG_SCHUR_LDS among the dense instances.
 // The sparse blocks of k_newton_mixed: out of line, so that the kernel's
 // Schur branch does not change the factorization's register allocation.
-template <bool LV>
-__device__ __attribute__((noinline)) void sp_newton_call(const CgInst* __restrict__ insts, int b, double* lds)
+template <int LV>
+__device__ __attribute__((noinline)) void sp_newton_call(const CgInst* __restrict__ insts, int b, double* lds,
+                                                         int lds_bytes)
 {
-    sp_newton_block<LV>(insts, b, lds);
+    sp_newton_block<LV>(insts, b, lds, lds_bytes);
 }
 
 // The Schur rows of one wave-map entry of k_newton_mixed.
@@ -874,19 +918,19 @@ CG_DEV void schur_rows_entry(const CgInst* __restrict__ insts, int nsp, const in
     else cg_schur_rhs_row(I, s);
 }
 
-template <bool LV>
+template <int LV>
 __global__ __launch_bounds__(SP_WG) void k_newton_mixed(const CgInst* __restrict__ insts, const int* __restrict__ ids,
                                                         int nsp, const int* __restrict__ sub_map, int nsub,
-                                                        const double* __restrict__ ctl)
+                                                        const double* __restrict__ ctl, int lds_bytes)
 {
     extern __shared__ double sp_lds[];
     if ((int)blockIdx.x < nsp)
-        sp_newton_call<LV>(insts, ids[blockIdx.x], sp_lds);
+        sp_newton_call<LV>(insts, ids[blockIdx.x], sp_lds, lds_bytes);
     else
         schur_rows_entry(insts, nsp, sub_map, nsub, ctl, sp_lds);
 }
-template __global__ void k_newton_mixed<false>(const CgInst*, const int*, int, const int*, int, const double*);
-template __global__ void k_newton_mixed<true>(const CgInst*, const int*, int, const int*, int, const double*);
+template __global__ void k_newton_mixed<0>(const CgInst*, const int*, int, const int*, int, const double*, int);
+template __global__ void k_newton_mixed<1>(const CgInst*, const int*, int, const int*, int, const double*, int);
 
 // Backward preconditioner of the adjoint instances ids[..] whose forward used
 // the sparse factor: sv = A^{-1} yv with the forward's stored factor.

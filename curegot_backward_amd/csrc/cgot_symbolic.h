@@ -144,4 +144,10 @@ inline size_t sp_lds_vals_bytes(const SpHdr& h)
 {
     return sp_lds_bytes(h.R) + sizeof(double) * ((size_t)h.nnzP + 2 * (size_t)h.N);
 }
+// ... and with the plan's integer arrays after the values (16-byte aligned)
+inline size_t sp_lds_all_bytes(const SpHdr& h)
+{
+    return sp_lds_bytes(h.R) + sizeof(double) * (((size_t)h.nnzP + 2 * (size_t)h.N + 1) & ~(size_t)1) +
+           (size_t)h.o_meta_end;
+}
 #endif
