@@ -2140,9 +2140,9 @@ int sp_rmax()
         int dev = 0, lds = 0;
         CG_HIP(hipGetDevice(&dev));
         CG_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        int R = SP_RMAX;
+        int R = SP_RMAX_BIG;
         while (R > 0 && sp_lds_bytes(R) > (size_t)lds) R--;
-        const int bytes = (int)sp_lds_bytes(R);
+        const int bytes = (int)sp_lds_bytes(R);   // increasing in R
         // raise the kernels' dynamic-LDS limit where the runtime asks for it;
         // a launch above what the device allows fails at the launch check
         (void)hipFuncSetAttribute((const void*)k_sp_newton<0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

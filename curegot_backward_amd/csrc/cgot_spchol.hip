@@ -337,6 +337,347 @@ CG_DEV void core_fwd_tiles(const CoreTiles& ct, int R, int T, double* w0, double
     }
 }
 
+// ---------------------------------------------------------------------------
+// Panel core (SP_SMALL_MAX < R <= SP_RMAX_BIG): the core matrix stays in the
+// plan's packed storage lcore (row-major lower triangle, entry (i, j) at
+// tri(i) + j) and is factored in place; thread i < R owns core row i and its
+// right-hand-side entries.  LDS (doubles): the panel's columns [0, 8R)
+// (column c of row i at c R + i, so that lanes over rows read consecutive
+// words), the two right-hand sides [8R, 10R), the 8 x 8 diagonal block, its
+// right-hand sides, two block buffers.
+// ---------------------------------------------------------------------------
+// The value of x in lane `src` of the wave (a compile-time lane here), as a
+// wave-uniform value: two readlane operations, no LDS round trip.
+CG_DEV double lane_value(double x, int src)
+{
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), src);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+struct BigLds {
+    double *pan, *w0, *w1, *dblk, *yb, *xb;
+};
+CG_DEV BigLds big_lds(double* lds, int R)
+{
+    BigLds b;
+    b.pan = lds;
+    b.w0 = lds + 8 * R;
+    b.w1 = b.w0 + R;
+    b.dblk = b.w1 + R;   // 64
+    b.yb = b.dblk + 64;  // 16: the block's two right-hand sides
+    b.xb = b.yb + 16;    // 2 x 80: block buffers of the substitutions
+    return b;
+}
+
+// Backward substitution L^T x = y of the panel core, by blocks of 8 rows from
+// the last: the block's rows publish their y (less the contributions of the
+// blocks below) to LDS; every thread above the block solves the block's
+// 8 x 8 triangle (the same arithmetic in every thread) and removes the
+// block's contribution from its own row; the block's threads keep their x.
+// One barrier per block (the block buffers alternate).  r0 / r1: this
+// thread's entries; par: the next buffer's parity.
+template <int NR>
+CG_DEV void big_bwd(const CG_AS1 double* lcore, int R, const BigLds& L, double& r0, double& r1, int& par)
+{
+    const int tid = threadIdx.x;
+    const bool own = tid < R;
+    for (int kb = (R + 7) / 8 - 1; kb >= 0; kb--) {
+        const int b0 = 8 * kb, nb = min(8, R - b0);
+        double* xs = L.xb + 80 * par;   // [0, 16): right-hand sides, [16, 80): the block's factor
+        double* lb = xs + 16;
+        par ^= 1;
+        const int me = tid - b0;
+        if (own && me >= 0 && me < nb) {
+            xs[me] = r0;
+            if (NR > 1) xs[8 + me] = r1;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (c <= me) lb[8 * me + c] = lcore[tri(tid) + b0 + c];
+        }
+        // this row's entries in the block's rows (needed after the block's solve)
+        double lr[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) lr[a] = (own && me < 0 && a < nb) ? lcore[tri(b0 + a) + tid] : 0.0;
+        __syncthreads();
+        if (own && me < nb) {
+            // the thread's own entry is taken as it is computed (a compare per
+            // entry: no runtime index into the arrays)
+            double x0[8], x1[8];
+#pragma unroll
+            for (int a = 7; a >= 0; a--) {
+                x0[a] = x1[a] = 0.0;
+                if (a >= nb) continue;
+                double s0 = xs[a], s1 = (NR > 1) ? xs[8 + a] : 0.0;
+#pragma unroll
+                for (int a2 = a + 1; a2 < 8; a2++) {
+                    if (a2 >= nb) continue;
+                    const double l = lb[8 * a2 + a];
+                    s0 -= l * x0[a2];
+                    if (NR > 1) s1 -= l * x1[a2];
+                }
+                const double d = lb[8 * a + a];
+                x0[a] = s0 / d;
+                if (NR > 1) x1[a] = s1 / d;
+                if (me == a) {
+                    r0 = x0[a];
+                    if (NR > 1) r1 = x1[a];
+                }
+            }
+            if (me < 0) {
+#pragma unroll
+                for (int a = 0; a < 8; a++) {
+                    if (a >= nb) continue;
+                    r0 -= lr[a] * x0[a];
+                    if (NR > 1) r1 -= lr[a] * x1[a];
+                }
+            }
+        }
+    }
+}
+
+// Trailing update of rows ib .. ib + nr - 1 (nr <= NRB) of the panel core:
+// C(i, j) -= sum_c L(i, c) L(j, c) for q <= j <= i, one wave, lanes over j,
+// NCH chunks of 64 columns per step.  The rows' L entries are held in
+// registers (LDS broadcast reads); the next step's C entries are loaded
+// before the current step is computed and stored, so that loads are in
+// flight during the arithmetic.
+#ifndef BIG_NRB
+#define BIG_NRB 2   // rows per wave step of the trailing update
+#endif
+#ifndef BIG_NCH
+#define BIG_NCH 2   // 64-column chunks per wave step
+#endif
+template <int NRB, int NCH>
+CG_DEV void big_trailing_rows(CG_AS1 double* lcore, const double* pan, int R, int q, int ib, int nr, int lane)
+{
+    double li[NRB][8];
+    int rbase[NRB];
+#pragma unroll
+    for (int r = 0; r < NRB; r++) {
+        const int i = ib + min(r, nr - 1);   // rows past nr repeat the last (never stored)
+        rbase[r] = tri(i);
+#pragma unroll
+        for (int c = 0; c < 8; c++) li[r][c] = pan[c * R + i];
+    }
+    const int jend = ib + nr - 1;   // the last row's diagonal
+    auto live = [&](int r, int j) { return r < nr && j <= ib + r; };
+    auto load = [&](int j0, double (&v)[NCH][NRB]) {
+#pragma unroll
+        for (int h = 0; h < NCH; h++)
+#pragma unroll
+            for (int r = 0; r < NRB; r++) {
+                const int j = j0 + 64 * h;
+                v[h][r] = live(r, j) ? lcore[rbase[r] + j] : 0.0;
+            }
+    };
+    double cur[NCH][NRB];
+    int j0 = q + lane;
+    load(j0, cur);
+    for (; j0 - lane <= jend; j0 += 64 * NCH) {
+        double nxt[NCH][NRB];
+        load(j0 + 64 * NCH, nxt);
+#pragma unroll
+        for (int h = 0; h < NCH; h++) {
+            const int j = j0 + 64 * h;
+            if (j > jend) continue;
+            double lj[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) lj[c] = pan[c * R + j];
+#pragma unroll
+            for (int r = 0; r < NRB; r++) {
+                if (!live(r, j)) continue;
+                double sv = cur[h][r];
+#pragma unroll
+                for (int c = 0; c < 8; c++) sv -= li[r][c] * lj[c];
+                lcore[rbase[r] + j] = sv;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < NCH; h++)
+#pragma unroll
+            for (int r = 0; r < NRB; r++) cur[h][r] = nxt[h][r];
+    }
+}
+
+// Factor the panel core C (in lcore, R <= SP_WG) in place into its Cholesky
+// factor L (L L^T = C, the same storage and meaning as the register-tile
+// core's) and solve L L^T x = w for the two right-hand sides in LDS (w0, w1
+// in, x out).  Per panel of nb <= 8 columns starting at p0:
+//   A  rows >= p0 read their panel entries; the diagonal block's rows
+//      publish them and their right-hand-side entries;
+//   B  wave 0, lane a = block row a: the block's Cholesky factor column by
+//      column (pivots and multipliers exchanged by shuffles) and its forward
+//      substitution; L's block to lcore;
+//   C  rows below the block: their L entries (triangular solve against the
+//      block) to lcore and LDS, and the block's contribution to their
+//      right-hand-side entries;
+//   D  the trailing triangle, C(i, j) -= sum_c L(i, c) L(j, c): a wave per
+//      pair of rows, lanes over the columns.
+// Returns 0 when a pivot is not positive.  Every sum runs in a fixed order.
+__device__ __attribute__((noinline)) int core_big_factor(CG_AS1 double* lcore, int R, double* lds)
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const BigLds L = big_lds(lds, R);
+    const bool own = tid < R;
+    double r0 = own ? L.w0[tid] : 0.0, r1 = own ? L.w1[tid] : 0.0;
+    int ok = 1;
+    for (int p0 = 0; p0 < R; p0 += 8) {
+        const int nb = min(8, R - p0);
+        const bool inp = own && tid >= p0;
+        CG_AS1 double* row = lcore + (inp ? tri(tid) + p0 : 0);
+        double x[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) x[c] = (inp && c < nb && p0 + c <= tid) ? row[c] : 0.0;
+        if (inp && tid < p0 + nb) {
+            const int a = tid - p0;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (c <= a) L.dblk[8 * a + c] = x[c];
+            L.yb[a] = r0;
+            L.yb[8 + a] = r1;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int a = lane;
+            const bool act = a < nb;
+            double d[8], y0 = 0.0, y1 = 0.0;
+#pragma unroll
+            for (int c = 0; c < 8; c++) d[c] = (act && c <= a) ? L.dblk[8 * a + c] : 0.0;
+            if (act) {
+                y0 = L.yb[a];
+                y1 = L.yb[8 + a];
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                if (c < nb) {   // uniform; no early exit, so that the loop unrolls (readlane is convergent)
+                    const double dc = lane_value(d[c], c);
+                    ok &= (dc > 0.0) ? 1 : 0;
+                    const double lcc = sqrt(dc);
+                    const double y0c = lane_value(y0, c) / lcc, y1c = lane_value(y1, c) / lcc;
+                    const double lac = (a > c) ? d[c] / lcc : lcc;   // L(a, c) for a >= c
+#pragma unroll
+                    for (int c2 = c + 1; c2 < 8; c2++) {
+                        const double l2 = lane_value(lac, c2);   // L(c2, c)
+                        if (a >= c2) d[c2] -= lac * l2;
+                    }
+                    if (a > c) {
+                        y0 -= lac * y0c;
+                        y1 -= lac * y1c;
+                    } else if (a == c) {
+                        y0 = y0c;
+                        y1 = y1c;
+                    }
+                    if (a >= c) d[c] = lac;
+                }
+            }
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    if (c <= a) {
+                        L.dblk[8 * a + c] = d[c];
+                        lcore[tri(p0 + a) + p0 + c] = d[c];
+                    }
+                L.yb[a] = y0;
+                L.yb[8 + a] = y1;
+            }
+        }
+        __syncthreads();
+        if (inp) {
+            if (tid < p0 + nb) {
+                r0 = L.yb[tid - p0];
+                r1 = L.yb[8 + tid - p0];
+            } else {
+                double l[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    l[c] = 0.0;
+                    if (c >= nb) continue;
+                    double s0 = x[c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; c2++) s0 -= l[c2] * L.dblk[8 * c + c2];
+                    l[c] = s0 / L.dblk[8 * c + c];
+                    row[c] = l[c];
+                    r0 -= l[c] * L.yb[c];
+                    r1 -= l[c] * L.yb[8 + c];
+                }
+#pragma unroll
+                for (int c = 0; c < 8; c++) L.pan[c * R + tid] = l[c];
+            }
+        }
+        __syncthreads();
+        // rows below the panel exist only when nb == 8
+        const int q = p0 + nb;
+        for (int ib = q + BIG_NRB * wave; ib < R; ib += BIG_NRB * (SP_WG / 64))
+            big_trailing_rows<BIG_NRB, BIG_NCH>(lcore, L.pan, R, q, ib, min(BIG_NRB, R - ib), lane);
+        __syncthreads();
+    }
+    int par = 0;
+    big_bwd<2>(lcore, R, L, r0, r1, par);
+    if (own) {
+        L.w0[tid] = r0;
+        L.w1[tid] = r1;
+    }
+    return ok;
+}
+
+// L L^T x = w with the panel core's stored factor, one right-hand side (w0 in
+// LDS in, x out): the forward substitution by blocks of 8 rows from the first
+// (as big_bwd, mirrored: the block's rows publish their right-hand-side
+// entries and factor rows, every thread below solves the block's triangle and
+// updates its own entry), then big_bwd.
+__device__ __attribute__((noinline)) void core_big_solve(const CG_AS1 double* lcore, int R, double* lds)
+{
+    const int tid = threadIdx.x;
+    const BigLds L = big_lds(lds, R);
+    const bool own = tid < R;
+    double r0 = own ? L.w0[tid] : 0.0, r1 = 0.0;
+    int par = 0;
+    for (int b0 = 0; b0 < R; b0 += 8) {
+        const int nb = min(8, R - b0);
+        double* xs = L.xb + 80 * par;   // [0, 8): right-hand side, [16, 80): the block's factor
+        double* lb = xs + 16;
+        par ^= 1;
+        const int me = tid - b0;
+        if (own && me >= 0 && me < nb) {
+            xs[me] = r0;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (c <= me) lb[8 * me + c] = lcore[tri(tid) + b0 + c];
+        }
+        // this row's entries in the block's columns (needed after the block's solve)
+        double lr[8];
+        const CG_AS1 double* row = lcore + (own && me >= nb ? tri(tid) + b0 : 0);
+#pragma unroll
+        for (int a = 0; a < 8; a++) lr[a] = (own && me >= nb && a < nb) ? row[a] : 0.0;
+        __syncthreads();
+        if (own && me >= 0) {
+            double y[8];
+            double mine = r0;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                y[a] = 0.0;
+                if (a >= nb) continue;
+                double s0 = xs[a];
+#pragma unroll
+                for (int a2 = 0; a2 < a; a2++) s0 -= lb[8 * a + a2] * y[a2];
+                y[a] = s0 / lb[8 * a + a];
+                if (me == a) mine = y[a];
+            }
+            if (me < nb) {
+                r0 = mine;
+            } else {
+#pragma unroll
+                for (int a = 0; a < 8; a++)
+                    if (a < nb) r0 -= lr[a] * y[a];
+            }
+        }
+    }
+    big_bwd<1>(lcore, R, L, r0, r1, par);
+    if (own) L.w0[tid] = r0;
+}
+
 // Forward phase of one periphery level: the gathers of the level's factor
 // entries (FACTOR; diagonal entries take the square root) and of its
 // columns' forward substitutions, b_j - sum L(j,k) y_k into yx[j] (divided
@@ -632,11 +973,12 @@ CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
     if (R > 0) {
         // column broadcast buffers of 4T + 2 entries: rows [0, R), zeros for the
         // padded rows [R, 4T), the two right-hand sides at 4T and 4T + 1
+        const bool big = R > SP_SMALL_MAX;   // the panel core (uniform)
         const int RS = 4 * T;
         double* cb0 = sp_lds;
         double* cb1 = sp_lds + (RS + 2);
         double* piv = sp_lds + 2 * (RS + 2);
-        double* w0 = piv + R;
+        double* w0 = big ? big_lds(sp_lds, R).w0 : piv + R;
         double* w1 = w0 + R;
         // the Schur complement into lcore (scratch here), short lists by
         // threads, long ones by waves; the right-hand sides into w0, w1
@@ -677,110 +1019,114 @@ CG_DEV void sp_newton_body(const CgInstD& I, const V& v, double* sp_lds)
         }
         core_rhs<2>(v, I.grad, I.yv, yx, w0, w1);
         __syncthreads();
-        CoreTiles ct;
-        core_tiles_init(ct, T);
-#pragma unroll
-        for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
-            if (!ct.own[s]) continue;
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int bb = 0; bb < 4; bb++) {
-                    const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
-                    if (i < R && j <= i) ct.c[s][a][bb] = v.lcore[tri(i) + j];
-                }
-        }
-        double r0 = 0.0, r1 = 0.0;   // this thread's core row tid of the two right-hand sides
-        if (tid < R) {
-            r0 = w0[tid];
-            r1 = w1[tid];
-        }
-        // column 0 of C (all rows of the tiles of tile column 0) and of the right-hand sides
-#pragma unroll
-        for (int s = 0; s < SP_TILES_PER_THREAD; s++)
-            if (ct.own[s] && ct.tj[s] == 0)
-#pragma unroll
-                for (int a = 0; a < 4; a++) cb0[4 * ct.ti[s] + a] = ct.c[s][a][0];
-        if (tid == 0) {
-            cb0[RS] = r0;
-            cb0[RS + 1] = r1;
-        }
-        __syncthreads();
-        for (int k = 0; k < R; k++) {
-            const double* cb = (k & 1) ? cb1 : cb0;
-            double* nb = (k & 1) ? cb0 : cb1;
-            const double dk = cb[k];
-            ok = ok && (dk > 0.0);
-            const double inv = 1.0 / dk;
-            if (tid == 0) piv[k] = dk;
-            // tiles with a column above k (the others are final); rows and
-            // columns <= k and the padded rows take zero multipliers, entries
-            // above the diagonal of a diagonal tile are never read
+        if (big) {
+            ok = core_big_factor(v.lcore, R, sp_lds) && ok;
+        } else {
+            CoreTiles ct;
+            core_tiles_init(ct, T);
 #pragma unroll
             for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
-                if (!ct.own[s] || 4 * ct.tj[s] + 3 <= k) continue;
-                double ci[4], cj[4];
-#pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    const int i = 4 * ct.ti[s] + a;
-                    ci[a] = (i > k) ? cb[i] : 0.0;
-                }
-#pragma unroll
-                for (int bb = 0; bb < 4; bb++) {
-                    const int j = 4 * ct.tj[s] + bb;
-                    cj[bb] = (j > k) ? cb[j] * inv : 0.0;
-                }
+                if (!ct.own[s]) continue;
 #pragma unroll
                 for (int a = 0; a < 4; a++)
 #pragma unroll
-                    for (int bb = 0; bb < 4; bb++) ct.c[s][a][bb] -= ci[a] * cj[bb];
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
+                        if (i < R && j <= i) ct.c[s][a][bb] = v.lcore[tri(i) + j];
+                    }
             }
-            if (tid > k && tid < R) {
-                const double l = cb[tid] * inv;
-                r0 -= cb[RS] * l;
-                r1 -= cb[RS + 1] * l;
+            double r0 = 0.0, r1 = 0.0;   // this thread's core row tid of the two right-hand sides
+            if (tid < R) {
+                r0 = w0[tid];
+                r1 = w1[tid];
             }
-            // column k + 1 (after this thread's own updates of it); rows above
-            // k + 1 of its tile column are written too and never read
-            const int kk = k + 1;
-            if (kk < R) {
+            // column 0 of C (all rows of the tiles of tile column 0) and of the right-hand sides
 #pragma unroll
-                for (int s = 0; s < SP_TILES_PER_THREAD; s++)
-                    if (ct.own[s] && ct.tj[s] == (kk >> 2))
+            for (int s = 0; s < SP_TILES_PER_THREAD; s++)
+                if (ct.own[s] && ct.tj[s] == 0)
 #pragma unroll
-                        for (int a = 0; a < 4; a++) nb[4 * ct.ti[s] + a] = tile_col(ct.c[s], a, kk & 3);
-                if (tid == kk) {
-                    nb[RS] = r0;
-                    nb[RS + 1] = r1;
-                }
+                    for (int a = 0; a < 4; a++) cb0[4 * ct.ti[s] + a] = ct.c[s][a][0];
+            if (tid == 0) {
+                cb0[RS] = r0;
+                cb0[RS + 1] = r1;
             }
             __syncthreads();
-        }
-        // scaled factor L = C D^{-1/2} (kept for the backward's preconditioner),
-        // forward result y = D^{-1/2} r, then L^T x = y
+            for (int k = 0; k < R; k++) {
+                const double* cb = (k & 1) ? cb1 : cb0;
+                double* nb = (k & 1) ? cb0 : cb1;
+                const double dk = cb[k];
+                ok = ok && (dk > 0.0);
+                const double inv = 1.0 / dk;
+                if (tid == 0) piv[k] = dk;
+                // tiles with a column above k (the others are final); rows and
+                // columns <= k and the padded rows take zero multipliers, entries
+                // above the diagonal of a diagonal tile are never read
 #pragma unroll
-        for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
-            if (!ct.own[s]) continue;
+                for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
+                    if (!ct.own[s] || 4 * ct.tj[s] + 3 <= k) continue;
+                    double ci[4], cj[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+                    for (int a = 0; a < 4; a++) {
+                        const int i = 4 * ct.ti[s] + a;
+                        ci[a] = (i > k) ? cb[i] : 0.0;
+                    }
 #pragma unroll
-                for (int bb = 0; bb < 4; bb++) {
-                    const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
-                    if (i < R && j <= i) {
-                        const double sq = sqrt(piv[j]);
-                        const double x = (i == j) ? sq : ct.c[s][a][bb] / sq;
-                        ct.c[s][a][bb] = x;
-                        v.lcore[tri(i) + j] = x;
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int j = 4 * ct.tj[s] + bb;
+                        cj[bb] = (j > k) ? cb[j] * inv : 0.0;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int bb = 0; bb < 4; bb++) ct.c[s][a][bb] -= ci[a] * cj[bb];
+                }
+                if (tid > k && tid < R) {
+                    const double l = cb[tid] * inv;
+                    r0 -= cb[RS] * l;
+                    r1 -= cb[RS + 1] * l;
+                }
+                // column k + 1 (after this thread's own updates of it); rows above
+                // k + 1 of its tile column are written too and never read
+                const int kk = k + 1;
+                if (kk < R) {
+#pragma unroll
+                    for (int s = 0; s < SP_TILES_PER_THREAD; s++)
+                        if (ct.own[s] && ct.tj[s] == (kk >> 2))
+#pragma unroll
+                            for (int a = 0; a < 4; a++) nb[4 * ct.ti[s] + a] = tile_col(ct.c[s], a, kk & 3);
+                    if (tid == kk) {
+                        nb[RS] = r0;
+                        nb[RS + 1] = r1;
                     }
                 }
+                __syncthreads();
+            }
+            // scaled factor L = C D^{-1/2} (kept for the backward's preconditioner),
+            // forward result y = D^{-1/2} r, then L^T x = y
+#pragma unroll
+            for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
+                if (!ct.own[s]) continue;
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
+                        if (i < R && j <= i) {
+                            const double sq = sqrt(piv[j]);
+                            const double x = (i == j) ? sq : ct.c[s][a][bb] / sq;
+                            ct.c[s][a][bb] = x;
+                            v.lcore[tri(i) + j] = x;
+                        }
+                    }
+            }
+            if (tid < R) {
+                const double sq = sqrt(piv[tid]);
+                w0[tid] = r0 / sq;
+                w1[tid] = r1 / sq;
+            }
+            __syncthreads();
+            core_bwd_tiles<2>(ct, R, T, w0, w1);
         }
-        if (tid < R) {
-            const double sq = sqrt(piv[tid]);
-            w0[tid] = r0 / sq;
-            w1[tid] = r1 / sq;
-        }
-        __syncthreads();
-        core_bwd_tiles<2>(ct, R, T, w0, w1);
         if (tid < R) {
             yx[t + tid] = w0[tid];
             yx[N + t + tid] = w1[tid];
@@ -951,24 +1297,30 @@ __global__ __launch_bounds__(SP_WG) void k_sp_precond(const CgInst* __restrict__
         __syncthreads();
     }
     if (R > 0) {
-        double* w0 = sp_lds;
+        const bool big = R > SP_SMALL_MAX;   // the panel core (uniform)
+        double* w0 = big ? big_lds(sp_lds, R).w0 : sp_lds;
         core_rhs<1>(v, I.yv, nullptr, yx, w0, nullptr);
-        CoreTiles ct;
-        core_tiles_init(ct, T);
+        if (big) {
+            __syncthreads();
+            core_big_solve(v.lcore, R, sp_lds);
+        } else {
+            CoreTiles ct;
+            core_tiles_init(ct, T);
 #pragma unroll
-        for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
-            if (!ct.own[s]) continue;
+            for (int s = 0; s < SP_TILES_PER_THREAD; s++) {
+                if (!ct.own[s]) continue;
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+                for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int bb = 0; bb < 4; bb++) {
-                    const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
-                    if (i < R && j <= i) ct.c[s][a][bb] = v.lcore[tri(i) + j];
-                }
+                    for (int bb = 0; bb < 4; bb++) {
+                        const int i = 4 * ct.ti[s] + a, j = 4 * ct.tj[s] + bb;
+                        if (i < R && j <= i) ct.c[s][a][bb] = v.lcore[tri(i) + j];
+                    }
+            }
+            __syncthreads();
+            core_fwd_tiles<1>(ct, R, T, w0, nullptr);
+            core_bwd_tiles<1>(ct, R, T, w0, nullptr);
         }
-        __syncthreads();
-        core_fwd_tiles<1>(ct, R, T, w0, nullptr);
-        core_bwd_tiles<1>(ct, R, T, w0, nullptr);
         if (tid < R) yx[t + tid] = w0[tid];
         __syncthreads();
     }

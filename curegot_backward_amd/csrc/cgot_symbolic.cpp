@@ -31,7 +31,7 @@ bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_by
 {
     const int N = n + m - 1;
     if (N <= 0) return false;
-    rmax = std::min(rmax, SP_RMAX);
+    rmax = std::min(rmax, SP_RMAX_BIG);
     // ---- bipartite graph of W: alpha_i -- beta_j for every selected (i, j) ----
     std::vector<std::vector<int>> adj(N);
     for (int i = 0; i < n; i++)

@@ -82,6 +82,16 @@ struct SpHdr {
 #define SP_WG 1024
 #define SP_TILES_PER_THREAD 2
 #define SP_RMAX 252
+// Cores of SP_SMALL_MAX < R <= SP_RMAX_BIG (<= SP_WG: one core row per
+// thread) are factored in place in the plan's packed core storage instead:
+// right-looking Cholesky by panels of 8 columns (the 8 x 8 diagonal block on
+// one wave, the panel's other rows by their threads, the trailing triangle by
+// all waves), the right-hand sides eliminated with the panels, then the
+// backward substitution by blocks of 8 rows.
+#ifndef SP_SMALL_MAX
+#define SP_SMALL_MAX SP_RMAX
+#endif
+#define SP_RMAX_BIG 768
 // Pair lists longer than these are summed by eight lanes (fixed-order
 // reduction) instead of one thread: SP_LONG for the periphery entries and the
 // right-hand-side rows (few items per level, so latency decides), SP_CLONG
@@ -99,13 +109,15 @@ struct SpHdr {
 // Latency estimates (microseconds per Newton solve of one instance), fitted
 // to measured MI355X timings, that decide whether an instance takes its sparse
 // plan.  Sparse: a fixed part, one barrier pair per periphery level, the dense
-// core's columns (linear and quadratic parts) and the prefetch of the plan
-// (about n + m).  Dense Schur chain: a fixed part (Schur build, right-hand
+// core (its Schur gather and factorization: linear and quadratic parts, one
+// fit per core kind) and the prefetch of the plan (about n + m).  Dense Schur chain: a fixed part (Schur build, right-hand
 // sides, triangular solves, back-substitution) and one dependent update
 // launch per CG_NB-block of the order P.
 inline double sp_cost_us(const SpHdr& h)
 {
-    return 10.7 + 6.44 * h.H + 0.34 * h.R + 8.6e-3 * (double)h.R * h.R + 20.4e-3 * (double)(h.n + h.m);
+    const double R = h.R;
+    const double core = (h.R <= SP_SMALL_MAX) ? 0.34 * R + 8.6e-3 * R * R : 0.34 * R + 7.0e-3 * R * R;
+    return 10.7 + 6.44 * h.H + core + 20.4e-3 * (double)(h.n + h.m);
 }
 inline double sp_dense_cost_us(int nblk) { return 150.0 + 32.0 * nblk; }
 
@@ -126,18 +138,20 @@ inline double sp_solve_flops(const SpHdr& h)
 // Host analysis.  Returns true and fills `plan` (the integer part of the
 // plan image, hdr.o_meta_end bytes, offsets relative to its start; the value
 // arrays up to hdr.bytes follow on the device) when a sparse plan fits in
-// `cap_bytes` with a core of at most `rmax` (<= SP_RMAX); false otherwise
+// `cap_bytes` with a core of at most `rmax` (<= SP_RMAX_BIG); false otherwise
 // (then the instance keeps the dense Schur path).  rowptr/colj: W's pattern
 // by row (columns j < m - 1).
 bool sp_analyze(int n, int m, const int* rowptr, const int* colj, int64_t cap_bytes, int rmax, SpHdr& hdr,
                 std::vector<char>& plan);
 
-// LDS bytes the factor/solve kernel needs for a core of R: two column
-// broadcast buffers (4 ceil(R/4) + 2 <= R + 5 each), the pivots and two
-// right-hand sides.
-inline size_t sp_lds_bytes(int R) { return sizeof(double) * (5 * (size_t)R + 16); }
-// doubles of the core's LDS buffers (the values of an LDS-resident launch follow them)
-#define SP_LDS_CORE(R) (5 * (R) + 16)
+// Doubles of the core's LDS buffers (the values of an LDS-resident launch
+// follow them).  Register-tile core: two column broadcast buffers
+// (4 ceil(R/4) + 2 <= R + 5 each), the pivots and two right-hand sides.
+// Panel core: the panel's rows (8 R), two right-hand sides, the diagonal
+// block (64), its right-hand sides (16) and two block buffers (2 x 80).
+#define SP_LDS_CORE(R) ((R) <= SP_SMALL_MAX ? 5 * (R) + 16 : 10 * (R) + 240)
+// LDS bytes the factor/solve kernel needs for a core of R.
+inline size_t sp_lds_bytes(int R) { return sizeof(double) * (size_t)SP_LDS_CORE(R); }
 // LDS of a launch that keeps the plan's values in LDS: the core's buffers, the
 // nnzP periphery factor entries and the two work vectors of N
 inline size_t sp_lds_vals_bytes(const SpHdr& h)

@@ -216,6 +216,52 @@ def test_newton_solve_vs_dense(cg, newton_path, n, m, K):
     assert rel(r["sol"], x) < 1e-9
 
 
+def _hook_core_size(n, m, K, seed):
+    """Core size of the sparse plan the library builds for the Newton hook's
+    top-K pattern (the oracle's selection, the library's host analysis with
+    the Schur buffer's capacity)."""
+    from tests import _sparse_plan as spp
+    rng = np.random.default_rng(seed)
+    M = rng.random((n, m))
+    alpha = rng.normal(scale=0.01, size=n)
+    beta = rng.normal(scale=0.01, size=m)
+    si, sj = oracle.topk_pattern(np.concatenate([alpha, beta]), M, 0.05, K)
+    sel = sorted((i, j) for i, j in zip(si, sj) if j < m - 1)
+    rp = np.zeros(n + 1, np.int32)
+    for i, _ in sel:
+        rp[i + 1] += 1
+    rp = np.cumsum(rp).astype(np.int32)
+    cj = np.array([j for _, j in sel], np.int32)
+    P = min(n, m - 1)
+    h, _ = spp.make_plan(n, m, rp, cj, cap=8 * P * P, rmax=768)
+    return None if h is None else h.R
+
+
+@pytest.mark.parametrize("n,m,K", [(1200, 900, 3400), (2000, 700, 4200)])
+def test_newton_solve_panel_core(cg, n, m, K):
+    # a sparse plan whose dense core exceeds the register-tile core's 252
+    # vertices: the core is factored in place by panels; vs numpy.linalg.solve
+    R = _hook_core_size(n, m, K, n + m)
+    print(f"newton panel core n={n} m={m} K={K} R={R}", flush=True)
+    assert R is not None and 252 < R <= 768
+    from curegot_backward_amd import _lib
+    prev = _lib.set_sparse_solve(True)
+    try:
+        rng = np.random.default_rng(n + m)
+        M = rng.random((n, m))
+        alpha = rng.normal(scale=0.01, size=n)
+        beta = rng.normal(scale=0.01, size=m)
+        a = np.full(n, 1 / n); b = np.full(m, 1 / m)
+        rhs = rng.normal(size=n + m - 1)
+        r = cg.tests.test_T_computation_sparsify(alpha, beta, torch.tensor(M), a, b, 0.05, 1e-4, K, rhs=rhs)
+    finally:
+        _lib.set_sparse_solve(prev)
+    A = _dense.csr_dense_sym(r["csr_val"].cpu().numpy(), r["csr_colind"].cpu().numpy(),
+                             r["csr_rowptr"].cpu().numpy(), n + m - 1)
+    x = np.linalg.solve(A, rhs)
+    assert rel(r["sol"], x) < 1e-9
+
+
 @pytest.mark.parametrize("n,m,which", [(100, 700, 0), (700, 100, 1), (33, 1031, 1), (1031, 33, 0)])
 def test_lse_update_vs_oracle(cg, n, m, which):
     from curegot_backward_amd import _native

@@ -98,3 +98,28 @@ def test_large_dense_core_keeps_dense_path():
     # a plan larger than the buffer it must fit in keeps the dense path too
     h, _ = spp.make_plan(80, 80, rp, cj, cap=1024)
     assert h is None
+
+
+def test_plan_with_panel_core():
+    """A top-K pattern (the oracle's selection on an n >> m problem) whose
+    dense core exceeds the register-tile core's 252 vertices: the analysis
+    keeps it as a core of up to 768 (factored by panels on the device) and
+    the plan's schedule solves the system."""
+    n, m, K = 1200, 900, 3400
+    rng = np.random.default_rng(n + m)
+    M = rng.random((n, m))
+    gamma = np.concatenate([rng.normal(scale=0.01, size=n), rng.normal(scale=0.01, size=m)])
+    si, sj = oracle.topk_pattern(gamma, M, 0.05, K)
+    sel = sorted((i, j) for i, j in zip(si, sj) if j < m - 1)
+    rp = np.zeros(n + 1, np.int32)
+    for i, _ in sel:
+        rp[i + 1] += 1
+    rp = np.cumsum(rp).astype(np.int32)
+    cj = np.array([j for _, j in sel], np.int32)
+    wv = rng.uniform(0.1, 1.0, size=len(cj))
+    Da = np.array([wv[rp[i]:rp[i + 1]].sum() for i in range(n)]) + 0.5
+    Db = np.zeros(m - 1)
+    np.add.at(Db, cj, wv)
+    assert spp.make_plan(n, m, rp, cj, rmax=252)[0] is None
+    h = check(n, m, rp, cj, wv, Da, Db + 0.5, rmax=768)
+    assert 252 < h.R <= 768
