@@ -96,7 +96,7 @@ CG_DEV void cg_tiles2_to_lds(double (*Ta)[CG_NB + 1], const CG_AS1 double* a, si
 // FP64 matrix cores: wave w owns rows 32*(w/2).. and columns 32*(w%2).., as
 // 2 x 2 blocks of v_mfma_f64_16x16x4f64.  Operand layout of that instruction:
 // lane l supplies A[l%16][l/16] and B[l/16][l%16] of the 16x4 and 4x16 blocks;
-// the result holds D[4*(l/16) + r][l%16] in register r.
+// the result holds D[4*r + l/16][l%16] in register r (cg_mfma_row below).
 CG_DEV void cg_mfma_abt64(double (*A)[CG_NB + 1], double (*Bm)[CG_NB + 1], cg_d4 (&acc)[2][2])
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;

@@ -437,67 +437,62 @@ CG_DEV void big_bwd(const CG_AS1 double* lcore, int R, const BigLds& L, double& 
     }
 }
 
-// Trailing update of rows ib .. ib + nr - 1 (nr <= NRB) of the panel core:
-// C(i, j) -= sum_c L(i, c) L(j, c) for q <= j <= i, one wave, lanes over j,
-// NCH chunks of 64 columns per step.  The rows' L entries are held in
-// registers (LDS broadcast reads); the next step's C entries are loaded
-// before the current step is computed and stored, so that loads are in
-// flight during the arithmetic.
-#ifndef BIG_NRB
-#define BIG_NRB 2   // rows per wave step of the trailing update
-#endif
-#ifndef BIG_NCH
-#define BIG_NCH 2   // 64-column chunks per wave step
-#endif
-template <int NRB, int NCH>
-CG_DEV void big_trailing_rows(CG_AS1 double* lcore, const double* pan, int R, int q, int ib, int nr, int lane)
+// Trailing update of the panel core's rows [q, R) on FP64 matrix cores:
+// C(i, j) -= sum_c L(i, c) L(j, c) for q <= j <= i, by 16 x 16 tiles of the
+// lower triangle (tile (ti, tj), tj <= ti, rows q + 16 ti.., columns
+// q + 16 tj..), tile u = w + 16 k to wave w, two tiles per step.  A tile is
+// two v_mfma_f64_16x16x4f64 (the panel's 8 columns as two k-steps of 4) with
+// the tile's C entries as the accumulator and the row operand negated.
+// Operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the
+// result holds D[4 r + l/16][l%16] in register r.  The panel's columns come
+// from LDS (column-major: 16 lanes read consecutive words); the C entries of
+// a row segment are 16 consecutive doubles of lcore.  Entries past R or
+// above the diagonal are neither loaded nor stored.
+typedef double sp_d4 __attribute__((ext_vector_type(4)));
+
+CG_DEV void big_trailing_mfma(CG_AS1 double* lcore, const double* pan, int R, int q, int wave, int lane)
 {
-    double li[NRB][8];
-    int rbase[NRB];
+    const int nt = (R - q + 15) / 16;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int u0 = wave; u0 < ntiles; u0 += 2 * (SP_WG / 64)) {
+        int ti[2], tj[2];
+        bool on[2];
+        sp_d4 acc[2];
+        double av[2][2], bv[2][2];
 #pragma unroll
-    for (int r = 0; r < NRB; r++) {
-        const int i = ib + min(r, nr - 1);   // rows past nr repeat the last (never stored)
-        rbase[r] = tri(i);
+        for (int t = 0; t < 2; t++) {
+            const int u = u0 + t * (SP_WG / 64);
+            on[t] = u < ntiles;   // uniform
+            ti[t] = tj[t] = 0;
+            if (on[t]) tri_decode(u, ti[t], tj[t]);
+            const int i0 = q + 16 * ti[t], j0 = q + 16 * tj[t];
+            const int jc = j0 + li;   // this lane's column
 #pragma unroll
-        for (int c = 0; c < 8; c++) li[r][c] = pan[c * R + i];
-    }
-    const int jend = ib + nr - 1;   // the last row's diagonal
-    auto live = [&](int r, int j) { return r < nr && j <= ib + r; };
-    auto load = [&](int j0, double (&v)[NCH][NRB]) {
-#pragma unroll
-        for (int h = 0; h < NCH; h++)
-#pragma unroll
-            for (int r = 0; r < NRB; r++) {
-                const int j = j0 + 64 * h;
-                v[h][r] = live(r, j) ? lcore[rbase[r] + j] : 0.0;
+            for (int r = 0; r < 4; r++) {
+                const int ir = i0 + 4 * r + lk;   // this lane's row of register r
+                acc[t][r] = (on[t] && ir < R && jc <= ir) ? lcore[tri(ir) + jc] : 0.0;
             }
-    };
-    double cur[NCH][NRB];
-    int j0 = q + lane;
-    load(j0, cur);
-    for (; j0 - lane <= jend; j0 += 64 * NCH) {
-        double nxt[NCH][NRB];
-        load(j0 + 64 * NCH, nxt);
+            const int ia = i0 + li, jb = j0 + li;
 #pragma unroll
-        for (int h = 0; h < NCH; h++) {
-            const int j = j0 + 64 * h;
-            if (j > jend) continue;
-            double lj[8];
-#pragma unroll
-            for (int c = 0; c < 8; c++) lj[c] = pan[c * R + j];
-#pragma unroll
-            for (int r = 0; r < NRB; r++) {
-                if (!live(r, j)) continue;
-                double sv = cur[h][r];
-#pragma unroll
-                for (int c = 0; c < 8; c++) sv -= li[r][c] * lj[c];
-                lcore[rbase[r] + j] = sv;
+            for (int sk = 0; sk < 2; sk++) {
+                const int c = 4 * sk + lk;
+                av[t][sk] = (on[t] && ia < R) ? -pan[c * R + ia] : 0.0;
+                bv[t][sk] = (on[t] && jb < R) ? pan[c * R + jb] : 0.0;
             }
         }
 #pragma unroll
-        for (int h = 0; h < NCH; h++)
+        for (int t = 0; t < 2; t++) {
+            if (!on[t]) continue;
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][0], bv[t][0], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][1], bv[t][1], acc[t], 0, 0, 0);
+            const int i0 = q + 16 * ti[t], jc = q + 16 * tj[t] + li;
 #pragma unroll
-            for (int r = 0; r < NRB; r++) cur[h][r] = nxt[h][r];
+            for (int r = 0; r < 4; r++) {
+                const int ir = i0 + 4 * r + lk;
+                if (ir < R && jc <= ir) lcore[tri(ir) + jc] = acc[t][r];
+            }
+        }
     }
 }
 
@@ -609,8 +604,7 @@ __device__ __attribute__((noinline)) int core_big_factor(CG_AS1 double* lcore, i
         __syncthreads();
         // rows below the panel exist only when nb == 8
         const int q = p0 + nb;
-        for (int ib = q + BIG_NRB * wave; ib < R; ib += BIG_NRB * (SP_WG / 64))
-            big_trailing_rows<BIG_NRB, BIG_NCH>(lcore, L.pan, R, q, ib, min(BIG_NRB, R - ib), lane);
+        big_trailing_mfma(lcore, L.pan, R, q, wave, lane);
         __syncthreads();
     }
     int par = 0;
