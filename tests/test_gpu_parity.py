@@ -395,6 +395,62 @@ def test_splr_preconditioner_apply_vs_dense(cg, n, m, eta, seed):
     assert rel(z, zref) < 1e-14 * np.linalg.cond(B) + 1e-12
 
 
+@pytest.mark.parametrize("req_idx,R_min", [(53, 253), (1448, 253)])
+def test_splr_preconditioner_with_panel_core(cg, req_idx, R_min):
+    """Serving-stream instances with n >> m whose last Newton system (sparse
+    factorization forced) has a dense core above the register-tile core's 252
+    vertices: the forward factors it by panels and the backward's "splr"
+    preconditioner solves with that stored factor.  z = B^{-1} r is checked by
+    its backward error against the context's A (sparse) and rank-2 term, and
+    the adjoint gradient with the "splr" preconditioner against the one with
+    "diag" (the same system, both to PCG tolerance)."""
+    import scipy.sparse as sps
+    from curegot_backward_amd import _lib
+    from curegot_backward_amd.workloads import serving_trace, serving_problem
+    from tests import _sparse_plan as spp
+    req = serving_trace()[req_idx]
+    M, a, b, eta = serving_problem(req, device="cuda")
+    n, m = M.shape
+    N = n + m - 1
+    print(f"case request {req_idx} n={n} m={m} eta={eta}", flush=True)
+    prev = _lib.set_sparse_solve(True)
+    try:
+        out = cg.sinkhorn_splr_bwd(M, a.cuda(), b.cuda(), eta, 1e-8, 500)
+        assert out["converged"]
+        ctx = out["context"]
+        vals, cols, rows = (ctx.csr_values().cpu().numpy(), ctx.csr_colind().cpu().numpy(),
+                            ctx.csr_rowptr().cpu().numpy())
+        rp, cj, _, _, _ = spp.w_pattern(vals, cols, rows, n, m)
+        P = min(n, m - 1)
+        h, _ = spp.make_plan(n, m, rp, cj, cap=8 * P * P, rmax=768)
+        print(f"core R={None if h is None else h.R}", flush=True)
+        assert h is not None and R_min <= h.R <= 768
+        r = np.random.default_rng(req_idx).normal(size=N)
+        z = ctx.apply_preconditioner(torch.from_numpy(r).cuda()).cpu().numpy()
+        L = sps.csr_matrix((vals, cols, rows), shape=(N, N))
+        A = L + sps.tril(L, -1).T
+        Bz = A @ z
+        if ctx.low_rank_enabled:
+            s, y, ys = ctx.secant_s().cpu().numpy(), ctx.secant_y().cpu().numpy(), ctx.ys
+            As = A @ s
+            Bz += -As * (As @ z) / (s @ As) + (eta / ys) * y * (y @ z)
+        # normwise backward error with A's norm alone (no larger than B's
+        # bound, so the check is at least as strict)
+        anorm = np.asarray(abs(A).sum(axis=1)).ravel().max()
+        berr = np.abs(Bz - r).max() / (anorm * np.abs(z).max() + np.abs(r).max())
+        rres = np.abs(Bz - r).max() / np.abs(r).max()
+        print("backward error", berr, "relative residual", rres, flush=True)
+        assert berr < 1e-13, berr
+        assert rres < 1e-12, rres
+        G = torch.from_numpy(np.random.default_rng(req_idx + 1).normal(size=(n, m))).cuda()
+        gs = cg.solve_backward(out["plan"], G, ctx, eta, preconditioner="splr")
+        gd = cg.solve_backward(out["plan"], G, ctx, eta, preconditioner="diag")
+        print("pcg iterations splr", gs["iterations"], "diag", gd["iterations"], flush=True)
+        assert rel(gs["grad"].cpu().numpy(), gd["grad"].cpu().numpy()) < 1e-8
+    finally:
+        _lib.set_sparse_solve(prev)
+
+
 def test_bcd_convergence_mechanism(cg):
     """The reference's BCD convergence checks (test/test_sinkhorn_bcd.py:123-237):
     niter within max_iter for every tolerance, max_iter = 10 truncates at 10,
