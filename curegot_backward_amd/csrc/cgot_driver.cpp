@@ -1137,8 +1137,10 @@ struct SplrSolver {
         for (int b = 0; b < B; b++) anydense = anydense || on[b];
         // the plans' values in LDS when every listed plan's fit, and the
         // integer arrays of each plan that fits with them (the same
-        // arithmetic: only where they live)
-        const int lv = (!ids.empty() && vals_max <= (size_t)g_lds_max) ? 1 : 0;
+        // arithmetic: only where they live).  Not when a plan has a panel
+        // core (R > SP_SMALL_MAX): that combination has faulted, cause not
+        // found, so such launches keep every plan in global memory.
+        const int lv = (!ids.empty() && vals_max <= (size_t)g_lds_max && rmax_used <= SP_SMALL_MAX) ? 1 : 0;
         const size_t sp_lds = lv ? std::max(vals_max, all_max <= (size_t)g_lds_max ? all_max : 0) : sp_lds_bytes(rmax_used);
         // both kinds in the group: the sparse blocks share one launch with the
         // Schur rows (k_newton_mixed) instead of running before the dense
